@@ -1,0 +1,215 @@
+"""ctypes binding of ``libqmcrt.so`` (the C ABI in ``include/qmcrt.h``).
+
+The library is loaded from ``solarenergy_chier_amd/lib``; there is no fallback: when it is
+missing or a symbol is absent, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libqmcrt.so"
+
+QMC_OK = 0
+QMC_E_INVALID_ARG = -1
+QMC_E_RANGE = -2
+QMC_E_HIP = -3
+QMC_E_OOM = -4
+QMC_E_UNSUPPORTED = -5
+QMC_E_IO = -6
+QMC_E_PARSE = -7
+QMC_E_STATE = -8
+
+RAY_SHADOWED, RAY_BLOCKED, RAY_MISSED, RAY_DEPOSITED = 0, 1, 2, 3
+RAYF_EDGE_CLAMPED = 1
+
+
+class Config(C.Structure):
+    _fields_ = [("sun_dir", C.c_float * 3), ("dni", C.c_float), ("csr", C.c_float),
+                ("num_sunshape_groups", C.c_int32), ("num_sunshape_lights_per_group", C.c_int32),
+                ("inverse_transform_sampling_groups", C.c_int32), ("receiver_pixel_length", C.c_float),
+                ("helio_disturb_std", C.c_float), ("helio_pixel_length", C.c_float),
+                ("helio_reflected_rate", C.c_float)]
+
+
+class ReceiverDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pos", C.c_float * 3), ("size", C.c_float * 3), ("normal", C.c_float * 3),
+                ("face", C.c_int32), ("res_x", C.c_int32), ("res_y", C.c_int32), ("tilt_deg", C.c_float)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("pos", C.c_float * 3), ("size", C.c_float * 3), ("interval", C.c_float * 3),
+                ("start_heliostat", C.c_int32), ("n_heliostats", C.c_int32), ("heliostat_type", C.c_int32),
+                ("receiver", C.c_int32)]
+
+
+class HeliostatDesc(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("size", C.c_float * 3), ("rows", C.c_int32), ("cols", C.c_int32),
+                ("gap", C.c_float * 2), ("surface_property", C.c_float * 6), ("grid", C.c_int32),
+                ("sub_count_override", C.c_int32), ("local_centers", C.POINTER(C.c_float)),
+                ("local_normals", C.POINTER(C.c_float))]
+
+
+class SceneCounts(C.Structure):
+    _fields_ = [("n_receivers", C.c_int32), ("n_grids", C.c_int32), ("n_heliostats", C.c_int32),
+                ("grid_num_declared", C.c_int32), ("ground_length", C.c_float), ("ground_width", C.c_float)]
+
+
+class ReceiverInfo(C.Structure):
+    _fields_ = [("type", C.c_int32), ("res_x", C.c_int32), ("res_y", C.c_int32), ("pixel_length", C.c_float),
+                ("factor_area", C.c_float), ("vertices", (C.c_float * 3) * 4), ("normal", C.c_float * 3),
+                ("focus_center", C.c_float * 3)]
+
+
+class HeliostatInfo(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("size", C.c_float * 3), ("normal", C.c_float * 3), ("aim", C.c_float * 3),
+                ("vertices", (C.c_float * 3) * 4), ("focal_length", C.c_float), ("type", C.c_int32),
+                ("grid", C.c_int32), ("n_micro", C.c_int32), ("sub_rows", C.c_int32), ("sub_cols", C.c_int32),
+                ("sub_begin", C.c_int32), ("sub_count", C.c_int32)]
+
+
+class GridInfo(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("n_match", C.c_int32),
+                ("n_sub", C.c_int32)]
+
+
+class RayRecord(C.Structure):
+    _fields_ = [("status", C.c_int32), ("index", C.c_int32), ("energy", C.c_float), ("flags", C.c_int32)]
+
+
+class TraceStats(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("unshadowed", C.c_uint64), ("unblocked", C.c_uint64),
+                ("deposits", C.c_uint64), ("cells", C.c_uint64), ("candidates", C.c_uint64),
+                ("tests", C.c_uint64), ("facets", C.c_uint64), ("edge_rays", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
+
+
+class PoolParams(C.Structure):
+    _fields_ = [("seed_sunshape", C.c_uint64), ("seed_perturbation", C.c_uint64)]
+
+
+RECORD_DTYPE = [("status", "<i4"), ("index", "<i4"), ("energy", "<f4"), ("flags", "<i4")]
+
+_P = C.c_void_p
+_I = C.c_int32
+_F = C.c_float
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int32)
+
+# name -> (argtypes); every function returns qmc_status (int) unless listed in _RESTYPES
+SIGNATURES = {
+    "qmc_last_error": [],
+    "qmc_status_string": [C.c_int],
+    "qmc_scene_create": [C.POINTER(_P)],
+    "qmc_scene_destroy": [_P],
+    "qmc_scene_load_config": [_P, C.c_char_p, _IP],
+    "qmc_scene_set_config": [_P, C.POINTER(Config)],
+    "qmc_scene_get_config": [_P, C.POINTER(Config)],
+    "qmc_config_default": [C.POINTER(Config)],
+    "qmc_scene_load_file": [_P, C.c_char_p],
+    "qmc_scene_load_text": [_P, C.c_char_p],
+    "qmc_scene_add_receiver": [_P, C.POINTER(ReceiverDesc)],
+    "qmc_scene_add_grid": [_P, C.POINTER(GridDesc)],
+    "qmc_scene_add_heliostat": [_P, C.POINTER(HeliostatDesc)],
+    "qmc_scene_process": [_P],
+    "qmc_scene_get_counts": [_P, C.POINTER(SceneCounts)],
+    "qmc_scene_get_receiver_desc": [_P, _I, C.POINTER(ReceiverDesc)],
+    "qmc_scene_get_grid_desc": [_P, _I, C.POINTER(GridDesc)],
+    "qmc_scene_get_heliostat_desc": [_P, _I, C.POINTER(HeliostatDesc)],
+    "qmc_scene_get_receiver_info": [_P, _I, C.POINTER(ReceiverInfo)],
+    "qmc_scene_get_heliostat_info": [_P, _I, C.POINTER(HeliostatInfo)],
+    "qmc_scene_get_grid_info": [_P, _I, C.POINTER(GridInfo)],
+    "qmc_scene_get_grid_cells": [_P, _I, _IP, _IP],
+    "qmc_scene_get_subheliostat_vertices": [_P, _I, _FP, _I, _IP],
+    "qmc_scene_get_factor": [_P, _I, _FP],
+    "qmc_scene_check_heliostat": [_P, _I],
+    "qmc_task_load": [_P, C.c_char_p, _IP, _I, _IP, _IP],
+    "qmc_create": [_I, C.POINTER(_P)],
+    "qmc_destroy": [_P],
+    "qmc_get_stream": [_P, C.POINTER(_P)],
+    "qmc_sync": [_P],
+    "qmc_upload": [_P, _P],
+    "qmc_pool_generate": [_P, C.POINTER(PoolParams)],
+    "qmc_pool_set": [_P, _FP, _FP, _I, _I],
+    "qmc_pool_get": [_P, _FP, _FP, _I],
+    "qmc_sunshape_table": [_P, _FP, _FP, _FP],
+    "qmc_pool_from_uniforms": [_P, _FP, _FP, _FP, _FP],
+    "qmc_offsets_generate": [_P, C.c_uint64],
+    "qmc_offsets_set": [_P, _I, _IP, _I],
+    "qmc_offsets_get": [_P, _I, _IP, _I],
+    "qmc_facets_get": [_P, _I, _FP, _FP, _I],
+    "qmc_trace": [_P, _IP, _I],
+    "qmc_trace_records": [_P, _I, C.POINTER(RayRecord), C.c_int64],
+    "qmc_trace_stats_run": [_P, _IP, _I, C.POINTER(TraceStats)],
+    "qmc_flux_clear": [_P, _I],
+    "qmc_flux_read": [_P, _I, _FP, C.c_int64],
+    "qmc_flux_write": [_P, _I, _FP, C.c_int64],
+    "qmc_flux_device_ptr": [_P, _I, C.POINTER(_FP), _IP, _IP],
+    "qmc_smooth": [_P, _FP, _I, _I, _I, _F],
+    "qmc_smooth_receiver": [_P, _I, _I, _F],
+    "qmc_event_record": [_P, _I],
+    "qmc_event_elapsed_ms": [_P, _I, _I, _FP],
+}
+_RESTYPES = {"qmc_last_error": C.c_char_p, "qmc_status_string": C.c_char_p}
+
+_lib = None
+
+
+class QmcError(RuntimeError):
+    """A C-ABI call returned a non-zero qmc_status."""
+
+    def __init__(self, status: int, message: str, func: str):
+        super().__init__(f"{func}: status {status}: {message}")
+        self.status = status
+        self.message = message
+        self.func = func
+
+
+def lib() -> C.CDLL:
+    """Load libqmcrt.so (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "(the HIP extension is required; there is no CPU fallback)")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, argtypes in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.argtypes = argtypes
+            fn.restype = _RESTYPES.get(name, C.c_int)
+        _lib = handle
+    return _lib
+
+
+def call(name: str, *args) -> int:
+    """Call a qmc_* function; raise QmcError on a negative status."""
+    fn = getattr(lib(), name)
+    st = fn(*args)
+    if st != QMC_OK:
+        msg = lib().qmc_last_error()
+        raise QmcError(st, msg.decode(errors="replace") if msg else "", name)
+    return st
+
+
+def status_of(name: str, *args) -> tuple[int, str]:
+    """Call a qmc_* function and return (status, last error message) without raising."""
+    st = getattr(lib(), name)(*args)
+    msg = lib().qmc_last_error() if st != QMC_OK else b""
+    return st, (msg.decode(errors="replace") if msg else "")
+
+
+def fptr(arr):
+    import numpy as np
+
+    assert arr.dtype == np.float32 and arr.flags["C_CONTIGUOUS"]
+    return arr.ctypes.data_as(_FP)
+
+
+def iptr(arr):
+    import numpy as np
+
+    assert arr.dtype == np.int32 and arr.flags["C_CONTIGUOUS"]
+    return arr.ctypes.data_as(_IP)

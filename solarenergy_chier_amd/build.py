@@ -1,0 +1,92 @@
+"""Build the in-tree native libraries.
+
+* ``solarenergy_chier_amd/lib/libqmcrt.so`` — the HIP kernels for gfx950 plus the host scene model
+  behind the C ABI declared in ``include/qmcrt.h`` (hipcc, one shared object).
+* ``oracle/liboracle.so`` — the CPU restatement used by the tests and the CPU baseline
+  (g++, built by ``oracle/build.py``).
+
+Every compile uses ``-ffp-contract=off`` and no fast-math so host and device round alike.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+LIBDIR = PKG / "lib"
+LIB = LIBDIR / "libqmcrt.so"
+
+HIP_SOURCES = ["qmc_device.hip"]
+HOST_SOURCES = ["scene_host.cpp", "facade.cpp"]
+
+ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found: the HIP toolchain is required to build libqmcrt.so")
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, cwd=str(ROOT), capture_output=True, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout + proc.stderr)
+        raise RuntimeError(f"build step failed: {' '.join(cmd[:3])} ... (exit {proc.returncode})")
+
+
+def build_qmcrt(verbose: bool = False) -> Path:
+    LIBDIR.mkdir(parents=True, exist_ok=True)
+    objdir = ROOT / "_build"
+    objdir.mkdir(exist_ok=True)
+    common = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+              "-I", str(ROOT / "include"), "-I", str(CSRC)]
+    objs = []
+    for src in HIP_SOURCES:
+        obj = objdir / (Path(src).stem + ".o")
+        cmd = [hipcc(), "-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+               "-packed-fp32-ops", "-fgpu-rdc" if False else "-fno-gpu-rdc", *common, "-c", str(CSRC / src),
+               "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        if not (CSRC / src).exists():
+            continue
+        obj = objdir / (Path(src).stem + ".o")
+        cmd = [hipcc(), "-x", "c++", *common, "-c", str(CSRC / src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objs], "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> Path:
+    sys.path.insert(0, str(ROOT))
+    from oracle import build as oracle_build  # noqa: E402  (oracle is test infrastructure)
+
+    return oracle_build.build(verbose=verbose)
+
+
+def build_all(verbose: bool = False) -> None:
+    build_qmcrt(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose="-v" in sys.argv)
+    print(f"built {LIB}")

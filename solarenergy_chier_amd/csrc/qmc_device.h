@@ -1,0 +1,94 @@
+// qmc_device.h — device-side tables and kernel argument blocks of the gfx950 tracer.
+//
+// HBM layout (one qmc_ctx):
+//   grid      cell_start[nx*ny*nz+1], match[n_match]          (CSR of grid-relative sub ids)
+//             quads[3*n_sub] float4: A, E1=B-A, E2=C-A         (48 B per blocking quad)
+//   helios    DevHelio[n_heliostats]                           (wave-uniform, scalar-loaded)
+//   facets    DevFacet[sum rows*cols of focus-flat heliostats]
+//   pools     pool[P], pert[P] float4 (x, y, z, 0)             (16 B aligned gathers)
+//   offsets   int32 group starts, F+1 per heliostat, concatenated
+//   flux      float res_x*res_y per receiver
+#pragma once
+
+#include <stdint.h>
+
+#include "../../include/qmcrt.h"
+#include "qmc_math.h"
+
+namespace qmc {
+
+struct DevFrame {
+    float ux, uy, uz, nx, ny, nz, vx, vy, vz;
+    int identity;
+};
+
+struct DevHelio {
+    DevFrame fr;        // local2world frame of the heliostat normal (flat: origin and slope error)
+    DevFrame fr_focus;  // focus-variant frame of the heliostat normal (focus-flat origins)
+    float pos[3];
+    float size[3];
+    float gap[2];
+    float len_row, len_col;  // facet extent along z (rows) and x (cols)
+    float pl;           // micro-facet pixel length
+    float factor;       // DNI*(pl_h/pl_r)^2*rho/L
+    int32_t rows, cols, sub_rows, sub_cols;
+    int32_t F;          // micro-facets
+    int32_t type;       // 0 flat, 1 focus-flat
+    int32_t sub_begin, sub_count;
+    int32_t facet_base; // first DevFacet of a focus-flat heliostat
+    int32_t grid, receiver;
+    int32_t pad_;
+    int64_t offsets_base;
+};
+
+struct DevFacet {
+    DevFrame fr_local;  // focus-variant frame of the facet's local normal
+    DevFrame fr_pert;   // local2world frame of the facet's world normal (slope-error rotation)
+    float center[3];    // facet centre in the heliostat frame
+    float normal[3];    // facet world normal
+};
+
+struct DevReceiver {
+    int32_t type;       // 0 rectangle, 1 cylinder
+    int32_t res_x, res_y;
+    int32_t pad_;
+    // rectangle: A = v0, E1 = v1 - v0, E2 = v3 - v0
+    float A[3], E1[3], E2[3];
+    // cylinder
+    float pos[3];
+    float radius, height;
+};
+
+struct DevGrid {
+    float pos[3];
+    float interval[3];
+    int32_t nx, ny, nz;
+    int32_t pad_;
+    const int32_t *cell_start;
+    const int32_t *match;
+    const float4 *quads;
+};
+
+struct StatsAcc {
+    unsigned long long rays, unshadowed, unblocked, deposits, cells, candidates, tests, facets, edge;
+};
+
+struct TraceArgs {
+    DevGrid grid;
+    DevReceiver recv;
+    DevFrame sun;              // local2world frame of -sun_dir
+    const DevHelio *helios;
+    const DevFacet *facets;
+    const float4 *pool;
+    const float4 *pert;
+    const int32_t *offsets;
+    const int32_t *list;       // heliostat ids of this launch
+    float *flux;
+    qmc_ray_record *records;   // nullable (one heliostat per launch when set)
+    StatsAcc *stats;           // nullable
+    int32_t P, L;
+    int32_t n_list;
+    int32_t blocks_per_helio;  // grid.x extent per heliostat
+};
+
+}  // namespace qmc

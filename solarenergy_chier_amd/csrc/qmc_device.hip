@@ -1,0 +1,1278 @@
+// qmc_device.hip — gfx950 kernels of the heliostat-field tracer and the device half of the C ABI.
+//
+// Kernels (each restates a reference kernel; the arithmetic order is the reference's so that the
+// per-ray results are bit-identical to the host oracle):
+//   k_trace<RECV, REC, STATS>   map_raytracing, QuasiMonteCarloRayTracing/
+//                               RectangleReceiverRectangleGridRayTracing.cu:6-41 and
+//                               CylinderReceiverRectangleGridRayTracing.cu:6-40, with the 3D-DDA of
+//                               Grid/rectGridDDA.cu:31-119 and the deposits of Receiver/*.cu.
+//                               One launch covers every heliostat of a task list (grid.y), micro-facets
+//                               are generated in registers instead of being materialised.
+//   k_pool_sunshape / k_pool_perturbation / k_pool_from_uniforms
+//                               samplelights::linear_interpolation, map_angle2xyz, map_permuation
+//                               (Scene/SceneProcessor.cu:61-103) fed by a scrambled Sobol sequence and a
+//                               counter-based Gaussian instead of cuRAND XORWOW.
+//   k_offsets                   map_float2int (Tool/RandomNumberGenerator/RandomGenerator.cu:20-25).
+//   k_facets                    micro-facet kernels of RectangleHelio.cu:54-80 and
+//                               FocusFlatRectangleHelio.cu:12-64.
+//   k_smooth                    trimmed_mean (Tool/Smoother/ImageSmoother.cu:13-64) reading an LDS tile.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qmc_device.h"
+#include "scene_host.h"
+
+using namespace qmc;
+
+// ---------------------------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ V3 fr_apply(const DevFrame &f, V3 d) {
+    if (f.identity) return d;
+    return V3{d.x * f.ux + d.y * f.nx + d.z * f.vx,
+              d.x * f.uy + d.y * f.ny + d.z * f.vy,
+              d.x * f.uz + d.y * f.nz + d.z * f.vz};
+}
+
+struct RayStats {
+    unsigned cells, candidates, tests;
+};
+
+// 3D-DDA over the uniform grid; true when a non-own sub-heliostat quad is hit (first hit in
+// traversal and cell-list order; its id goes to hit_id).
+template <bool STATS>
+__device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, int own_lo, int own_hi,
+                                              int &hit_id, RayStats &st) {
+    const V3 gpos{g.pos[0], g.pos[1], g.pos[2]};
+    const V3 gint{g.interval[0], g.interval[1], g.interval[2]};
+    const V3 cell = (o - gpos) / gint;
+    int ix = (int)cell.x, iy = (int)cell.y, iz = (int)cell.z;
+    const int sx = (d.x >= 0) ? 1 : -1;
+    const int sy = (d.y >= 0) ? 1 : -1;
+    const int sz = (d.z >= 0) ? 1 : -1;
+    float tx = abs_divide(cal_tmax(d.x, gint.x, ix, o.x - gpos.x), d.x);
+    float ty = abs_divide(cal_tmax(d.y, gint.y, iy, o.y - gpos.y), d.y);
+    float tz = abs_divide(cal_tmax(d.z, gint.z, iz, o.z - gpos.z), d.z);
+    const float dx = abs_divide(gint.x, d.x);
+    const float dy = abs_divide(gint.y, d.y);
+    const float dz = abs_divide(gint.z, d.z);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    while (ix >= 0 && ix < nx && iy >= 0 && iy < ny && iz >= 0 && iz < nz) {
+        const int addr = ix * ny * nz + iy * nz + iz;
+        const int b = g.cell_start[addr], e = g.cell_start[addr + 1];
+        if (STATS) {
+            st.cells += 1;
+            st.candidates += (unsigned)(e - b);
+        }
+        for (int i = b; i < e; ++i) {
+            const int id = g.match[i];
+            if (id < own_lo || id >= own_hi) {
+                if (STATS) st.tests += 1;
+                const float4 qa = g.quads[3 * id];
+                const float4 qb = g.quads[3 * id + 1];
+                const float4 qc = g.quads[3 * id + 2];
+                float t, u, v;
+                if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) {
+                    hit_id = id;
+                    return true;
+                }
+            }
+        }
+        bool fx = false, fy = false, fz = false;
+        if (dda_less(tx, ty)) {
+            if (dda_less(tx, tz) || dda_equal(tx, tz)) { ix += sx; fx = true; }
+            if (dda_less(tz, tx) || dda_equal(tx, tz)) { iz += sz; fz = true; }
+        } else if (dda_less(ty, tx)) {
+            if (dda_less(ty, tz) || dda_equal(ty, tz)) { iy += sy; fy = true; }
+            if (dda_less(tz, ty) || dda_equal(ty, tz)) { iz += sz; fz = true; }
+        } else {
+            if (dda_less(tx, tz) || dda_equal(tx, tz)) { ix += sx; iy += sy; fx = fy = true; }
+            if (dda_less(tz, tx) || dda_equal(tx, tz)) { iz += sz; fz = true; }
+        }
+        tx += fx ? dx : 0.0f;
+        ty += fy ? dy : 0.0f;
+        tz += fz ? dz : 0.0f;
+    }
+    return false;
+}
+
+// Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
+// onto the facet normal.
+__device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
+                                               const DevFrame *&pert_frame, V3 &normal) {
+    const int cpr = H.cols * H.sub_cols;
+    int row = m / cpr;
+    int col = m % cpr;
+    const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
+    if (H.type == 0) {
+        // RectangleHelio.cu:64-79
+        const int brow = row / H.sub_rows;
+        const int bcol = col / H.sub_cols;
+        const float pl = H.pl;
+        V3 local{col * pl + bcol * H.gap[0] + pl / 2 - H.size[0] / 2, H.size[1] / 2,
+                 row * pl + brow * H.gap[1] + pl / 2 - H.size[2] / 2};
+        origin = fr_apply(H.fr, local) + pos;
+        pert_frame = &H.fr;
+        normal = V3{H.fr.nx, H.fr.ny, H.fr.nz};
+    } else {
+        // FocusFlatRectangleHelio.cu:21-39,53-64
+        const int block = (row / H.sub_rows) * H.cols + (col / H.sub_cols);
+        const DevFacet &Fc = facets[H.facet_base + block];
+        row %= H.sub_rows;
+        col %= H.sub_cols;
+        const float pl = H.pl;
+        V3 local{(col + 0.5f) * pl - H.len_col / 2, 0.0f, (row + 0.5f) * pl - H.len_row / 2};
+        local = fr_apply(Fc.fr_local, local) + V3{Fc.center[0], Fc.center[1], Fc.center[2]};
+        origin = fr_apply(H.fr_focus, local) + pos;
+        pert_frame = &Fc.fr_pert;
+        normal = V3{Fc.normal[0], Fc.normal[1], Fc.normal[2]};
+    }
+}
+
+// Receiver hit: true with (t, u, v) when the ray reaches the receiver surface.
+template <int RT>
+__device__ __forceinline__ bool receiver_hit(const DevReceiver &R, V3 o, V3 d, float &t, float &u, float &v) {
+    if (RT == 0) {
+        // RectangleReceiver::GIntersect (RectangleReceiver.cuh:32-34)
+        return ray_quad(o, d, V3{R.A[0], R.A[1], R.A[2]}, V3{R.E1[0], R.E1[1], R.E1[2]},
+                        V3{R.E2[0], R.E2[1], R.E2[2]}, t, u, v);
+    } else {
+        // CylinderReceiver::GIntersect (CylinderReceiver.cuh:39-89)
+        const float R2 = R.radius * R.radius;
+        V2 l{o.x - R.pos[0], o.z - R.pos[2]};
+        if (dot2(l, l) <= R2) return false;
+        V2 Ro{R.pos[0] - o.x, R.pos[2] - o.z};
+        float tp = dot2(Ro, normalize2(V2{d.x, d.z}));
+        float delta = dot2(Ro, Ro) - tp * tp;
+        if (tp < -kEps || delta > R2) return false;
+        float t_plus = delta <= 0.0f ? R.radius : sqrtf(R2 - delta);
+        t = (tp - t_plus) / length2(V2{d.x, d.z});
+        V3 ip = t * d + o;
+        u = (ip.y - R.pos[1]) / R.height + 0.5f;
+        if (u < 0.0f || u > 1.0f) return false;
+        V2 c = normalize2(V2{ip.x - R.pos[0], ip.z - R.pos[2]});
+        float cx = fminf(fmaxf(c.x, -1.0f), 1.0f);
+        v = (float)((double)acos_r(cx) / (2 * M_PI));
+        if (c.y < 0) v = 1 - v;
+        return true;
+    }
+}
+
+template <int RT, bool REC, bool STATS>
+__global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
+    const int h = a.list[blockIdx.y];
+    const DevHelio &H = a.helios[h];
+    const int L = a.L;
+    const long long nrays = (long long)H.F * L;
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+
+    __shared__ unsigned long long s_acc[9];
+    if (STATS) {
+        if (threadIdx.x < 9) s_acc[threadIdx.x] = 0ull;
+        __syncthreads();
+    }
+    RayStats st{0u, 0u, 0u};
+    unsigned c_unshadowed = 0, c_unblocked = 0, c_dep = 0, c_edge = 0;
+    const bool active = r < nrays;
+    if (!STATS && !active) return;
+
+    if (active) {
+        const int m = (int)(r / L);
+        const int j = (int)(r % L);
+        const int32_t *grp = a.offsets + H.offsets_base;
+        const int P = a.P;
+        const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
+
+        // Step 1: incident direction and shadowing (RectangleReceiverRectangleGridRayTracing.cu:14-22)
+        const int a1 = (grp[m] + j) % P;
+        const float4 s = a.pool[a1];
+        V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
+        V3 origin, fnormal;
+        const DevFrame *pf;
+        facet_geometry(H, a.facets, m, origin, pf, fnormal);
+        int hit = -1;
+        int status;
+        int index = -1;
+        float energy = 0.0f;
+        int flags = 0;
+        if (dda_collision<STATS>(origin, dir, a.grid, own_lo, own_hi, hit, st)) {
+            status = QMC_RAY_SHADOWED;
+            index = hit;
+        } else {
+            c_unshadowed = 1;
+            // Step 2: perturbed facet normal, reflection, blocking (:24-36)
+            const int a2 = (grp[(m + 1) % P] + j) % P;
+            const float4 q = a.pert[a2];
+            V3 normal = normalize(fr_apply(*pf, V3{q.x, q.y, q.z}));
+            dir = normalize(reflect(-dir, normal));
+            if (dda_collision<STATS>(origin, dir, a.grid, own_lo, own_hi, hit, st)) {
+                status = QMC_RAY_BLOCKED;
+                index = hit;
+            } else {
+                c_unblocked = 1;
+                // Step 3: receiver hit and deposit (Receiver/*ReceiverIntersection.cu)
+                float t, u, v;
+                if (receiver_hit<RT>(a.recv, origin, dir, t, u, v)) {
+                    energy = ray_energy(t, dir, normal, H.factor);
+                    int row = (int)(u * (float)a.recv.res_y);
+                    int col = (int)(v * (float)a.recv.res_x);
+                    if (row >= a.recv.res_y) { row = a.recv.res_y - 1; flags |= QMC_RAYF_EDGE_CLAMPED; }
+                    if (col >= a.recv.res_x) { col = a.recv.res_x - 1; flags |= QMC_RAYF_EDGE_CLAMPED; }
+                    const int addr = row * a.recv.res_x + col;
+                    unsafeAtomicAdd(a.flux + addr, energy);
+                    status = QMC_RAY_DEPOSITED;
+                    index = addr;
+                    c_dep = 1;
+                    c_edge = flags ? 1u : 0u;
+                } else {
+                    status = QMC_RAY_MISSED;
+                }
+            }
+        }
+        if (REC) {
+            qmc_ray_record rec;
+            rec.status = status;
+            rec.index = index;
+            rec.energy = energy;
+            rec.flags = flags;
+            a.records[r] = rec;
+        }
+        if (STATS) {
+            atomicAdd(&s_acc[0], 1ull);
+            if (j == 0) atomicAdd(&s_acc[7], 1ull);
+        }
+    }
+    if (STATS) {
+        if (active) {
+            if (c_unshadowed) atomicAdd(&s_acc[1], 1ull);
+            if (c_unblocked) atomicAdd(&s_acc[2], 1ull);
+            if (c_dep) atomicAdd(&s_acc[3], 1ull);
+            atomicAdd(&s_acc[4], (unsigned long long)st.cells);
+            atomicAdd(&s_acc[5], (unsigned long long)st.candidates);
+            atomicAdd(&s_acc[6], (unsigned long long)st.tests);
+            if (c_edge) atomicAdd(&s_acc[8], 1ull);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long *g = reinterpret_cast<unsigned long long *>(a.stats);
+            for (int k = 0; k < 9; ++k)
+                if (s_acc[k]) atomicAdd(g + k, s_acc[k]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Sample pools
+// ---------------------------------------------------------------------------------------------
+struct BuieArgs {
+    const float2 *cdf;   // n+1 (theta, cumulative)
+    const float *slope;  // n
+    int n;
+    float A, B, C, I, gamma;
+};
+
+// samplelights::max_less_index (Scene/SceneProcessor.cu:46-59)
+__device__ __forceinline__ int max_less_index(const float2 *cdf, float value, int n) {
+    int left = 0, right = n;
+    while (left <= right) {
+        int mid = (left + right) >> 1;
+        if (value > cdf[mid].y)
+            left = mid + 1;
+        else if (value < cdf[mid].y)
+            right = mid - 1;
+        else
+            return mid;
+    }
+    return right;
+}
+
+// samplelights::linear_interpolation body (Scene/SceneProcessor.cu:67-79): uniform -> theta [rad]
+__device__ __forceinline__ float buie_theta(const BuieArgs &b, float U) {
+    float u = U * b.A;
+    if (u < b.I) {
+        int id = max_less_index(b.cdf, u, b.n);
+        u = (u - b.cdf[id].y) * b.slope[id] + b.cdf[id].x;
+        u = atan_r(sqrtf(u / 4.65f) * 0.5016916f);
+    } else {
+        u = pow_r((u - b.I) * b.B + b.C, 1 / (b.gamma + 1));
+        float ri2 = 0.5016916f * 0.5016916f;
+        float ro2 = 1.340874f * 1.340874f;
+        u = atan_r(sqrtf(ro2 + (u - 4.65f) / 4.65f * (ro2 - ri2)));
+    }
+    return u / 100.0f;
+}
+
+__device__ __forceinline__ float4 sunshape_dir(const BuieArgs &b, float u_theta, float u_phi) {
+    float theta = buie_theta(b, u_theta);
+    float phi = u_phi * 2 * kMathPi;
+    // global_func::angle2xyz (Util/global_function.cuh:211-216)
+    float st = sin_r(theta);
+    return make_float4(st * cos_r(phi), cos_r(theta), st * sin_r(phi), 0.0f);
+}
+
+__device__ __forceinline__ float4 slope_dir(float gx, float gz) {
+    V3 n = normalize(V3{gx, 1.0f, gz});  // map_permuation (Scene/SceneProcessor.cu:83-93)
+    return make_float4(n.x, n.y, n.z, 0.0f);
+}
+
+struct SobolArgs {
+    uint32_t v0[32];
+    uint32_t v1[32];
+    uint32_t mask0, mask1;
+};
+
+__device__ __forceinline__ uint32_t sobol_point(const uint32_t *v, uint32_t i) {
+    uint32_t x = 0;
+    for (int k = 0; i; ++k, i >>= 1)
+        if (i & 1u) x ^= v[k];
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_pool_sunshape(float4 *pool, int P, SobolArgs s, BuieArgs b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const uint32_t x0 = sobol_point(s.v0, (uint32_t)i) ^ s.mask0;
+    const uint32_t x1 = sobol_point(s.v1, (uint32_t)i) ^ s.mask1;
+    pool[i] = sunshape_dir(b, u01_open0(x0), u01_open0(x1));
+}
+
+// Box-Muller on two 53-bit counter-hash uniforms; N(0, sigma) pairs (gx, gz).
+__device__ __forceinline__ void gauss_pair(uint64_t key, uint32_t i, float sigma, float &gx, float &gz) {
+    const uint64_t w1 = mix64(key + 2ull * i);
+    const uint64_t w2 = mix64(key + 2ull * i + 1ull);
+    const double d1 = (double)((w1 >> 11) + 1ull) * (1.0 / 9007199254740992.0);
+    const double d2 = (double)(w2 >> 11) * (1.0 / 9007199254740992.0);
+    const double rad = sqrt(-2.0 * log(d1));
+    const double ang = 6.283185307179586 * d2;
+    gx = (float)((double)sigma * rad * cos(ang));
+    gz = (float)((double)sigma * rad * sin(ang));
+}
+
+__global__ __launch_bounds__(256) void k_pool_perturbation(float4 *pert, int P, uint64_t key, float sigma) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float gx, gz;
+    gauss_pair(key, (uint32_t)i, sigma, gx, gz);
+    pert[i] = slope_dir(gx, gz);
+}
+
+__global__ __launch_bounds__(256) void k_pool_from_uniforms(float4 *pool, float4 *pert, int P, const float *ut,
+                                                            const float *up, const float *gx, const float *gz,
+                                                            BuieArgs b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    pool[i] = sunshape_dir(b, ut[i], up[i]);
+    pert[i] = slope_dir(gx[i], gz[i]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Group offsets, micro-facets
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int32_t group_offset(uint64_t seed, int h, int m, int P) {
+    const uint64_t w = mix64(mix64(seed ^ ((uint64_t)(uint32_t)h << 32)) + (uint64_t)(uint32_t)m);
+    const float u = u01_open0((uint32_t)(w >> 32));
+    return (int32_t)(u * (float)P);  // map_float2int: int(u*(high-low)+low), low = 0
+}
+
+__global__ __launch_bounds__(256) void k_offsets(int32_t *off, const DevHelio *helios, uint64_t seed, int P) {
+    const int h = blockIdx.y;
+    const DevHelio &H = helios[h];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > H.F) return;  // F+1 entries
+    off[H.offsets_base + m] = group_offset(seed, h, m, P);
+}
+
+__global__ __launch_bounds__(256) void k_facets(const DevHelio *helios, const DevFacet *facets, int h, float *origins,
+                                                float *normals) {
+    const DevHelio &H = helios[h];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= H.F) return;
+    V3 o, n;
+    const DevFrame *pf;
+    facet_geometry(H, facets, m, o, pf, n);
+    origins[3 * m] = o.x;
+    origins[3 * m + 1] = o.y;
+    origins[3 * m + 2] = o.z;
+    normals[3 * m] = n.x;
+    normals[3 * m + 1] = n.y;
+    normals[3 * m + 2] = n.z;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Trimmed-mean smoother (Tool/Smoother/ImageSmoother.cu:13-64, Tool/Smoother/heap.cuh)
+// ---------------------------------------------------------------------------------------------
+// Binary heap over e[1..cnt-1] (1-based, e[0] unused) with the reference's insert and
+// delete-and-insert; MAXH selects the comparison (max-heap: a > b, min-heap: a < b).
+template <bool MAXH>
+__device__ __forceinline__ bool hcmp(float a, float b) {
+    return MAXH ? (a > b) : (a < b);
+}
+
+template <bool MAXH>
+__device__ __forceinline__ void heap_insert(float *e, int &num, float x) {
+    int i = num++;
+    for (; i != 1 && hcmp<MAXH>(x, e[i / 2]); i /= 2) e[i] = e[i / 2];
+    e[i] = x;
+}
+
+template <bool MAXH>
+__device__ __forceinline__ void heap_replace_top(float *e, int num, float x) {
+    if (num == 1) return;  // empty heap: no change (returns a sentinel in the reference)
+    int i = 1;
+    for (; 2 * i < num;) {
+        int c = (2 * i + 1 >= num || hcmp<MAXH>(e[2 * i], e[2 * i + 1])) ? 2 * i : 2 * i + 1;
+        if (hcmp<MAXH>(e[c], x)) {
+            e[i] = e[c];
+            i = c;
+        } else {
+            break;
+        }
+    }
+    e[i] = x;
+}
+
+__device__ __forceinline__ float heap_sum(const float *e, int num) {
+    float s = 0.0f;
+    for (int i = 1; i < num; ++i) s += e[i];
+    return s;
+}
+
+__global__ void k_smooth(float *out, const float *src, int width, int height, int radius, float ratio, int kalloc,
+                         int tile_w) {
+    extern __shared__ float lds[];
+    const int bx0 = blockIdx.x * blockDim.x, by0 = blockIdx.y * blockDim.y;
+    // tile covers [bx0 - radius, bx0 + bdx + radius) x [by0 - radius, by0 + bdy + radius), clamped reads only
+    float *tile = lds;
+    const int tw = tile_w;
+    const int th = blockDim.y + 2 * radius;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int nthreads = blockDim.x * blockDim.y;
+    for (int idx = tid; idx < tw * th; idx += nthreads) {
+        int ty = idx / tw, tx = idx % tw;
+        int gx = bx0 - radius + tx, gy = by0 - radius + ty;
+        float val = 0.0f;
+        if (gx >= 0 && gx < width && gy >= 0 && gy < height) val = src[gy * width + gx];
+        tile[idx] = val;
+    }
+    __syncthreads();
+    const int x = bx0 + threadIdx.x, y = by0 + threadIdx.y;
+    if (x > width - 1 || y > height - 1) return;
+    float *max_e = lds + tw * th + (2 * tid) * kalloc;
+    float *min_e = lds + tw * th + (2 * tid + 1) * kalloc;
+    const int lx = (x - radius >= 0) ? (x - radius) : 0;
+    const int ly = (y - radius >= 0) ? (y - radius) : 0;
+    const int rx = (x + radius < width) ? (x + radius) : width - 1;
+    const int ry = (y + radius < height) ? (y + radius) : height - 1;
+    const int cnt = (rx - lx + 1) * (ry - ly + 1);
+    const int k = (int)(cnt * ratio);
+    const int max_size = k + 1;
+    int nmax = 1, nmin = 1;
+    float value = 0.0f;
+    for (int rr = ly; rr <= ry; ++rr) {
+        for (int cc = lx; cc <= rx; ++cc) {
+            const float cur = tile[(rr - by0 + radius) * tw + (cc - bx0 + radius)];
+            value += cur;
+            if (nmax != max_size) {
+                heap_insert<true>(max_e, nmax, cur);
+                heap_insert<false>(min_e, nmin, cur);
+            } else if (cur < (nmax > 1 ? max_e[1] : 0.0f)) {
+                heap_replace_top<true>(max_e, nmax, cur);
+            } else if (cur > (nmin > 1 ? min_e[1] : 0.0f)) {
+                heap_replace_top<false>(min_e, nmin, cur);
+            }
+        }
+    }
+    out[y * width + x] = (value - heap_sum(max_e, nmax) - heap_sum(min_e, nmin)) / float(cnt - 2 * k);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct GridBuf {
+    DevGrid g{};
+    int32_t *cell_start = nullptr;
+    int32_t *match = nullptr;
+    float4 *quads = nullptr;
+    int n_sub = 0;
+    int receiver = 0;
+};
+
+struct RecvBuf {
+    DevReceiver r{};
+    float *flux = nullptr;
+    int w = 0, h = 0;
+};
+
+}  // namespace
+
+struct qmc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[16] = {};
+    bool uploaded = false;
+    qmc_config cfg{};
+    std::vector<GridBuf> grids;
+    std::vector<RecvBuf> recvs;
+    std::vector<DevHelio> h_helios;
+    DevHelio *d_helios = nullptr;
+    DevFacet *d_facets = nullptr;
+    int32_t *d_offsets = nullptr;
+    int64_t offsets_total = 0;
+    float4 *d_pool = nullptr;
+    float4 *d_pert = nullptr;
+    int P = 0, L = 0;
+    bool pool_ready = false, offsets_ready = false;
+    int32_t *d_list = nullptr;
+    int list_cap = 0;
+    StatsAcc *d_stats = nullptr;
+    qmc_ray_record *d_rec = nullptr;
+    size_t rec_cap = 0;
+    float *d_scratch = nullptr;
+    size_t scratch_cap = 0;
+    DevFrame sun{};
+    float2 *d_cdf = nullptr;
+    float *d_slope = nullptr;
+    BuieArgs buie{};
+};
+
+namespace {
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(QMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+template <class T>
+qmc_status dev_alloc(T *&p, size_t count) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? QMC_E_OOM : QMC_E_HIP,
+                    std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " B): " + hipGetErrorString(e));
+    }
+    return QMC_OK;
+}
+
+template <class T>
+void dev_free(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+DevFrame to_dev(const Frame &f) {
+    DevFrame d;
+    d.ux = f.u.x; d.uy = f.u.y; d.uz = f.u.z;
+    d.nx = f.n.x; d.ny = f.n.y; d.nz = f.n.z;
+    d.vx = f.v.x; d.vy = f.v.y; d.vz = f.v.z;
+    d.identity = f.identity;
+    return d;
+}
+
+void free_scene(qmc_ctx *c) {
+    for (auto &g : c->grids) {
+        dev_free(g.cell_start);
+        dev_free(g.match);
+        dev_free(g.quads);
+    }
+    c->grids.clear();
+    for (auto &r : c->recvs) dev_free(r.flux);
+    c->recvs.clear();
+    dev_free(c->d_helios);
+    dev_free(c->d_facets);
+    dev_free(c->d_offsets);
+    c->h_helios.clear();
+    c->uploaded = false;
+    c->offsets_ready = false;
+}
+
+qmc_status set_device(qmc_ctx *c) {
+    HIP_TRY(hipSetDevice(c->device));
+    return QMC_OK;
+}
+
+// Sobol direction numbers of the first two dimensions (dimension 0: van der Corput; dimension 1:
+// primitive polynomial x + 1 with m_1 = 1).
+void sobol_directions(uint32_t v0[32], uint32_t v1[32]) {
+    uint32_t m[33];
+    for (int k = 0; k < 32; ++k) v0[k] = 1u << (31 - k);
+    m[1] = 1;
+    for (int k = 2; k <= 32; ++k) m[k] = (m[k - 1] << 1) ^ m[k - 1];
+    for (int k = 1; k <= 32; ++k) v1[k - 1] = m[k] << (32 - k);
+}
+
+}  // namespace
+
+extern "C" qmc_status qmc_create(int32_t device, qmc_ctx **out) {
+    if (!out) return fail(QMC_E_INVALID_ARG, "qmc_create: null out");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        return fail(QMC_E_HIP, std::string("qmc_create: no HIP device available (") +
+                                   (e != hipSuccess ? hipGetErrorString(e) : "0 devices") + ")");
+    if (device < 0 || device >= n) return fail(QMC_E_RANGE, "qmc_create: device index out of range");
+    auto *c = new (std::nothrow) qmc_ctx();
+    if (!c) return fail(QMC_E_OOM, "qmc_create: allocation failed");
+    c->device = device;
+    qmc_status st = set_device(c);
+    if (st != QMC_OK) {
+        delete c;
+        return st;
+    }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(QMC_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    for (int k = 0; k < 16; ++k) {
+        e = hipEventCreate(&c->ev[k]);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(QMC_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    if (dev_alloc(c->d_stats, 1) != QMC_OK) {
+        delete c;
+        return QMC_E_OOM;
+    }
+    *out = c;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
+    if (!c) return QMC_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_scene(c);
+    dev_free(c->d_pool);
+    dev_free(c->d_pert);
+    dev_free(c->d_list);
+    dev_free(c->d_stats);
+    dev_free(c->d_rec);
+    dev_free(c->d_scratch);
+    dev_free(c->d_cdf);
+    dev_free(c->d_slope);
+    for (int k = 0; k < 16; ++k)
+        if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_get_stream(qmc_ctx *c, void **stream) {
+    if (!c || !stream) return fail(QMC_E_INVALID_ARG, "qmc_get_stream: null argument");
+    *stream = (void *)c->stream;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_sync(qmc_ctx *c) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_sync: null ctx");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
+    if (!c || !s) return fail(QMC_E_INVALID_ARG, "qmc_upload: null argument");
+    if (!s->processed) return fail(QMC_E_STATE, "qmc_upload: scene not processed (call qmc_scene_process)");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    free_scene(c);
+    c->cfg = s->cfg;
+    // sun frame: local2world_rotate(sample, -sun_dir)
+    V3 sun{s->cfg.sun_dir[0], s->cfg.sun_dir[1], s->cfg.sun_dir[2]};
+    c->sun = to_dev(make_frame(-sun, false));
+    // receivers
+    c->recvs.resize(s->receivers.size());
+    for (size_t i = 0; i < s->receivers.size(); ++i) {
+        const HostReceiver &hr = s->receivers[i];
+        RecvBuf &rb = c->recvs[i];
+        DevReceiver &d = rb.r;
+        std::memset(&d, 0, sizeof(d));
+        d.type = hr.d.type;
+        d.res_x = hr.res_x;
+        d.res_y = hr.res_y;
+        if (d.type == 0) {
+            V3 E1 = hr.vert[1] - hr.vert[0], E2 = hr.vert[3] - hr.vert[0];
+            d.A[0] = hr.vert[0].x; d.A[1] = hr.vert[0].y; d.A[2] = hr.vert[0].z;
+            d.E1[0] = E1.x; d.E1[1] = E1.y; d.E1[2] = E1.z;
+            d.E2[0] = E2.x; d.E2[1] = E2.y; d.E2[2] = E2.z;
+        } else {
+            d.pos[0] = hr.d.pos[0]; d.pos[1] = hr.d.pos[1]; d.pos[2] = hr.d.pos[2];
+            d.radius = hr.d.size[0];
+            d.height = hr.d.size[1];
+        }
+        rb.w = hr.res_x;
+        rb.h = hr.res_y;
+        if (rb.w <= 0 || rb.h <= 0) {
+            // an unprocessed receiver (no grid uses it) keeps an empty 1x1 map
+            rb.w = std::max(rb.w, 1);
+            rb.h = std::max(rb.h, 1);
+        }
+        if ((st = dev_alloc(rb.flux, (size_t)rb.w * rb.h))) return st;
+        HIP_TRY(hipMemsetAsync(rb.flux, 0, sizeof(float) * (size_t)rb.w * rb.h, c->stream));
+    }
+    // grids
+    c->grids.resize(s->grids.size());
+    for (size_t gi = 0; gi < s->grids.size(); ++gi) {
+        const HostGrid &hg = s->grids[gi];
+        GridBuf &gb = c->grids[gi];
+        if ((int)hg.sub_verts.size() != 3 * hg.n_sub)
+            return fail(QMC_E_UNSUPPORTED, "qmc_upload: grid " + std::to_string(gi) +
+                                               " has a sub-heliostat count override without matching quads");
+        std::vector<float4> quads((size_t)hg.n_sub * 3);
+        for (int i = 0; i < hg.n_sub; ++i) {
+            // quad (A, B, C) = (V[3i+1], V[3i], V[3i+2]) as in rectGridDDA::intersect (rectGridDDA.cu:15-18)
+            V3 A = hg.sub_verts[3 * i + 1], B = hg.sub_verts[3 * i], C = hg.sub_verts[3 * i + 2];
+            V3 E1 = B - A, E2 = C - A;
+            quads[3 * i] = make_float4(A.x, A.y, A.z, 0.0f);
+            quads[3 * i + 1] = make_float4(E1.x, E1.y, E1.z, 0.0f);
+            quads[3 * i + 2] = make_float4(E2.x, E2.y, E2.z, 0.0f);
+        }
+        if ((st = dev_alloc(gb.cell_start, hg.cell_start.size()))) return st;
+        if ((st = dev_alloc(gb.match, hg.match.size()))) return st;
+        if ((st = dev_alloc(gb.quads, quads.size()))) return st;
+        HIP_TRY(hipMemcpyAsync(gb.cell_start, hg.cell_start.data(), hg.cell_start.size() * sizeof(int32_t),
+                               hipMemcpyHostToDevice, c->stream));
+        if (!hg.match.empty())
+            HIP_TRY(hipMemcpyAsync(gb.match, hg.match.data(), hg.match.size() * sizeof(int32_t),
+                                   hipMemcpyHostToDevice, c->stream));
+        if (!quads.empty())
+            HIP_TRY(hipMemcpyAsync(gb.quads, quads.data(), quads.size() * sizeof(float4), hipMemcpyHostToDevice,
+                                   c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        DevGrid &g = gb.g;
+        for (int k = 0; k < 3; ++k) {
+            g.pos[k] = hg.d.pos[k];
+            g.interval[k] = hg.d.interval[k];
+        }
+        g.nx = hg.nx;
+        g.ny = hg.ny;
+        g.nz = hg.nz;
+        g.cell_start = gb.cell_start;
+        g.match = gb.match;
+        g.quads = gb.quads;
+        gb.n_sub = hg.n_sub;
+        gb.receiver = hg.d.receiver;
+    }
+    // heliostats and focus-flat facets
+    std::vector<DevFacet> facets;
+    c->h_helios.resize(s->heliostats.size());
+    int64_t off_base = 0;
+    for (size_t h = 0; h < s->heliostats.size(); ++h) {
+        const HostHeliostat &hh = s->heliostats[h];
+        DevHelio &d = c->h_helios[h];
+        std::memset(&d, 0, sizeof(d));
+        d.fr = to_dev(make_frame(hh.normal, false));
+        d.fr_focus = to_dev(make_frame(hh.normal, true));
+        for (int k = 0; k < 3; ++k) {
+            d.pos[k] = hh.d.pos[k];
+            d.size[k] = hh.d.size[k];
+        }
+        d.gap[0] = hh.d.gap[0];
+        d.gap[1] = hh.d.gap[1];
+        d.len_row = hh.len_row;
+        d.len_col = hh.len_col;
+        d.pl = hh.pixel_length;
+        d.factor = heliostat_factor(*s, (int)h);
+        d.rows = hh.d.rows;
+        d.cols = hh.d.cols;
+        d.sub_rows = hh.sub_rows;
+        d.sub_cols = hh.sub_cols;
+        d.F = hh.n_micro;
+        d.type = hh.type;
+        d.sub_begin = hh.sub_begin;
+        d.sub_count = hh.sub_count;
+        d.grid = hh.d.grid;
+        d.receiver = s->grids[hh.d.grid].d.receiver;
+        d.offsets_base = off_base;
+        off_base += (int64_t)hh.n_micro + 1;
+        d.facet_base = (int)facets.size();
+        if (hh.type == 1) {
+            Frame fh = make_frame(hh.normal, true);
+            for (size_t k = 0; k < hh.loc_normals.size(); ++k) {
+                DevFacet f;
+                std::memset(&f, 0, sizeof(f));
+                f.fr_local = to_dev(make_frame(hh.loc_normals[k], true));
+                V3 wn = frame_apply(fh, hh.loc_normals[k]);  // map_microhelio_normals
+                f.fr_pert = to_dev(make_frame(wn, false));
+                f.center[0] = hh.loc_centers[k].x;
+                f.center[1] = hh.loc_centers[k].y;
+                f.center[2] = hh.loc_centers[k].z;
+                f.normal[0] = wn.x;
+                f.normal[1] = wn.y;
+                f.normal[2] = wn.z;
+                facets.push_back(f);
+            }
+        }
+    }
+    c->offsets_total = off_base;
+    if ((st = dev_alloc(c->d_helios, c->h_helios.size()))) return st;
+    if ((st = dev_alloc(c->d_facets, facets.size()))) return st;
+    if ((st = dev_alloc(c->d_offsets, (size_t)off_base))) return st;
+    if (!c->h_helios.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_helios, c->h_helios.data(), c->h_helios.size() * sizeof(DevHelio),
+                               hipMemcpyHostToDevice, c->stream));
+    if (!facets.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_facets, facets.data(), facets.size() * sizeof(DevFacet), hipMemcpyHostToDevice,
+                               c->stream));
+    if ((int)s->heliostats.size() > c->list_cap) {
+        if ((st = dev_alloc(c->d_list, s->heliostats.size()))) return st;
+        c->list_cap = (int)s->heliostats.size();
+    }
+    // Buie table of this configuration
+    if (s->cfg.inverse_transform_sampling_groups > 0 && s->cfg.csr > 0.0f) {
+        std::vector<float> cdf, sl;
+        float abcig[5];
+        sunshape_params(s->cfg, cdf, sl, abcig);
+        if ((st = dev_alloc(c->d_cdf, cdf.size() / 2))) return st;
+        if ((st = dev_alloc(c->d_slope, sl.size()))) return st;
+        HIP_TRY(hipMemcpyAsync(c->d_cdf, cdf.data(), cdf.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_slope, sl.data(), sl.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        c->buie.cdf = c->d_cdf;
+        c->buie.slope = c->d_slope;
+        c->buie.n = s->cfg.inverse_transform_sampling_groups;
+        c->buie.A = abcig[0];
+        c->buie.B = abcig[1];
+        c->buie.C = abcig[2];
+        c->buie.I = abcig[3];
+        c->buie.gamma = abcig[4];
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->uploaded = true;
+    c->offsets_ready = false;
+    if (c->pool_ready && c->P != s->cfg.num_sunshape_groups * s->cfg.num_sunshape_lights_per_group)
+        c->pool_ready = false;
+    return QMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Pools
+// ---------------------------------------------------------------------------------------------
+static qmc_status ensure_pool(qmc_ctx *c, int P, int L) {
+    if (P <= 0 || L <= 0) return fail(QMC_E_INVALID_ARG, "pool size and lights per group must be positive");
+    if (c->P != P || !c->d_pool) {
+        qmc_status st;
+        if ((st = dev_alloc(c->d_pool, (size_t)P))) return st;
+        if ((st = dev_alloc(c->d_pert, (size_t)P))) return st;
+    }
+    c->P = P;
+    c->L = L;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_pool_generate(qmc_ctx *c, const qmc_pool_params *p) {
+    if (!c || !p) return fail(QMC_E_INVALID_ARG, "qmc_pool_generate: null argument");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_pool_generate: upload a scene first");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    const int L = c->cfg.num_sunshape_lights_per_group;
+    const int P = c->cfg.num_sunshape_groups * L;
+    if (!c->d_cdf) return fail(QMC_E_INVALID_ARG, "qmc_pool_generate: sun-shape table missing (csr, groups)");
+    if ((st = ensure_pool(c, P, L))) return st;
+    SobolArgs sa;
+    sobol_directions(sa.v0, sa.v1);
+    sa.mask0 = (uint32_t)(mix64(p->seed_sunshape) >> 32);
+    sa.mask1 = (uint32_t)(mix64(p->seed_sunshape ^ 0xA5A5A5A5A5A5A5A5ull) >> 32);
+    const int blocks = (P + 255) / 256;
+    hipLaunchKernelGGL(k_pool_sunshape, dim3(blocks), dim3(256), 0, c->stream, c->d_pool, P, sa, c->buie);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_pool_perturbation, dim3(blocks), dim3(256), 0, c->stream, c->d_pert, P,
+                       mix64(p->seed_perturbation), c->cfg.helio_disturb_std);
+    HIP_TRY(hipGetLastError());
+    c->pool_ready = true;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_pool_from_uniforms(qmc_ctx *c, const float *ut, const float *up, const float *gx,
+                                             const float *gz) {
+    if (!c || !ut || !up || !gx || !gz) return fail(QMC_E_INVALID_ARG, "qmc_pool_from_uniforms: null argument");
+    if (!c->uploaded || !c->d_cdf) return fail(QMC_E_STATE, "qmc_pool_from_uniforms: upload a scene first");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    const int L = c->cfg.num_sunshape_lights_per_group;
+    const int P = c->cfg.num_sunshape_groups * L;
+    if ((st = ensure_pool(c, P, L))) return st;
+    float *tmp = nullptr;
+    if ((st = dev_alloc(tmp, 4 * (size_t)P))) return st;
+    const float *src[4] = {ut, up, gx, gz};
+    for (int k = 0; k < 4; ++k) {
+        hipError_t e = hipMemcpyAsync(tmp + (size_t)k * P, src[k], sizeof(float) * P, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) {
+            dev_free(tmp);
+            return fail(QMC_E_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+        }
+    }
+    hipLaunchKernelGGL(k_pool_from_uniforms, dim3((P + 255) / 256), dim3(256), 0, c->stream, c->d_pool, c->d_pert, P,
+                       tmp, tmp + P, tmp + 2 * (size_t)P, tmp + 3 * (size_t)P, c->buie);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free(tmp);
+    if (e != hipSuccess) return fail(QMC_E_HIP, std::string("k_pool_from_uniforms: ") + hipGetErrorString(e));
+    c->pool_ready = true;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_pool_set(qmc_ctx *c, const float *dirs, const float *perts, int32_t P, int32_t L) {
+    if (!c || !dirs || !perts) return fail(QMC_E_INVALID_ARG, "qmc_pool_set: null argument");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    if ((st = ensure_pool(c, P, L))) return st;
+    std::vector<float4> a((size_t)P), b((size_t)P);
+    for (int i = 0; i < P; ++i) {
+        a[i] = make_float4(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2], 0.0f);
+        b[i] = make_float4(perts[3 * i], perts[3 * i + 1], perts[3 * i + 2], 0.0f);
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_pool, a.data(), sizeof(float4) * P, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_pert, b.data(), sizeof(float4) * P, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->pool_ready = true;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_pool_get(qmc_ctx *c, float *dirs, float *perts, int32_t P) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_pool_get: null ctx");
+    if (!c->pool_ready) return fail(QMC_E_STATE, "qmc_pool_get: no pool");
+    if (P != c->P) return fail(QMC_E_INVALID_ARG, "qmc_pool_get: P mismatch");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    std::vector<float4> a((size_t)P), b((size_t)P);
+    HIP_TRY(hipMemcpyAsync(a.data(), c->d_pool, sizeof(float4) * P, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(b.data(), c->d_pert, sizeof(float4) * P, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < P; ++i) {
+        if (dirs) {
+            dirs[3 * i] = a[i].x;
+            dirs[3 * i + 1] = a[i].y;
+            dirs[3 * i + 2] = a[i].z;
+        }
+        if (perts) {
+            perts[3 * i] = b[i].x;
+            perts[3 * i + 1] = b[i].y;
+            perts[3 * i + 2] = b[i].z;
+        }
+    }
+    return QMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Offsets and facets
+// ---------------------------------------------------------------------------------------------
+static int max_F(const qmc_ctx *c) {
+    int m = 0;
+    for (auto &h : c->h_helios) m = std::max(m, h.F);
+    return m;
+}
+
+extern "C" qmc_status qmc_offsets_generate(qmc_ctx *c, uint64_t seed) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_offsets_generate: null ctx");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_offsets_generate: upload a scene first");
+    if (!c->pool_ready) return fail(QMC_E_STATE, "qmc_offsets_generate: set or generate the pool first (offsets are in [0, P])");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    const int nh = (int)c->h_helios.size();
+    if (nh == 0) return QMC_OK;
+    if (nh > 65535) return fail(QMC_E_UNSUPPORTED, "qmc_offsets_generate: more than 65535 heliostats");
+    const int bx = (max_F(c) + 1 + 255) / 256;
+    hipLaunchKernelGGL(k_offsets, dim3(bx, nh), dim3(256), 0, c->stream, c->d_offsets, c->d_helios, seed, c->P);
+    HIP_TRY(hipGetLastError());
+    c->offsets_ready = true;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_offsets_set(qmc_ctx *c, int32_t h, const int32_t *off, int32_t n) {
+    if (!c || !off) return fail(QMC_E_INVALID_ARG, "qmc_offsets_set: null argument");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_offsets_set: upload a scene first");
+    if (h < 0 || h >= (int)c->h_helios.size()) return fail(QMC_E_RANGE, "qmc_offsets_set: heliostat out of range");
+    const DevHelio &H = c->h_helios[h];
+    if (n != H.F + 1) return fail(QMC_E_INVALID_ARG, "qmc_offsets_set: need F+1 = " + std::to_string(H.F + 1) + " offsets");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(c->d_offsets + H.offsets_base, off, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->offsets_ready = true;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_offsets_get(qmc_ctx *c, int32_t h, int32_t *off, int32_t n) {
+    if (!c || !off) return fail(QMC_E_INVALID_ARG, "qmc_offsets_get: null argument");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_offsets_get: upload a scene first");
+    if (h < 0 || h >= (int)c->h_helios.size()) return fail(QMC_E_RANGE, "qmc_offsets_get: heliostat out of range");
+    const DevHelio &H = c->h_helios[h];
+    if (n != H.F + 1) return fail(QMC_E_INVALID_ARG, "qmc_offsets_get: need F+1 offsets");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(off, c->d_offsets + H.offsets_base, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_facets_get(qmc_ctx *c, int32_t h, float *origins, float *normals, int32_t F) {
+    if (!c || !origins || !normals) return fail(QMC_E_INVALID_ARG, "qmc_facets_get: null argument");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_facets_get: upload a scene first");
+    if (h < 0 || h >= (int)c->h_helios.size()) return fail(QMC_E_RANGE, "qmc_facets_get: heliostat out of range");
+    if (F != c->h_helios[h].F) return fail(QMC_E_INVALID_ARG, "qmc_facets_get: F mismatch");
+    if (F == 0) return QMC_OK;
+    qmc_status st = set_device(c);
+    if (st) return st;
+    float *buf = nullptr;
+    if ((st = dev_alloc(buf, 6 * (size_t)F))) return st;
+    hipLaunchKernelGGL(k_facets, dim3((F + 255) / 256), dim3(256), 0, c->stream, c->d_helios, c->d_facets, h, buf,
+                       buf + 3 * (size_t)F);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(origins, buf, sizeof(float) * 3 * F, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(normals, buf + 3 * (size_t)F, sizeof(float) * 3 * F, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free(buf);
+    if (e != hipSuccess) return fail(QMC_E_HIP, std::string("qmc_facets_get: ") + hipGetErrorString(e));
+    return QMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Trace
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, int maxF, qmc_ray_record *rec,
+                        StatsAcc *stats) {
+    const GridBuf &gb = c->grids[grid];
+    const RecvBuf &rb = c->recvs[gb.receiver];
+    TraceArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.grid = gb.g;
+    a.recv = rb.r;
+    a.sun = c->sun;
+    a.helios = c->d_helios;
+    a.facets = c->d_facets;
+    a.pool = c->d_pool;
+    a.pert = c->d_pert;
+    a.offsets = c->d_offsets;
+    a.list = d_list;
+    a.flux = rb.flux;
+    a.records = rec;
+    a.stats = stats;
+    a.P = c->P;
+    a.L = c->L;
+    a.n_list = n;
+    const long long rays = (long long)maxF * c->L;
+    const long long bx = (rays + 255) / 256;
+    if (bx > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
+    if (n > 65535) return fail(QMC_E_UNSUPPORTED, "trace: more than 65535 heliostats in one grid launch");
+    a.blocks_per_helio = (int)bx;
+    if (bx == 0 || n == 0) return QMC_OK;
+    dim3 g((unsigned)bx, (unsigned)n), b(256);
+    const int rt = rb.r.type;
+    if (rec) {
+        if (rt == 0) hipLaunchKernelGGL((k_trace<0, true, false>), g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trace<1, true, false>), g, b, 0, c->stream, a);
+    } else if (stats) {
+        if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, true>), g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trace<1, false, true>), g, b, 0, c->stream, a);
+    } else {
+        if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, false>), g, b, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trace<1, false, false>), g, b, 0, c->stream, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return QMC_OK;
+}
+
+qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *rec, StatsAcc *stats) {
+    if (!c->uploaded) return fail(QMC_E_STATE, "trace: upload a scene first");
+    if (!c->pool_ready) return fail(QMC_E_STATE, "trace: set or generate the sample pools first");
+    if (!c->offsets_ready) return fail(QMC_E_STATE, "trace: set or generate the micro-facet group offsets first");
+    if (n < 0 || (n > 0 && !ids)) return fail(QMC_E_INVALID_ARG, "trace: bad heliostat list");
+    const int nh = (int)c->h_helios.size();
+    for (int i = 0; i < n; ++i)
+        if (ids[i] < 0 || ids[i] >= nh)
+            return fail(QMC_E_RANGE, "The ray tracing heliostat index " + std::to_string(ids[i]) +
+                                         " is invalid.The heliostat index should between 0 and " + std::to_string(nh) + ".");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    // group by grid, keep task order inside a grid
+    std::vector<int32_t> sorted;
+    sorted.reserve(n);
+    std::vector<std::pair<int, int>> spans;  // (grid, count)
+    for (size_t g = 0; g < c->grids.size(); ++g) {
+        int before = (int)sorted.size();
+        for (int i = 0; i < n; ++i)
+            if (c->h_helios[ids[i]].grid == (int)g) sorted.push_back(ids[i]);
+        spans.emplace_back((int)g, (int)sorted.size() - before);
+    }
+    if (sorted.empty()) return QMC_OK;
+    HIP_TRY(hipMemcpyAsync(c->d_list, sorted.data(), sizeof(int32_t) * sorted.size(), hipMemcpyHostToDevice, c->stream));
+    int at = 0;
+    for (auto &sp : spans) {
+        if (sp.second == 0) continue;
+        int maxF = 0;
+        for (int i = at; i < at + sp.second; ++i) maxF = std::max(maxF, c->h_helios[sorted[i]].F);
+        if ((st = launch_trace(c, sp.first, c->d_list + at, sp.second, maxF, rec, stats))) return st;
+        at += sp.second;
+    }
+    return QMC_OK;
+}
+
+}  // namespace
+
+extern "C" qmc_status qmc_trace(qmc_ctx *c, const int32_t *ids, int32_t n) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_trace: null ctx");
+    return trace_common(c, ids, n, nullptr, nullptr);
+}
+
+extern "C" qmc_status qmc_trace_records(qmc_ctx *c, int32_t h, qmc_ray_record *rec, int64_t n) {
+    if (!c || !rec) return fail(QMC_E_INVALID_ARG, "qmc_trace_records: null argument");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_trace_records: upload a scene first");
+    if (h < 0 || h >= (int)c->h_helios.size()) return fail(QMC_E_RANGE, "qmc_trace_records: heliostat out of range");
+    const int64_t need = (int64_t)c->h_helios[h].F * c->L;
+    if (n != need) return fail(QMC_E_INVALID_ARG, "qmc_trace_records: need F*L = " + std::to_string(need) + " records");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    if ((size_t)need > c->rec_cap) {
+        if ((st = dev_alloc(c->d_rec, (size_t)need))) return st;
+        c->rec_cap = (size_t)need;
+    }
+    int32_t id = h;
+    if ((st = trace_common(c, &id, 1, c->d_rec, nullptr))) return st;
+    HIP_TRY(hipMemcpyAsync(rec, c->d_rec, sizeof(qmc_ray_record) * need, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_trace_stats_run(qmc_ctx *c, const int32_t *ids, int32_t n, qmc_trace_stats *out) {
+    if (!c || !out) return fail(QMC_E_INVALID_ARG, "qmc_trace_stats_run: null argument");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    HIP_TRY(hipMemsetAsync(c->d_stats, 0, sizeof(StatsAcc), c->stream));
+    if ((st = trace_common(c, ids, n, nullptr, c->d_stats))) return st;
+    StatsAcc h;
+    HIP_TRY(hipMemcpyAsync(&h, c->d_stats, sizeof(StatsAcc), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out->rays = h.rays;
+    out->unshadowed = h.unshadowed;
+    out->unblocked = h.unblocked;
+    out->deposits = h.deposits;
+    out->cells = h.cells;
+    out->candidates = h.candidates;
+    out->tests = h.tests;
+    out->facets = h.facets;
+    out->edge_rays = h.edge;
+    return QMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Flux maps
+// ---------------------------------------------------------------------------------------------
+extern "C" qmc_status qmc_flux_clear(qmc_ctx *c, int32_t r) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_flux_clear: null ctx");
+    if (r < 0 || r >= (int)c->recvs.size()) return fail(QMC_E_RANGE, "qmc_flux_clear: receiver out of range");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    const RecvBuf &rb = c->recvs[r];
+    HIP_TRY(hipMemsetAsync(rb.flux, 0, sizeof(float) * (size_t)rb.w * rb.h, c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_flux_read(qmc_ctx *c, int32_t r, float *out, int64_t n) {
+    if (!c || !out) return fail(QMC_E_INVALID_ARG, "qmc_flux_read: null argument");
+    if (r < 0 || r >= (int)c->recvs.size()) return fail(QMC_E_RANGE, "qmc_flux_read: receiver out of range");
+    const RecvBuf &rb = c->recvs[r];
+    if (n != (int64_t)rb.w * rb.h) return fail(QMC_E_INVALID_ARG, "qmc_flux_read: size mismatch");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(out, rb.flux, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_flux_write(qmc_ctx *c, int32_t r, const float *in, int64_t n) {
+    if (!c || !in) return fail(QMC_E_INVALID_ARG, "qmc_flux_write: null argument");
+    if (r < 0 || r >= (int)c->recvs.size()) return fail(QMC_E_RANGE, "qmc_flux_write: receiver out of range");
+    const RecvBuf &rb = c->recvs[r];
+    if (n != (int64_t)rb.w * rb.h) return fail(QMC_E_INVALID_ARG, "qmc_flux_write: size mismatch");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(rb.flux, in, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_flux_device_ptr(qmc_ctx *c, int32_t r, float **p, int32_t *w, int32_t *h) {
+    if (!c || !p) return fail(QMC_E_INVALID_ARG, "qmc_flux_device_ptr: null argument");
+    if (r < 0 || r >= (int)c->recvs.size()) return fail(QMC_E_RANGE, "qmc_flux_device_ptr: receiver out of range");
+    *p = c->recvs[r].flux;
+    if (w) *w = c->recvs[r].w;
+    if (h) *h = c->recvs[r].h;
+    return QMC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Smoother
+// ---------------------------------------------------------------------------------------------
+extern "C" qmc_status qmc_smooth(qmc_ctx *c, float *img, int32_t w, int32_t h, int32_t radius, float ratio) {
+    if (!c || !img) return fail(QMC_E_INVALID_ARG, "qmc_smooth: null argument");
+    if (w <= 0 || h <= 0 || radius < 0 || !(ratio >= 0.0f) || ratio >= 0.5f)
+        return fail(QMC_E_INVALID_ARG, "qmc_smooth: need w, h > 0, radius >= 0 and 0 <= ratio < 0.5");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    // heap slots per thread, as the reference sizes them (ImageSmoother.cu:103)
+    const int kalloc = (int)((2 * radius + 1) * (2 * radius + 1) * ratio + 1);
+    int bdx = 16, bdy = 16;
+    auto lds_bytes = [&](int bx, int by) {
+        return sizeof(float) * ((size_t)(bx + 2 * radius) * (by + 2 * radius) + (size_t)2 * kalloc * bx * by);
+    };
+    while (lds_bytes(bdx, bdy) > 64 * 1024 && bdx * bdy > 16) {
+        if (bdx >= bdy) bdx /= 2;
+        else bdy /= 2;
+    }
+    if (lds_bytes(bdx, bdy) > 64 * 1024)
+        return fail(QMC_E_UNSUPPORTED, "qmc_smooth: radius/ratio need more than 64 KiB of LDS per block");
+    const size_t n = (size_t)w * h;
+    if (n > c->scratch_cap) {
+        if ((st = dev_alloc(c->d_scratch, n))) return st;
+        c->scratch_cap = n;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_scratch, img, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
+    dim3 b(bdx, bdy), g((w + bdx - 1) / bdx, (h + bdy - 1) / bdy);
+    hipLaunchKernelGGL(k_smooth, g, b, lds_bytes(bdx, bdy), c->stream, img, c->d_scratch, w, h, radius, ratio, kalloc,
+                       bdx + 2 * radius);
+    HIP_TRY(hipGetLastError());
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_smooth_receiver(qmc_ctx *c, int32_t r, int32_t radius, float ratio) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_smooth_receiver: null ctx");
+    if (r < 0 || r >= (int)c->recvs.size()) return fail(QMC_E_RANGE, "qmc_smooth_receiver: receiver out of range");
+    return qmc_smooth(c, c->recvs[r].flux, c->recvs[r].w, c->recvs[r].h, radius, ratio);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Timing
+// ---------------------------------------------------------------------------------------------
+extern "C" qmc_status qmc_event_record(qmc_ctx *c, int32_t slot) {
+    if (!c || slot < 0 || slot >= 16) return fail(QMC_E_INVALID_ARG, "qmc_event_record: bad slot");
+    HIP_TRY(hipEventRecord(c->ev[slot], c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_event_elapsed_ms(qmc_ctx *c, int32_t a, int32_t b, float *ms) {
+    if (!c || !ms || a < 0 || a >= 16 || b < 0 || b >= 16) return fail(QMC_E_INVALID_ARG, "qmc_event_elapsed_ms: bad slot");
+    HIP_TRY(hipEventSynchronize(c->ev[b]));
+    HIP_TRY(hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return QMC_OK;
+}

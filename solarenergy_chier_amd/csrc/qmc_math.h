@@ -1,0 +1,178 @@
+// qmc_math.h — float3 arithmetic with the exact operation order of the reference tracer,
+// shared by the host scene processor and the gfx950 kernels so both round identically.
+//
+// Semantics restated (not copied) from the reference's vector helpers:
+//   dot(a,b) = a.x*b.x + a.y*b.y + a.z*b.z (left to right), cross = the usual determinant
+//   expansion, normalize(v) = v * (1.0f / sqrtf(dot(v,v))) with a correctly rounded sqrt and
+//   divide (the reference's host rsqrtf, Util/vector_arithmetic.cuh:50-53,397-401),
+//   reflect(i,n) = i - (2*n)*dot(n,i) (Util/vector_arithmetic.cuh:410-413),
+//   float->int conversion truncates toward zero (Util/vector_arithmetic.cuh:571-574).
+// Epsilon is the double 1e-5 (Util/global_constant.h:11): every comparison against it is
+// done in double, as C++ promotion does in the reference.
+//
+// All code including this header must be compiled with -ffp-contract=off and without
+// fast-math, so that no multiply-add is fused and host and device produce the same bits.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define QMC_HD __host__ __device__ __forceinline__
+#else
+#define QMC_HD inline
+#endif
+
+namespace qmc {
+
+constexpr double kEps = 1e-5;               // Util/global_constant.h:11 (a double)
+constexpr float kMathPi = 3.141592654f;      // Util/global_constant.h:12
+constexpr float kIntMaxF = 2147483648.0f;    // float(INT_MAX), rectGridDDA.cuh:17-20
+
+struct V3 {
+    float x, y, z;
+};
+
+QMC_HD V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+QMC_HD V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+QMC_HD V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+QMC_HD V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+QMC_HD V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+QMC_HD V3 operator*(float s, V3 a) { return V3{s * a.x, s * a.y, s * a.z}; }
+QMC_HD V3 operator/(V3 a, float s) { return V3{a.x / s, a.y / s, a.z / s}; }
+QMC_HD V3 operator/(V3 a, V3 b) { return V3{a.x / b.x, a.y / b.y, a.z / b.z}; }
+QMC_HD V3 operator-(V3 a, float s) { return V3{a.x - s, a.y - s, a.z - s}; }
+QMC_HD V3 operator+(V3 a, float s) { return V3{a.x + s, a.y + s, a.z + s}; }
+
+QMC_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+QMC_HD V3 cross(V3 a, V3 b) {
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+QMC_HD float length(V3 v) { return sqrtf(dot(v, v)); }
+QMC_HD V3 normalize(V3 v) {
+    float inv = 1.0f / sqrtf(dot(v, v));
+    return v * inv;
+}
+QMC_HD V3 reflect(V3 i, V3 n) { return i - (2.0f * n) * dot(n, i); }
+
+// 2-D helpers (cylinder receiver works in the xz plane).
+struct V2 {
+    float x, y;
+};
+QMC_HD float dot2(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+QMC_HD V2 normalize2(V2 v) {
+    float inv = 1.0f / sqrtf(dot2(v, v));
+    return V2{v.x * inv, v.y * inv};
+}
+QMC_HD float length2(V2 v) { return sqrtf(dot2(v, v)); }
+
+// Local frame of an "aligned normal": u = normalize((0,1,0) x n), v = normalize(u x n); a local
+// vector d maps to d.x*u + d.y*n + d.z*v. When |n.x| and |n.z| are below Epsilon the map is the
+// identity (Util/global_function.cuh:19-36). The focus-flat variant additionally negates u and v
+// when n.z < 0 (Scene/DataStructure/Heliostat/FocusFlatRectangleHelio.cuh:15-46).
+struct Frame {
+    V3 u, n, v;
+    int identity;
+};
+
+QMC_HD Frame make_frame(V3 n, bool focus_variant) {
+    Frame f;
+    f.n = n;
+    if (fabsf(n.x) < kEps && fabsf(n.z) < kEps) {
+        f.identity = 1;
+        f.u = V3{1.0f, 0.0f, 0.0f};
+        f.v = V3{0.0f, 0.0f, 1.0f};
+        return f;
+    }
+    f.identity = 0;
+    V3 u = normalize(cross(V3{0.0f, 1.0f, 0.0f}, n));
+    V3 v = normalize(cross(u, n));
+    if (focus_variant && n.z < 0) {
+        u = -u;
+        v = -v;
+    }
+    f.u = u;
+    f.v = v;
+    return f;
+}
+
+QMC_HD V3 frame_apply(const Frame &f, V3 d) {
+    if (f.identity) return d;
+    return V3{d.x * f.u.x + d.y * f.n.x + d.z * f.v.x,
+              d.x * f.u.y + d.y * f.n.y + d.z * f.v.y,
+              d.x * f.u.z + d.y * f.n.z + d.z * f.v.z};
+}
+
+QMC_HD V3 local2world_rotate(V3 d, V3 n) { return frame_apply(make_frame(n, false), d); }
+QMC_HD V3 focus_rotate(V3 d, V3 n) { return frame_apply(make_frame(n, true), d); }
+
+// Yaw-only rotation taking old_dir to new_dir (Util/global_function.cuh:51-59).
+QMC_HD V3 rotate_y(V3 p, V3 old_dir, V3 new_dir) {
+    int dir = (cross(old_dir, new_dir).y > 0) ? 1 : -1;
+    float c = dot(old_dir, new_dir);
+    float s = dir * sqrtf(1 - c * c);
+    return V3{c * p.x + s * p.z, p.y, -s * p.x + c * p.z};
+}
+
+// Transcendentals evaluated in double and rounded once to float. Both glibc and the device
+// math library return these double results to within an ulp of double, so the rounded float is
+// the same on host and device (it is the correctly rounded float except in cases closer than
+// ~1e-16 relative to a rounding boundary). Replaces the reference's single-precision expf,
+// acosf, atanf, powf, sinf, cosf at the places where they feed traced geometry.
+QMC_HD float exp_r(float x) { return (float)exp((double)x); }
+QMC_HD float acos_r(float x) { return (float)acos((double)x); }
+QMC_HD float atan_r(float x) { return (float)atan((double)x); }
+QMC_HD float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
+QMC_HD float sin_r(float x) { return (float)sin((double)x); }
+QMC_HD float cos_r(float x) { return (float)cos((double)x); }
+
+// Atmospheric attenuation and per-ray energy (QuasiMonteCarloRayTracing/Receiver/
+// receiverIntersectionUtil.cuh:17-21,31-34).
+QMC_HD float eta_atm(float d) {
+    if (d <= 1000.0f) return 0.99331f - 0.0001176f * d + 1.97f * (1e-8f) * d * d;
+    return exp_r(-0.0001106f * d);
+}
+QMC_HD float ray_energy(float t, V3 dir, V3 normal, float factor) {
+    return fabsf(dot(dir, normal)) * eta_atm(t) * factor;
+}
+
+// Ray vs parallelogram (A, A+E1, A+E2) — Moeller-Trumbore on a parallelogram, the reference's
+// rayParallelogramIntersect (Util/global_function.cuh:101-136) with E1 = B-A and E2 = C-A
+// supplied precomputed (the subtraction is exact-same on host and device).
+QMC_HD bool ray_quad(V3 o, V3 d, V3 A, V3 E1, V3 E2, float &t, float &u, float &v) {
+    V3 p = cross(d, E2);
+    float det = dot(E1, p);
+    if (fabsf(det) < kEps) return false;
+    V3 T = o - A;
+    u = dot(T, p) / det;
+    if (u < 0 || u > 1) return false;
+    V3 q = cross(T, E1);
+    v = dot(d, q) / det;
+    if (v < 0 || v > 1) return false;
+    t = dot(E2, q) / det;
+    if (t < kEps) return false;
+    return true;
+}
+
+// DDA helpers (QuasiMonteCarloRayTracing/Grid/rectGridDDA.cuh:15-32).
+QMC_HD float abs_divide(float den, float num) {
+    if (num <= kEps && num >= -kEps) return kIntMaxF;
+    return fabsf(den / num);
+}
+QMC_HD float cal_tmax(float dir, float interval, int idx, float cur) {
+    return fabsf(float(idx + (dir >= 0)) * interval - cur);
+}
+QMC_HD bool dda_equal(float a, float b) { return (a < b + kEps) && (a > b - kEps); }
+QMC_HD bool dda_less(float a, float b) { return a < b - kEps; }
+
+// Counter-based hashing (splitmix64 finaliser) for seeded offsets and Gaussian slope errors.
+QMC_HD uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// Uniform float in (0, 1] from the top 24 bits of a 32-bit word (exactly representable).
+QMC_HD float u01_open0(uint32_t x) { return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f); }
+
+}  // namespace qmc

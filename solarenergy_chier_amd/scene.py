@@ -1,0 +1,390 @@
+"""Numpy-facing wrappers of the two C-ABI objects: :class:`Scene` (host ``qmc_scene``) and
+:class:`Device` (``qmc_ctx``: one HIP device, one stream, the device copy of a scene)."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _native as N
+
+
+def _f3(v) -> tuple:
+    return tuple(float(x) for x in v)
+
+
+@dataclass
+class ReceiverSpec:
+    """Recv block of a .scn file (Scene/SceneLoader.cpp:104-127) plus the res/tilt extensions."""
+    type: int = 0
+    pos: tuple = (0.0, 0.0, 0.0)
+    size: tuple = (1.0, 1.0, 1.0)
+    normal: tuple = (0.0, 0.0, 1.0)
+    face: int = 0
+    res_x: int = 0
+    res_y: int = 0
+    tilt_deg: float = 0.0
+
+    def to_c(self) -> N.ReceiverDesc:
+        d = N.ReceiverDesc()
+        d.type = self.type
+        d.pos[:] = _f3(self.pos)
+        d.size[:] = _f3(self.size)
+        d.normal[:] = _f3(self.normal)
+        d.face = self.face
+        d.res_x, d.res_y = self.res_x, self.res_y
+        d.tilt_deg = self.tilt_deg
+        return d
+
+
+@dataclass
+class GridSpec:
+    """Grid block (Scene/SceneLoader.cpp:130-159)."""
+    pos: tuple
+    size: tuple
+    interval: tuple
+    start_heliostat: int
+    n_heliostats: int
+    heliostat_type: int = 0
+    receiver: int = 0
+    type: int = 0
+
+    def to_c(self) -> N.GridDesc:
+        d = N.GridDesc()
+        d.type = self.type
+        d.pos[:] = _f3(self.pos)
+        d.size[:] = _f3(self.size)
+        d.interval[:] = _f3(self.interval)
+        d.start_heliostat = self.start_heliostat
+        d.n_heliostats = self.n_heliostats
+        d.heliostat_type = self.heliostat_type
+        d.receiver = self.receiver
+        return d
+
+
+@dataclass
+class HeliostatSpec:
+    """helio line + sticky modifiers (Scene/SceneLoader.cpp:161-198,280-333)."""
+    pos: tuple
+    size: tuple
+    grid: int = 0
+    rows: int = 1
+    cols: int = 1
+    gap: tuple = (0.0, 0.0)
+    surface_property: tuple = (-1.0, -1.0, -1.0, -1.0, -1.0, -1.0)
+    sub_count_override: int = 0
+    local_centers: np.ndarray | None = None
+    local_normals: np.ndarray | None = None
+    _keep: list = field(default_factory=list, repr=False)
+
+    def to_c(self) -> N.HeliostatDesc:
+        d = N.HeliostatDesc()
+        d.pos[:] = _f3(self.pos)
+        d.size[:] = _f3(self.size)
+        d.rows, d.cols = self.rows, self.cols
+        d.gap[:] = _f3(self.gap)
+        d.surface_property[:] = _f3(self.surface_property)
+        d.grid = self.grid
+        d.sub_count_override = self.sub_count_override
+        self._keep.clear()
+        if self.local_centers is not None:
+            a = np.ascontiguousarray(self.local_centers, dtype=np.float32).reshape(-1)
+            self._keep.append(a)
+            d.local_centers = N.fptr(a)
+        if self.local_normals is not None:
+            a = np.ascontiguousarray(self.local_normals, dtype=np.float32).reshape(-1)
+            self._keep.append(a)
+            d.local_normals = N.fptr(a)
+        return d
+
+
+def default_config() -> N.Config:
+    c = N.Config()
+    N.call("qmc_config_default", C.byref(c))
+    return c
+
+
+def make_config(**kw) -> N.Config:
+    c = default_config()
+    for k, v in kw.items():
+        if k == "sun_dir":
+            c.sun_dir[:] = _f3(v)
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class Scene:
+    """Host-side scene (qmc_scene): configuration, receivers, grids, heliostats, processed geometry."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        N.call("qmc_scene_create", C.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            N.lib().qmc_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # construction -----------------------------------------------------------------------------
+    @classmethod
+    def from_files(cls, scn_path: str, json_path: str | None = None, process: bool = True) -> "Scene":
+        s = cls()
+        if json_path:
+            s.load_config(json_path)
+        N.call("qmc_scene_load_file", s.handle, str(scn_path).encode())
+        if process:
+            s.process()
+        return s
+
+    @classmethod
+    def from_specs(cls, config: N.Config, receivers: Sequence[ReceiverSpec], grids: Sequence[GridSpec],
+                   heliostats: Sequence[HeliostatSpec], process: bool = True) -> "Scene":
+        s = cls()
+        s.set_config(config)
+        for r in receivers:
+            N.call("qmc_scene_add_receiver", s.handle, C.byref(r.to_c()))
+        for g in grids:
+            N.call("qmc_scene_add_grid", s.handle, C.byref(g.to_c()))
+        for h in heliostats:
+            d = h.to_c()
+            N.call("qmc_scene_add_heliostat", s.handle, C.byref(d))
+        if process:
+            s.process()
+        return s
+
+    def load_config(self, path: str) -> int:
+        n = C.c_int32(0)
+        N.call("qmc_scene_load_config", self.handle, str(path).encode(), C.byref(n))
+        return n.value
+
+    def set_config(self, cfg: N.Config) -> None:
+        N.call("qmc_scene_set_config", self.handle, C.byref(cfg))
+
+    @property
+    def config(self) -> N.Config:
+        c = N.Config()
+        N.call("qmc_scene_get_config", self.handle, C.byref(c))
+        return c
+
+    def process(self) -> None:
+        N.call("qmc_scene_process", self.handle)
+
+    # inspection -------------------------------------------------------------------------------
+    @property
+    def counts(self) -> N.SceneCounts:
+        c = N.SceneCounts()
+        N.call("qmc_scene_get_counts", self.handle, C.byref(c))
+        return c
+
+    def receiver_desc(self, r: int) -> N.ReceiverDesc:
+        d = N.ReceiverDesc()
+        N.call("qmc_scene_get_receiver_desc", self.handle, r, C.byref(d))
+        return d
+
+    def grid_desc(self, g: int) -> N.GridDesc:
+        d = N.GridDesc()
+        N.call("qmc_scene_get_grid_desc", self.handle, g, C.byref(d))
+        return d
+
+    def heliostat_desc(self, h: int) -> N.HeliostatDesc:
+        d = N.HeliostatDesc()
+        N.call("qmc_scene_get_heliostat_desc", self.handle, h, C.byref(d))
+        return d
+
+    def receiver_info(self, r: int) -> N.ReceiverInfo:
+        o = N.ReceiverInfo()
+        N.call("qmc_scene_get_receiver_info", self.handle, r, C.byref(o))
+        return o
+
+    def heliostat_info(self, h: int) -> N.HeliostatInfo:
+        o = N.HeliostatInfo()
+        N.call("qmc_scene_get_heliostat_info", self.handle, h, C.byref(o))
+        return o
+
+    def grid_info(self, g: int) -> N.GridInfo:
+        o = N.GridInfo()
+        N.call("qmc_scene_get_grid_info", self.handle, g, C.byref(o))
+        return o
+
+    def grid_cells(self, g: int) -> tuple[np.ndarray, np.ndarray]:
+        gi = self.grid_info(g)
+        cs = np.zeros(gi.nx * gi.ny * gi.nz + 1, np.int32)
+        m = np.zeros(max(gi.n_match, 1), np.int32)
+        N.call("qmc_scene_get_grid_cells", self.handle, g, N.iptr(cs), N.iptr(m))
+        return cs, m[: gi.n_match]
+
+    def sub_vertices(self, g: int) -> np.ndarray:
+        n = C.c_int32(0)
+        N.call("qmc_scene_get_subheliostat_vertices", self.handle, g, None, 0, C.byref(n))
+        out = np.zeros((max(n.value, 1), 3), np.float32)
+        N.call("qmc_scene_get_subheliostat_vertices", self.handle, g, N.fptr(out), n.value, C.byref(n))
+        return out[: n.value]
+
+    def factor(self, h: int) -> float:
+        f = C.c_float(0)
+        N.call("qmc_scene_get_factor", self.handle, h, C.byref(f))
+        return f.value
+
+    def check_heliostat(self, h: int) -> None:
+        N.call("qmc_scene_check_heliostat", self.handle, h)
+
+    def load_tasks(self, path: str) -> tuple[list[int], bool]:
+        nh = self.counts.n_heliostats
+        ids = np.zeros(max(nh, 1), np.int32)
+        n = C.c_int32(0)
+        clean = C.c_int32(0)
+        N.call("qmc_task_load", self.handle, str(path).encode(), N.iptr(ids), len(ids), C.byref(n), C.byref(clean))
+        return [int(x) for x in ids[: n.value]], bool(clean.value)
+
+    def specs(self):
+        """Descriptor arrays (C structs) of the loaded scene: (config, receivers, grids, heliostats)."""
+        c = self.counts
+        R = (N.ReceiverDesc * max(c.n_receivers, 1))()
+        G = (N.GridDesc * max(c.n_grids, 1))()
+        H = (N.HeliostatDesc * max(c.n_heliostats, 1))()
+        for i in range(c.n_receivers):
+            R[i] = self.receiver_desc(i)
+        for i in range(c.n_grids):
+            G[i] = self.grid_desc(i)
+        for i in range(c.n_heliostats):
+            H[i] = self.heliostat_desc(i)
+        return self.config, R, c.n_receivers, G, c.n_grids, H, c.n_heliostats
+
+
+class Device:
+    """One HIP device context (qmc_ctx) holding the device copy of one processed scene."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        N.call("qmc_create", device, C.byref(h))
+        self.handle = h
+        self.scene: Scene | None = None
+
+    def close(self):
+        if self.handle:
+            N.lib().qmc_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene: Scene) -> None:
+        N.call("qmc_upload", self.handle, scene.handle)
+        self.scene = scene
+
+    @property
+    def stream(self) -> int:
+        p = C.c_void_p()
+        N.call("qmc_get_stream", self.handle, C.byref(p))
+        return int(p.value or 0)
+
+    def sync(self) -> None:
+        N.call("qmc_sync", self.handle)
+
+    # pools and offsets --------------------------------------------------------------------------
+    def pool_generate(self, seed_sunshape: int = 2, seed_perturbation: int = 4) -> None:
+        p = N.PoolParams(seed_sunshape, seed_perturbation)
+        N.call("qmc_pool_generate", self.handle, C.byref(p))
+
+    def pool_set(self, dirs: np.ndarray, perts: np.ndarray, L: int) -> None:
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        p = np.ascontiguousarray(perts, np.float32).reshape(-1, 3)
+        N.call("qmc_pool_set", self.handle, N.fptr(d), N.fptr(p), d.shape[0], L)
+
+    def pool_from_uniforms(self, ut, up, gx, gz) -> None:
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (ut, up, gx, gz)]
+        N.call("qmc_pool_from_uniforms", self.handle, *[N.fptr(a) for a in arrs])
+
+    def pool_get(self, P: int) -> tuple[np.ndarray, np.ndarray]:
+        d = np.zeros((P, 3), np.float32)
+        p = np.zeros((P, 3), np.float32)
+        N.call("qmc_pool_get", self.handle, N.fptr(d), N.fptr(p), P)
+        return d, p
+
+    def offsets_generate(self, seed: int = 3) -> None:
+        N.call("qmc_offsets_generate", self.handle, C.c_uint64(seed))
+
+    def offsets_set(self, h: int, offsets: np.ndarray) -> None:
+        a = np.ascontiguousarray(offsets, np.int32)
+        N.call("qmc_offsets_set", self.handle, h, N.iptr(a), len(a))
+
+    def offsets_get(self, h: int, F: int) -> np.ndarray:
+        a = np.zeros(F + 1, np.int32)
+        N.call("qmc_offsets_get", self.handle, h, N.iptr(a), F + 1)
+        return a
+
+    def facets(self, h: int, F: int) -> tuple[np.ndarray, np.ndarray]:
+        o = np.zeros((F, 3), np.float32)
+        n = np.zeros((F, 3), np.float32)
+        N.call("qmc_facets_get", self.handle, h, N.fptr(o), N.fptr(n), F)
+        return o, n
+
+    # tracing ---------------------------------------------------------------------------------------
+    def trace(self, ids: Iterable[int]) -> None:
+        a = np.ascontiguousarray(list(ids), np.int32)
+        N.call("qmc_trace", self.handle, N.iptr(a), len(a))
+
+    def trace_records(self, h: int, n_rays: int) -> np.ndarray:
+        rec = np.zeros(n_rays, dtype=N.RECORD_DTYPE)
+        N.call("qmc_trace_records", self.handle, h, rec.ctypes.data_as(C.POINTER(N.RayRecord)), n_rays)
+        return rec
+
+    def trace_stats(self, ids: Iterable[int]) -> dict:
+        a = np.ascontiguousarray(list(ids), np.int32)
+        st = N.TraceStats()
+        N.call("qmc_trace_stats_run", self.handle, N.iptr(a), len(a), C.byref(st))
+        return st.as_dict()
+
+    # flux ---------------------------------------------------------------------------------------------
+    def flux_shape(self, r: int) -> tuple[int, int]:
+        p = N._FP()
+        w = C.c_int32()
+        h = C.c_int32()
+        N.call("qmc_flux_device_ptr", self.handle, r, C.byref(p), C.byref(w), C.byref(h))
+        return h.value, w.value
+
+    def flux(self, r: int = 0) -> np.ndarray:
+        h, w = self.flux_shape(r)
+        out = np.zeros((h, w), np.float32)
+        N.call("qmc_flux_read", self.handle, r, N.fptr(out), out.size)
+        return out
+
+    def flux_write(self, r: int, img: np.ndarray) -> None:
+        a = np.ascontiguousarray(img, np.float32)
+        N.call("qmc_flux_write", self.handle, r, N.fptr(a), a.size)
+
+    def flux_device_ptr(self, r: int = 0) -> int:
+        p = N._FP()
+        N.call("qmc_flux_device_ptr", self.handle, r, C.byref(p), None, None)
+        return C.cast(p, C.c_void_p).value
+
+    def flux_clear(self, r: int = 0) -> None:
+        N.call("qmc_flux_clear", self.handle, r)
+
+    def smooth(self, r: int, radius: int, ratio: float) -> None:
+        N.call("qmc_smooth_receiver", self.handle, r, radius, C.c_float(ratio))
+
+    def smooth_device(self, ptr: int, w: int, h: int, radius: int, ratio: float) -> None:
+        N.call("qmc_smooth", self.handle, C.cast(C.c_void_p(ptr), N._FP), w, h, radius, C.c_float(ratio))
+
+    # timing -------------------------------------------------------------------------------------------
+    def event_record(self, slot: int) -> None:
+        N.call("qmc_event_record", self.handle, slot)
+
+    def event_elapsed_ms(self, a: int, b: int) -> float:
+        ms = C.c_float(0)
+        N.call("qmc_event_elapsed_ms", self.handle, a, b, C.byref(ms))
+        return ms.value

@@ -1,0 +1,154 @@
+"""Synthetic heliostat fields used by the tests, the smoke check and the benchmark.
+
+Coordinates follow the tracer: y is up, x east, z north; ``sun_dir`` is the direction sunlight
+travels. A sun at azimuth ``az`` (clockwise from north) and elevation ``el`` gives
+``sun_dir = -(cos el sin az, sin el, cos el cos az)``.
+
+* :func:`small_field` — 5x5 flat 2x2 m heliostats at 10 m spacing aimed at a 4x4 m receiver at
+  40 m, sun az 180 / el 45, 1600 micro-facets x 64 rays per heliostat (2.56M rays), 128x128 map.
+* :func:`ps10_field` — 624 flat 12.84x9.45 m heliostats in a seeded radial-staggered north field
+  (radii 100-750 m), 12x12 m receiver at 100 m tilted 12.5 deg, sun el 50, 48,384 micro-facets x
+  16 rays per heliostat (483M rays), 256x256 map, 5 m grid cells.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from .scene import GridSpec, HeliostatSpec, ReceiverSpec, make_config
+
+
+def sun_direction(azimuth_deg: float, elevation_deg: float) -> tuple[float, float, float]:
+    az, el = math.radians(azimuth_deg), math.radians(elevation_deg)
+    v = np.array([-math.cos(el) * math.sin(az), -math.sin(el), -math.cos(el) * math.cos(az)], np.float64)
+    v /= np.linalg.norm(v)
+    return tuple(float(np.float32(x)) for x in v)
+
+
+@dataclass
+class Workload:
+    name: str
+    config: object
+    receivers: list
+    grids: list
+    heliostats: list
+    smooth_radius: int = 1
+    smooth_ratio: float = 0.15
+
+    @property
+    def n_heliostats(self) -> int:
+        return len(self.heliostats)
+
+
+def small_field(L: int = 64, groups: int = 1024) -> Workload:
+    cfg = make_config(sun_dir=sun_direction(180.0, 45.0), dni=1000.0, csr=0.1, num_sunshape_groups=groups,
+                      num_sunshape_lights_per_group=L, inverse_transform_sampling_groups=64,
+                      receiver_pixel_length=0.03125, helio_disturb_std=0.002, helio_pixel_length=0.05,
+                      helio_reflected_rate=0.88)
+    recv = ReceiverSpec(type=0, pos=(0.0, 40.0, 0.0), size=(4.0, 4.0, 0.0), normal=(0.0, 0.0, 1.0), face=0)
+    hs = []
+    for x in (-20.0, -10.0, 0.0, 10.0, 20.0):
+        for z in (30.0, 40.0, 50.0, 60.0, 70.0):
+            hs.append(HeliostatSpec(pos=(x, 2.0, z), size=(2.0, 0.1, 2.0), grid=0))
+    grid = GridSpec(pos=(-22.0, 0.0, 28.0), size=(44.0, 4.0, 44.0), interval=(2.0, 2.0, 2.0), start_heliostat=0,
+                    n_heliostats=len(hs), heliostat_type=0, receiver=0)
+    return Workload("small-field 25 heliostats, 2.56M rays, 128x128", cfg, [recv], [grid], hs, 1, 0.15)
+
+
+def ps10_layout(n: int = 624, r_min: float = 100.0, r_max: float = 750.0, span_deg: float = 40.0,
+                seed: int = 1) -> np.ndarray:
+    """Radial-staggered north field: rows every dR metres from r_min, azimuthal pitch chosen so that
+    n heliostats fill the rows up to about r_max; odd rows are shifted by half a pitch; positions get
+    a seeded +-0.5 m jitter. Returns (n, 2) float64 (x, z)."""
+    rng = np.random.default_rng(seed)
+    n_rows = 33
+    d_r = (r_max - r_min - 10.0) / (n_rows - 1)
+    radii = r_min + d_r * np.arange(n_rows)
+    span = math.radians(span_deg)
+    pitch = 2 * span * radii.sum() / n
+    pts = []
+    for i, r in enumerate(radii):
+        k = max(1, int(2 * span * r / pitch))
+        dth = 2 * span / k
+        shift = 0.5 * dth if i % 2 else 0.0
+        for j in range(k):
+            th = -span + (j + 0.5) * dth + shift
+            if th > span:
+                th -= 2 * span
+            pts.append((r * math.sin(th), r * math.cos(th), r))
+    pts.sort(key=lambda p: (p[2], p[0]))
+    while len(pts) < n:  # pad the outermost row when integer rounding left a deficit
+        r = radii[-1] + d_r
+        th = -span + (len(pts) % 17) * (2 * span / 17)
+        pts.append((r * math.sin(th), r * math.cos(th), r))
+    arr = np.array([(p[0], p[1]) for p in pts[:n]], np.float64)
+    arr += rng.uniform(-0.5, 0.5, size=arr.shape)
+    return arr
+
+
+def ps10_field(L: int = 16, groups: int = 4096, n: int = 624, seed: int = 1) -> Workload:
+    cfg = make_config(sun_dir=sun_direction(180.0, 50.0), dni=1000.0, csr=0.05, num_sunshape_groups=groups,
+                      num_sunshape_lights_per_group=L, inverse_transform_sampling_groups=64,
+                      receiver_pixel_length=12.0 / 256.0, helio_disturb_std=0.0015, helio_pixel_length=0.05,
+                      helio_reflected_rate=0.88)
+    recv = ReceiverSpec(type=0, pos=(0.0, 100.0, 0.0), size=(12.0, 12.0, 0.0), normal=(0.0, 0.0, 1.0), face=0,
+                        res_x=256, res_y=256, tilt_deg=12.5)
+    xz = ps10_layout(n=n, seed=seed)
+    hs = [HeliostatSpec(pos=(float(np.float32(x)), 5.5, float(np.float32(z))), size=(12.84, 0.1, 9.45), grid=0)
+          for x, z in xz]
+    cell = 5.0
+    lo_x = math.floor((xz[:, 0].min() - 10.0) / cell) * cell
+    hi_x = math.ceil((xz[:, 0].max() + 10.0) / cell) * cell
+    lo_z = math.floor((xz[:, 1].min() - 10.0) / cell) * cell
+    hi_z = math.ceil((xz[:, 1].max() + 10.0) / cell) * cell
+    grid = GridSpec(pos=(lo_x, 0.0, lo_z), size=(hi_x - lo_x, 15.0, hi_z - lo_z), interval=(cell, cell, cell),
+                    start_heliostat=0, n_heliostats=len(hs), heliostat_type=0, receiver=0)
+    return Workload("PS10-like 624 heliostats, 483M rays, 256x256", cfg, [recv], [grid], hs, 1, 0.15)
+
+
+def _fmt(x: float) -> str:
+    return repr(float(np.float32(x)))
+
+
+def write_scn(path: str | Path, wl: Workload) -> None:
+    """Write the workload in the .scn scene format (Scene/SceneLoader.cpp:243-353). The res/tilt
+    receiver extensions have no .scn keyword and are not written."""
+    lines = ["# generated scene", "ground 1024.0 1024.0", f"ngrid {len(wl.grids)}", ""]
+    written_recv = set()
+    for gi, g in enumerate(wl.grids):
+        if g.receiver not in written_recv:
+            r = wl.receivers[g.receiver]
+            lines += [f"Recv {r.type}", "pos " + " ".join(_fmt(v) for v in r.pos),
+                      "size " + " ".join(_fmt(v) for v in r.size), "norm " + " ".join(_fmt(v) for v in r.normal),
+                      f"face {r.face}", "end", ""]
+            written_recv.add(g.receiver)
+        lines += [f"Grid {g.type}", "pos " + " ".join(_fmt(v) for v in g.pos),
+                  "size " + " ".join(_fmt(v) for v in g.size), "inter " + " ".join(_fmt(v) for v in g.interval),
+                  f"n {g.n_heliostats}", f"type {g.heliostat_type}", "end", ""]
+        for h in wl.heliostats[g.start_heliostat:g.start_heliostat + g.n_heliostats]:
+            lines += [f"gap {_fmt(h.gap[0])} {_fmt(h.gap[1])}", f"matrix {h.rows} {h.cols}",
+                      "surface_property " + " ".join(_fmt(v) for v in h.surface_property),
+                      "helio " + " ".join(_fmt(v) for v in h.pos), "      " + " ".join(_fmt(v) for v in h.size)]
+        lines.append("")
+    Path(path).write_text("\n".join(lines) + "\n")
+
+
+def write_json(path: str | Path, cfg) -> None:
+    import json
+
+    d = {
+        "sun_dir": [float(v) for v in cfg.sun_dir],
+        "dni": float(cfg.dni),
+        "csr": float(cfg.csr),
+        "num_of_sunshape_groups": int(cfg.num_sunshape_groups),
+        "num_per_sunshape_group": int(cfg.num_sunshape_lights_per_group),
+        "inverse_transform_sampling_groups": int(cfg.inverse_transform_sampling_groups),
+        "receiver_pixel_length": float(cfg.receiver_pixel_length),
+        "helio_disturb_std": float(cfg.helio_disturb_std),
+        "helio_reflected_rate": float(cfg.helio_reflected_rate),
+        "helio_pixel_length": float(cfg.helio_pixel_length),
+    }
+    Path(path).write_text(json.dumps(d, indent=2))
