@@ -1,0 +1,192 @@
+"""Benchmark: rays/s of the plain tracer on one GPU, PS10-like field (624 heliostats, 483M rays per
+step, 256x256 flux map), with the trace kernel's algorithmic-byte roofline and the CPU oracle timed
+on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ps10|small] [--no-cpu-baseline]
+
+A step = clear the flux map + trace every heliostat of the field (one kernel launch) with the
+sample pools and micro-facet offsets resident in HBM. Prints ONE JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(st: dict, pool_entries: int = 0) -> int:
+    """Bytes the reference control flow reads/writes with no on-chip reuse (per trace launch):
+    28 B per ray (offset 4 + pool sample 12 + facet origin 12), 28 B per unshadowed ray (facet normal
+    12 + next offset 4 + slope sample 12), 8 B per deposit (atomic read-modify-write), 8 B per DDA
+    cell (two cell-list offsets), 4 B per cell-list entry read, 36 B per non-own quad tested."""
+    return (28 * st["rays"] + 28 * st["unshadowed"] + 8 * st["deposits"] + 8 * st["cells"]
+            + 4 * st["candidates"] + 36 * st["tests"])
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="ps10", choices=["ps10", "small"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build runs on a single MI355X; --gpus {args.gpus} is not "
+              "supported (the field would shard by heliostat ranges with one flux all-reduce).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from solarenergy_chier_amd import Device, Scene, workloads
+
+    wl = workloads.ps10_field() if args.workload == "ps10" else workloads.small_field()
+    log(f"workload: {wl.name}")
+    t0 = time.time()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    dev = Device(0)
+    dev.upload(scene)
+    dev.pool_generate(2, 4)
+    dev.offsets_generate(3)
+    dev.sync()
+    n = scene.counts.n_heliostats
+    ids = list(range(n))
+    cfg = scene.config
+    L = cfg.num_sunshape_lights_per_group
+    P = cfg.num_sunshape_groups * L
+    F = [scene.heliostat_info(h).n_micro for h in ids]
+    rays_per_step = sum(F) * L
+    log(f"scene ready in {time.time() - t0:.1f}s: {n} heliostats, {rays_per_step} rays per step")
+
+    # algorithmic work of one launch (instrumented kernel, untimed)
+    st = dev.trace_stats(ids)
+    alg_bytes = algorithmic_bytes(st)
+    log(f"counted work: {st}; algorithmic bytes per launch {alg_bytes / 1e9:.2f} GB")
+
+    for _ in range(args.warmup):
+        dev.flux_clear(0)
+        dev.trace(ids)
+    dev.sync()
+    log(f"warmup done ({args.warmup} steps)")
+
+    # timed region: K steps, events around each trace launch and around the whole region
+    kernel_ms = 0.0
+    dev.sync()
+    wall0 = time.perf_counter()
+    dev.event_record(0)
+    for k in range(args.steps):
+        dev.flux_clear(0)
+        dev.event_record(2)
+        dev.trace(ids)
+        dev.event_record(3)
+        kernel_ms += dev.event_elapsed_ms(2, 3)
+    dev.event_record(1)
+    dev.sync()
+    wall = time.perf_counter() - wall0
+    total_ms = dev.event_elapsed_ms(0, 1)
+    ms_per_step = total_ms / args.steps
+    kernel_avg_ms = kernel_ms / args.steps
+    rays_per_s = rays_per_step / (ms_per_step / 1e3)
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall:.3f}s), trace kernel {kernel_avg_ms:.3f} ms, "
+        f"{rays_per_s / 1e9:.3f} G rays/s")
+    flux = dev.flux(0)
+    log(f"flux map sum {float(flux.sum(dtype=np.float64)):.6e}, max {float(flux.max()):.3f}")
+
+    achieved = alg_bytes / (kernel_avg_ms / 1e3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / f"trace_{args.workload}_hbm_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, dev, ids, L, P, F, args.cpu_seconds)
+
+    line = {
+        "metric": "rays/s on 1 GPU (PS10-like); batched scenes/s; adjoint:forward cost",
+        "value": rays_per_s,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded field layout, Sobol sun-shape pool, counter-hash slope errors)",
+        "config": {
+            "workload": wl.name,
+            "heliostats": n,
+            "rays_per_step": rays_per_step,
+            "micro_facets_per_heliostat": F[0],
+            "rays_per_facet": L,
+            "pool": P,
+            "flux_map": list(flux.shape),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_trace (rectangle receiver)",
+            "kernel_avg_ms": kernel_avg_ms,
+            "algorithmic_bytes_per_launch": alg_bytes,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    dev.close()
+    return 0
+
+
+def cpu_baseline(scene, dev, ids, L, P, F, seconds):
+    """The oracle (CPU restatement) on a strided sample of the same rays, all host threads (<= 16)."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O  # the checker / CPU baseline only
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    orc = O.OracleScene.from_product_scene(scene)
+    dirs, perts = dev.pool_get(P)
+    offs = np.concatenate([dev.offsets_get(h, F[h]) for h in ids])
+    # calibrate on a tiny sample, then size the sample for about `seconds` of CPU work
+    stride = 4096
+    t = time.perf_counter()
+    _, st = orc.trace_many(ids, dirs, perts, L, offs, nthreads=threads, stride=stride)
+    dt = max(time.perf_counter() - t, 1e-3)
+    rate = st["rays"] / dt
+    want = rate * seconds
+    total = sum(F) * L
+    stride = max(1, int(total / max(want, 1)))
+    log(f"cpu baseline: calibration {rate / 1e6:.2f} M rays/s on {threads} threads; sample stride {stride}")
+    t = time.perf_counter()
+    _, st = orc.trace_many(ids, dirs, perts, L, offs, nthreads=threads, stride=stride)
+    dt = time.perf_counter() - t
+    value = st["rays"] / dt
+    log(f"cpu baseline: {st['rays']} rays in {dt:.2f}s = {value / 1e6:.3f} M rays/s")
+    return {"value": value, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"every {stride}-th ray of all {len(ids)} heliostats ({st['rays']} rays, {dt:.1f} s)"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

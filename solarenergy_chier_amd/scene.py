@@ -100,6 +100,11 @@ class HeliostatSpec:
         return d
 
 
+def _as_c(spec):
+    """A spec dataclass or an already-built C descriptor."""
+    return spec if isinstance(spec, C.Structure) else spec.to_c()
+
+
 def default_config() -> N.Config:
     c = N.Config()
     N.call("qmc_config_default", C.byref(c))
@@ -152,12 +157,11 @@ class Scene:
         s = cls()
         s.set_config(config)
         for r in receivers:
-            N.call("qmc_scene_add_receiver", s.handle, C.byref(r.to_c()))
+            N.call("qmc_scene_add_receiver", s.handle, C.byref(_as_c(r)))
         for g in grids:
-            N.call("qmc_scene_add_grid", s.handle, C.byref(g.to_c()))
+            N.call("qmc_scene_add_grid", s.handle, C.byref(_as_c(g)))
         for h in heliostats:
-            d = h.to_c()
-            N.call("qmc_scene_add_heliostat", s.handle, C.byref(d))
+            N.call("qmc_scene_add_heliostat", s.handle, C.byref(_as_c(h)))
         if process:
             s.process()
         return s

@@ -1,0 +1,76 @@
+"""The product's host scene processing (what qmc_upload packs for the GPU) equals the oracle's
+restatement bit for bit: aim points, normals, world vertices, sub-heliostat quads, grid cell lists,
+receiver geometry, resolutions, energy factors."""
+import numpy as np
+import pytest
+
+import kat_scenes as K
+import oracle_lib as O
+from solarenergy_chier_amd import Scene, workloads
+
+
+def bits(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32)).view(np.uint32)
+
+
+def assert_same_scene(s: Scene, o: O.OracleScene):
+    c = s.counts
+    for r in range(c.n_receivers):
+        a, b = s.receiver_info(r), o.receiver_info(r)
+        assert (a.type, a.res_x, a.res_y) == (b.type, b.res_x, b.res_y)
+        assert bits([a.pixel_length, a.factor_area]).tolist() == bits([b.pixel_length, b.factor_area]).tolist()
+        assert np.array_equal(bits(np.array(a.vertices)), bits(np.array(b.vertices)))
+        assert np.array_equal(bits(list(a.focus_center)), bits(list(b.focus_center)))
+    for g in range(c.n_grids):
+        ga, gb = s.grid_info(g), o.grid_info(g)
+        assert (ga.nx, ga.ny, ga.nz, ga.n_match) == (gb.nx, gb.ny, gb.nz, gb.n_match)
+        ca, ma = s.grid_cells(g)
+        cb, mb = o.grid_cells(g)
+        assert np.array_equal(ca, cb) and np.array_equal(ma, mb)
+        assert np.array_equal(bits(s.sub_vertices(g)), bits(o.sub_vertices(g)))
+    for h in range(c.n_heliostats):
+        a, b = s.heliostat_info(h), o.heliostat_info(h)
+        for f in ("normal", "aim"):
+            assert np.array_equal(bits(list(getattr(a, f))), bits(list(getattr(b, f)))), (h, f)
+        assert np.array_equal(bits(np.array(a.vertices)), bits(np.array(b.vertices))), h
+        assert (a.n_micro, a.sub_rows, a.sub_cols, a.sub_begin, a.sub_count, a.type) == \
+               (b.n_micro, b.sub_rows, b.sub_cols, b.sub_begin, b.sub_count, b.type), h
+        assert bits([a.focal_length]) == bits([b.focal_length])
+        assert bits([s.factor(h)]) == bits([o.factor(h)]), h
+
+
+def test_small_field():
+    wl = workloads.small_field()
+    s = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    assert_same_scene(s, O.OracleScene.from_workload(wl))
+
+
+def test_ps10_field():
+    wl = workloads.ps10_field()
+    s = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    ri = s.receiver_info(0)
+    assert (ri.res_x, ri.res_y) == (256, 256)
+    assert s.counts.n_heliostats == 624
+    assert s.heliostat_info(0).n_micro == 189 * 256
+    assert_same_scene(s, O.OracleScene.from_workload(wl))
+
+
+@pytest.mark.parametrize("scn,cfg", [
+    ("qmcrt_test_scene.scn", "qmcrt_test_configuration.json"),
+    ("qmcrt_test_scene_cylinder_receiver.scn", "qmcrt_test_configuration2.json"),
+    ("ds_test_scene_good.scn", "ds_test_configuration.json"),
+    ("ds_test_scene_good_for_focusFlatRectangleHelio.scn", "ds_test_configuration.json"),
+    ("inputs/PS10_typical/PS10_typical_4x7_scene.scn", "inputs/PS10_typical/PS10_typical_4x7_configuration.json"),
+    ("inputs/CompleteInputFilesExample/twoReceiverScene.scn",
+     "inputs/CompleteInputFilesExample/twoReceiverConfiguration.json"),
+    ("inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_scene.scn",
+     "inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_configuration.json"),
+])
+def test_reference_scene_files(scn, cfg):
+    s = Scene.from_files(str(K.FIX / scn), str(K.FIX / cfg))
+    assert_same_scene(s, O.OracleScene.from_product_scene(s))
+
+
+def test_focus_kat_scene():
+    s = Scene.from_specs(*K.focus_helio_scene())
+    assert_same_scene(s, O.OracleScene(*K.focus_helio_scene()))
