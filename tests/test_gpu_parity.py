@@ -86,3 +86,176 @@ def test_small_field_batched_trace_equals_records(device):
     fc, cst = orc.trace_many(list(range(n)), dirs, perts, L, offs, nthreads=8)
     assert st == cst
     assert rel_l1(a, fc) < FLUX_RTOL_L1
+
+
+def scene_from_files(scn, cfg):
+    import kat_scenes as K
+    return Scene.from_files(str(K.FIX / scn), str(K.FIX / cfg))
+
+
+def parity_on_scene(device, scene, heliostats, pools=None, seeds=(2, 4, 3)):
+    """Trace each heliostat on the GPU with records and compare to the oracle; returns mismatches."""
+    device.upload(scene)
+    cfg = scene.config
+    L = cfg.num_sunshape_lights_per_group
+    P = cfg.num_sunshape_groups * L
+    if pools is None:
+        device.pool_generate(seeds[0], seeds[1])
+    else:
+        device.pool_set(pools[0], pools[1], L)
+    device.offsets_generate(seeds[2])
+    dirs, perts = device.pool_get(P)
+    orc = O.OracleScene.from_product_scene(scene)
+    bad = 0
+    for h in heliostats:
+        g = scene.heliostat_info(h).grid
+        r = scene.grid_desc(g).receiver
+        F = scene.heliostat_info(h).n_micro
+        offs = device.offsets_get(h, F)
+        device.flux_clear(r)
+        rec = device.trace_records(h, F * L)
+        flux = device.flux(r)
+        cflux, crec, _ = orc.trace_heliostat(h, dirs, perts, L, offs)
+        b = compare_records(rec, crec)
+        print(f"heliostat {h}: {b} mismatches of {F * L}; deposited {(rec['status'] == 3).sum()}", flush=True)
+        bad += b
+        if cflux.sum() > 0:
+            assert rel_l1(flux, cflux) < FLUX_RTOL_L1
+        else:
+            assert flux.sum() == 0
+    return bad
+
+
+def test_cylinder_reference_scene_parallel_and_sampled(device):
+    # Tests/QuasiMonteCarloRayTracingTests/cylinderReceiverIntersectionText.cpp: three grids on one
+    # cylinder receiver; heliostat 2 of grid 2 shadows heliostat 1
+    scene = scene_from_files("qmcrt_test_scene_cylinder_receiver.scn", "qmcrt_test_configuration2.json")
+    cfg = scene.config
+    P = cfg.num_sunshape_groups * cfg.num_sunshape_lights_per_group
+    par = np.tile([0.0, 1.0, 0.0], (P, 1)).astype(np.float32)
+    n = scene.counts.n_heliostats
+    assert parity_on_scene(device, scene, range(n), pools=(par, par)) == 0
+    assert parity_on_scene(device, scene, range(n)) == 0
+
+
+def test_rectangle_reference_scene(device):
+    scene = scene_from_files("qmcrt_test_scene.scn", "qmcrt_test_configuration.json")
+    assert parity_on_scene(device, scene, range(scene.counts.n_heliostats)) == 0
+
+
+def test_focus_flat_reference_scenes(device):
+    for scn, cfg in [("inputs/PS10_typical/PS10_typical_4x7_scene.scn",
+                      "inputs/PS10_typical/PS10_typical_4x7_configuration.json"),
+                     ("inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_scene.scn",
+                      "inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_configuration.json")]:
+        scene = scene_from_files(scn, cfg)
+        assert parity_on_scene(device, scene, range(scene.counts.n_heliostats)) == 0
+
+
+def test_two_receiver_scene(device):
+    scene = scene_from_files("inputs/CompleteInputFilesExample/twoReceiverScene.scn",
+                             "inputs/CompleteInputFilesExample/twoReceiverConfiguration.json")
+    n = scene.counts.n_heliostats
+    assert parity_on_scene(device, scene, range(min(n, 6))) == 0
+
+
+def test_ps10_field_sampled_heliostats(device):
+    wl = workloads.ps10_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    # innermost, a mid-field and the outermost heliostats (blocking varies with radius)
+    assert parity_on_scene(device, scene, [0, 17, 300, 623]) == 0
+
+
+def test_facets_kernel_bit_exact(device):
+    import kat_scenes as K
+    for specs in (K.rectangle_helio_scene(), K.focus_helio_scene()):
+        scene = Scene.from_specs(*specs)
+        device.upload(scene)
+        orc = O.OracleScene(*specs)
+        F = scene.heliostat_info(0).n_micro
+        o, n = device.facets(0, F)
+        co, cn = orc.facets(0)
+        assert np.array_equal(o.view(np.uint32), co.view(np.uint32))
+        assert np.array_equal(n.view(np.uint32), cn.view(np.uint32))
+    np.testing.assert_allclose(device.facets(0, 12)[0], K.focus_expected()[1], atol=1e-3)
+    wl = workloads.ps10_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    device.upload(scene)
+    orc = O.OracleScene.from_workload(wl)
+    for h in (0, 311, 623):
+        F = scene.heliostat_info(h).n_micro
+        o, n = device.facets(h, F)
+        co, cn = orc.facets(h)
+        assert np.array_equal(o.view(np.uint32), co.view(np.uint32)), h
+        assert np.array_equal(n.view(np.uint32), cn.view(np.uint32)), h
+
+
+def test_pool_and_offsets_generation_match_oracle(device):
+    wl = workloads.small_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    device.upload(scene)
+    device.pool_generate(2, 4)
+    cfg = scene.config
+    P = cfg.num_sunshape_groups * cfg.num_sunshape_lights_per_group
+    d, p = device.pool_get(P)
+    cd, cp, (ut, up, gx, gz) = O.pool_generate(cfg, 2, 4)
+    # transcendentals are double-evaluated and rounded once on both sides: expect identity, allow 1 ulp
+    for a, b in ((d, cd), (p, cp)):
+        ulp = np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+        print("pool max ulp", int(ulp.max()), "differing", int((ulp > 0).sum()))
+        assert ulp.max() <= 1
+    device.pool_from_uniforms(ut, up, gx, gz)
+    d2, p2 = device.pool_get(P)
+    assert np.abs(d2.view(np.int32).astype(np.int64) - cd.view(np.int32).astype(np.int64)).max() <= 1
+    assert np.array_equal(p2.view(np.uint32), cp.view(np.uint32))
+    device.offsets_generate(3)
+    orc = O.OracleScene.from_workload(wl)
+    for h in (0, 24):
+        F = scene.heliostat_info(h).n_micro
+        assert np.array_equal(device.offsets_get(h, F), orc.offsets(3, P, h))
+
+
+@pytest.mark.parametrize("shape,r,ratio", [((128, 128), 1, 0.15), ((256, 512), 5, 0.05), ((37, 53), 2, 0.1),
+                                            ((7, 5), 4, 0.2)])
+def test_smoother_bit_exact(device, shape, r, ratio):
+    from solarenergy_chier_amd import ReceiverSpec
+    wl = workloads.small_field()
+    h, w = shape
+    rs = ReceiverSpec(type=0, pos=(0.0, 40.0, 0.0), size=(4.0, 4.0, 0.0), res_x=w, res_y=h)
+    device.upload(Scene.from_specs(wl.config, [rs], wl.grids, wl.heliostats))
+    rng = np.random.default_rng(7)
+    img = rng.gamma(2.0, 300.0, size=shape).astype(np.float32)
+    img[rng.random(shape) < 0.3] = 0.0
+    device.flux_write(0, img)
+    device.smooth(0, r, ratio)
+    got = device.flux(0)
+    want = O.smooth(img, r, ratio)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_ps10_full_size_properties(device):
+    """Full 483M-ray field: counts consistent, trace is additive over heliostat subsets and repeatable."""
+    wl = workloads.ps10_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    device.upload(scene)
+    device.pool_generate(2, 4)
+    device.offsets_generate(3)
+    n = scene.counts.n_heliostats
+    device.flux_clear(0)
+    st = device.trace_stats(range(n))
+    full = device.flux(0).astype(np.float64)
+    assert st["rays"] == n * 48384 * 16 and st["facets"] == n * 48384
+    assert st["rays"] >= st["unshadowed"] >= st["unblocked"] >= st["deposits"] > 0
+    device.flux_clear(0)
+    device.trace(range(0, n, 2))
+    a = device.flux(0).astype(np.float64)
+    device.flux_clear(0)
+    device.trace(range(1, n, 2))
+    b = device.flux(0).astype(np.float64)
+    assert rel_l1(a + b, full) < FLUX_RTOL_L1
+    device.flux_clear(0)
+    device.trace(range(n))
+    device.trace(range(n))
+    assert rel_l1(device.flux(0) / 2.0, full) < FLUX_RTOL_L1
+    st2 = device.trace_stats(range(n))
+    assert st2 == st
