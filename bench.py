@@ -43,7 +43,11 @@ def main() -> int:
     ap.add_argument("--workload", default="ps10", choices=["ps10", "small"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (calibrates HBM byte counters), then exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build runs on a single MI355X; --gpus {args.gpus} is not "
               "supported (the field would shard by heliostat ranges with one flux all-reduce).", file=sys.stderr)
@@ -169,16 +173,21 @@ def cpu_baseline(scene, dev, ids, L, P, F, seconds):
     orc = O.OracleScene.from_product_scene(scene)
     dirs, perts = dev.pool_get(P)
     offs = np.concatenate([dev.offsets_get(h, F[h]) for h in ids])
-    # calibrate on a tiny sample, then size the sample for about `seconds` of CPU work
-    stride = 4096
-    t = time.perf_counter()
-    _, st = orc.trace_many(ids, dirs, perts, L, offs, nthreads=threads, stride=stride)
-    dt = max(time.perf_counter() - t, 1e-3)
-    rate = st["rays"] / dt
-    want = rate * seconds
+    # two calibration samples separate the per-heliostat facet set-up from the per-ray cost, then
+    # the stride is chosen for about `seconds` of CPU work
     total = sum(F) * L
-    stride = max(1, int(total / max(want, 1)))
-    log(f"cpu baseline: calibration {rate / 1e6:.2f} M rays/s on {threads} threads; sample stride {stride}")
+    runs = []
+    for stride in (4096, 1024):
+        t = time.perf_counter()
+        _, st = orc.trace_many(ids, dirs, perts, L, offs, nthreads=threads, stride=stride)
+        runs.append((st["rays"], time.perf_counter() - t))
+    (r1, t1), (r2, t2) = runs
+    rate = (r2 - r1) / max(t2 - t1, 1e-6)
+    setup = max(t1 - r1 / rate, 0.0)
+    want = max(rate * max(seconds - setup, 1.0), 1.0)
+    stride = max(1, int(total / want))
+    log(f"cpu baseline: calibration {rate / 1e6:.2f} M rays/s on {threads} threads (+{setup:.2f}s set-up); "
+        f"sample stride {stride}")
     t = time.perf_counter()
     _, st = orc.trace_many(ids, dirs, perts, L, offs, nthreads=threads, stride=stride)
     dt = time.perf_counter() - t
@@ -186,6 +195,20 @@ def cpu_baseline(scene, dev, ids, L, P, F, seconds):
     log(f"cpu baseline: {st['rays']} rays in {dt:.2f}s = {value / 1e6:.3f} M rays/s")
     return {"value": value, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"every {stride}-th ray of all {len(ids)} heliostats ({st['rays']} rays, {dt:.1f} s)"}
+
+
+def copy_probe(mib: int) -> int:
+    """Device-to-device copy of a known size (HBM counter calibration for rocprofv3 --pmc runs)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: {mib} MiB read + {mib} MiB written")
+    return 0
 
 
 if __name__ == "__main__":
