@@ -99,6 +99,10 @@ typedef struct qmc_heliostat_desc {
     int32_t sub_count_override;/* 0: type default; >0: sub-heliostat count used by the grid
                                   cell lists (reference Heliostat::setNumberOfSubHelio,
                                   Scene/DataStructure/Heliostat/Heliostat.cuh:26-28) */
+    float aim_offset[2];       /* extension: aim point = receiver focus centre + offset in the
+                                  receiver surface ([0] across: rectangle width / cylinder
+                                  tangent, [1] up: rectangle height / cylinder axis), metres;
+                                  {0, 0} keeps the reference aim exactly */
     const float *local_centers;/* read method 1: rows*cols*3 floats, else NULL */
     const float *local_normals;/* read method 1: rows*cols*3 floats, else NULL */
 } qmc_heliostat_desc;

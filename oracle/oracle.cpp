@@ -669,6 +669,19 @@ extern "C" orc_scene *orc_scene_build(const qmc_config *cfg, const qmc_receiver_
         for (int i = 0; i < g.d.n_heliostats; ++i) {
             Helio &h = s->H[g.d.start_heliostat + i];
             f3 fc = (r.d.type == 0) ? r.focus : cyl_focus(r.d.pos, r.d.size, load3(h.d.pos));
+            if (h.d.aim_offset[0] != 0.0f || h.d.aim_offset[1] != 0.0f) {
+                // aim-offset extension: focus + off0 * across + off1 * up in the receiver surface
+                f3 across, up;
+                if (r.d.type == 0) {
+                    across = normalize(sub(r.v[3], r.v[0]));
+                    up = normalize(sub(r.v[1], r.v[0]));
+                } else {
+                    f3 rad = sub(fc, load3(r.d.pos));
+                    across = (rad.x == 0.0f && rad.z == 0.0f) ? mk(1.0f, 0.0f, 0.0f) : normalize(mk(-rad.z, 0.0f, rad.x));
+                    up = mk(0.0f, 1.0f, 0.0f);
+                }
+                fc = add(add(fc, scale(across, h.d.aim_offset[0])), scale(up, h.d.aim_offset[1]));
+            }
             helio_process(h, fc, sun, cfg->helio_pixel_length);
         }
         // QuasiMonteCarloRayTracer::setFlatRectangleHeliostatVertexs and the subHeliostat_id sums

@@ -46,7 +46,8 @@ class GridDesc(C.Structure):
 class HeliostatDesc(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("size", C.c_float * 3), ("rows", C.c_int32), ("cols", C.c_int32),
                 ("gap", C.c_float * 2), ("surface_property", C.c_float * 6), ("grid", C.c_int32),
-                ("sub_count_override", C.c_int32), ("local_centers", C.POINTER(C.c_float)),
+                ("sub_count_override", C.c_int32), ("aim_offset", C.c_float * 2),
+                ("local_centers", C.POINTER(C.c_float)),
                 ("local_normals", C.POINTER(C.c_float))]
 
 

@@ -1,0 +1,147 @@
+// qmc_ctx.h — the device context behind qmc_ctx and the host helpers shared by the .hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "qmc_device.h"
+#include "scene_host.h"
+
+namespace qmc {
+
+struct GridBuf {
+    DevGrid g{};
+    int32_t *cell_start = nullptr;
+    int32_t *match = nullptr;
+    float4 *quads = nullptr;
+    int n_sub = 0;
+    int receiver = 0;
+};
+
+struct RecvBuf {
+    DevReceiver r{};
+    float *flux = nullptr;
+    int w = 0, h = 0;
+};
+
+}  // namespace qmc
+
+struct qmc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[16] = {};
+    bool uploaded = false;
+    qmc_config cfg{};
+    std::vector<qmc::GridBuf> grids;
+    std::vector<qmc::RecvBuf> recvs;
+    std::vector<qmc::DevHelio> h_helios;
+    qmc::DevHelio *d_helios = nullptr;
+    qmc::DevFacet *d_facets = nullptr;
+    int32_t *d_offsets = nullptr;
+    int64_t offsets_total = 0;
+    float4 *d_pool = nullptr;
+    float4 *d_pert = nullptr;
+    int P = 0, L = 0;
+    bool pool_ready = false, offsets_ready = false;
+    int32_t *d_list = nullptr;
+    int list_cap = 0;
+    qmc::StatsAcc *d_stats = nullptr;
+    qmc_ray_record *d_rec = nullptr;
+    size_t rec_cap = 0;
+    float *d_scratch = nullptr;
+    size_t scratch_cap = 0;
+    qmc::DevFrame sun{};
+    float2 *d_cdf = nullptr;
+    float *d_slope = nullptr;
+    qmc::BuieArgs buie{};
+    // batched traces (sun x aim instances)
+    float *d_aims = nullptr;       // aim points, n_aim_sets * n_heliostats * 3
+    int n_aim_sets = 0;
+    float *d_batch_maps = nullptr; // ctx-owned per-instance flux maps
+    size_t batch_cap = 0;          // floats
+    void *d_batch_inst = nullptr;  // instance table
+    size_t batch_inst_cap = 0;
+    int32_t batch_n = 0;           // instances of the last batched trace
+    int32_t batch_npix = 0;
+    int32_t batch_grid = 0;
+    int num_cus = 0;
+    size_t max_lds = 0;
+    // adjoint
+    void *adj = nullptr;
+};
+
+namespace qmc {
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(QMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+template <class T>
+inline qmc_status dev_alloc(T *&p, size_t count) {
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+    }
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+        p = nullptr;
+        return fail(e == hipErrorOutOfMemory ? QMC_E_OOM : QMC_E_HIP,
+                    std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " B): " + hipGetErrorString(e));
+    }
+    return QMC_OK;
+}
+
+template <class T>
+inline void dev_free(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+inline DevFrame to_dev(const Frame &f) {
+    DevFrame d;
+    d.ux = f.u.x; d.uy = f.u.y; d.uz = f.u.z;
+    d.nx = f.n.x; d.ny = f.n.y; d.nz = f.n.z;
+    d.vx = f.v.x; d.vy = f.v.y; d.vz = f.v.z;
+    d.identity = f.identity;
+    return d;
+}
+
+inline void free_scene(qmc_ctx *c) {
+    for (auto &g : c->grids) {
+        dev_free(g.cell_start);
+        dev_free(g.match);
+        dev_free(g.quads);
+    }
+    c->grids.clear();
+    for (auto &r : c->recvs) dev_free(r.flux);
+    c->recvs.clear();
+    dev_free(c->d_helios);
+    dev_free(c->d_facets);
+    dev_free(c->d_offsets);
+    c->h_helios.clear();
+    c->uploaded = false;
+    c->offsets_ready = false;
+}
+
+inline qmc_status set_device(qmc_ctx *c) {
+    HIP_TRY(hipSetDevice(c->device));
+    return QMC_OK;
+}
+
+// Sobol direction numbers of the first two dimensions (dimension 0: van der Corput; dimension 1:
+// primitive polynomial x + 1 with m_1 = 1).
+inline void sobol_directions(uint32_t v0[32], uint32_t v1[32]) {
+    uint32_t m[33];
+    for (int k = 0; k < 32; ++k) v0[k] = 1u << (31 - k);
+    m[1] = 1;
+    for (int k = 2; k <= 32; ++k) m[k] = (m[k - 1] << 1) ^ m[k - 1];
+    for (int k = 1; k <= 32; ++k) v1[k - 1] = m[k] << (32 - k);
+}
+
+}  // namespace qmc

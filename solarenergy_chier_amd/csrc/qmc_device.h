@@ -69,6 +69,14 @@ struct DevGrid {
     const float4 *quads;
 };
 
+// Buie sun-shape table on the device (Scene/SceneProcessor.cu:23-44,237-239)
+struct BuieArgs {
+    const float2 *cdf;   // n+1 (theta, cumulative)
+    const float *slope;  // n
+    int n;
+    float A, B, C, I, gamma;
+};
+
 struct StatsAcc {
     unsigned long long rays, unshadowed, unblocked, deposits, cells, candidates, tests, facets, edge;
 };

@@ -23,8 +23,8 @@
 #include <string>
 #include <vector>
 
-#include "qmc_device.h"
-#include "scene_host.h"
+#include "qmc_ctx.h"
+#include "qmc_trace_core.h"
 
 using namespace qmc;
 
@@ -32,137 +32,6 @@ using namespace qmc;
 // Device helpers
 // ---------------------------------------------------------------------------------------------
 namespace {
-
-__device__ __forceinline__ V3 fr_apply(const DevFrame &f, V3 d) {
-    if (f.identity) return d;
-    return V3{d.x * f.ux + d.y * f.nx + d.z * f.vx,
-              d.x * f.uy + d.y * f.ny + d.z * f.vy,
-              d.x * f.uz + d.y * f.nz + d.z * f.vz};
-}
-
-struct RayStats {
-    unsigned cells, candidates, tests;
-};
-
-// 3D-DDA over the uniform grid; true when a non-own sub-heliostat quad is hit (first hit in
-// traversal and cell-list order; its id goes to hit_id).
-template <bool STATS>
-__device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, int own_lo, int own_hi,
-                                              int &hit_id, RayStats &st) {
-    const V3 gpos{g.pos[0], g.pos[1], g.pos[2]};
-    const V3 gint{g.interval[0], g.interval[1], g.interval[2]};
-    const V3 cell = (o - gpos) / gint;
-    int ix = (int)cell.x, iy = (int)cell.y, iz = (int)cell.z;
-    const int sx = (d.x >= 0) ? 1 : -1;
-    const int sy = (d.y >= 0) ? 1 : -1;
-    const int sz = (d.z >= 0) ? 1 : -1;
-    float tx = abs_divide(cal_tmax(d.x, gint.x, ix, o.x - gpos.x), d.x);
-    float ty = abs_divide(cal_tmax(d.y, gint.y, iy, o.y - gpos.y), d.y);
-    float tz = abs_divide(cal_tmax(d.z, gint.z, iz, o.z - gpos.z), d.z);
-    const float dx = abs_divide(gint.x, d.x);
-    const float dy = abs_divide(gint.y, d.y);
-    const float dz = abs_divide(gint.z, d.z);
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    while (ix >= 0 && ix < nx && iy >= 0 && iy < ny && iz >= 0 && iz < nz) {
-        const int addr = ix * ny * nz + iy * nz + iz;
-        const int b = g.cell_start[addr], e = g.cell_start[addr + 1];
-        if (STATS) {
-            st.cells += 1;
-            st.candidates += (unsigned)(e - b);
-        }
-        for (int i = b; i < e; ++i) {
-            const int id = g.match[i];
-            if (id < own_lo || id >= own_hi) {
-                if (STATS) st.tests += 1;
-                const float4 qa = g.quads[3 * id];
-                const float4 qb = g.quads[3 * id + 1];
-                const float4 qc = g.quads[3 * id + 2];
-                float t, u, v;
-                if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) {
-                    hit_id = id;
-                    return true;
-                }
-            }
-        }
-        bool fx = false, fy = false, fz = false;
-        if (dda_less(tx, ty)) {
-            if (dda_less(tx, tz) || dda_equal(tx, tz)) { ix += sx; fx = true; }
-            if (dda_less(tz, tx) || dda_equal(tx, tz)) { iz += sz; fz = true; }
-        } else if (dda_less(ty, tx)) {
-            if (dda_less(ty, tz) || dda_equal(ty, tz)) { iy += sy; fy = true; }
-            if (dda_less(tz, ty) || dda_equal(ty, tz)) { iz += sz; fz = true; }
-        } else {
-            if (dda_less(tx, tz) || dda_equal(tx, tz)) { ix += sx; iy += sy; fx = fy = true; }
-            if (dda_less(tz, tx) || dda_equal(tx, tz)) { iz += sz; fz = true; }
-        }
-        tx += fx ? dx : 0.0f;
-        ty += fy ? dy : 0.0f;
-        tz += fz ? dz : 0.0f;
-    }
-    return false;
-}
-
-// Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
-// onto the facet normal.
-__device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
-                                               const DevFrame *&pert_frame, V3 &normal) {
-    const int cpr = H.cols * H.sub_cols;
-    int row = m / cpr;
-    int col = m % cpr;
-    const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
-    if (H.type == 0) {
-        // RectangleHelio.cu:64-79
-        const int brow = row / H.sub_rows;
-        const int bcol = col / H.sub_cols;
-        const float pl = H.pl;
-        V3 local{col * pl + bcol * H.gap[0] + pl / 2 - H.size[0] / 2, H.size[1] / 2,
-                 row * pl + brow * H.gap[1] + pl / 2 - H.size[2] / 2};
-        origin = fr_apply(H.fr, local) + pos;
-        pert_frame = &H.fr;
-        normal = V3{H.fr.nx, H.fr.ny, H.fr.nz};
-    } else {
-        // FocusFlatRectangleHelio.cu:21-39,53-64
-        const int block = (row / H.sub_rows) * H.cols + (col / H.sub_cols);
-        const DevFacet &Fc = facets[H.facet_base + block];
-        row %= H.sub_rows;
-        col %= H.sub_cols;
-        const float pl = H.pl;
-        V3 local{(col + 0.5f) * pl - H.len_col / 2, 0.0f, (row + 0.5f) * pl - H.len_row / 2};
-        local = fr_apply(Fc.fr_local, local) + V3{Fc.center[0], Fc.center[1], Fc.center[2]};
-        origin = fr_apply(H.fr_focus, local) + pos;
-        pert_frame = &Fc.fr_pert;
-        normal = V3{Fc.normal[0], Fc.normal[1], Fc.normal[2]};
-    }
-}
-
-// Receiver hit: true with (t, u, v) when the ray reaches the receiver surface.
-template <int RT>
-__device__ __forceinline__ bool receiver_hit(const DevReceiver &R, V3 o, V3 d, float &t, float &u, float &v) {
-    if (RT == 0) {
-        // RectangleReceiver::GIntersect (RectangleReceiver.cuh:32-34)
-        return ray_quad(o, d, V3{R.A[0], R.A[1], R.A[2]}, V3{R.E1[0], R.E1[1], R.E1[2]},
-                        V3{R.E2[0], R.E2[1], R.E2[2]}, t, u, v);
-    } else {
-        // CylinderReceiver::GIntersect (CylinderReceiver.cuh:39-89)
-        const float R2 = R.radius * R.radius;
-        V2 l{o.x - R.pos[0], o.z - R.pos[2]};
-        if (dot2(l, l) <= R2) return false;
-        V2 Ro{R.pos[0] - o.x, R.pos[2] - o.z};
-        float tp = dot2(Ro, normalize2(V2{d.x, d.z}));
-        float delta = dot2(Ro, Ro) - tp * tp;
-        if (tp < -kEps || delta > R2) return false;
-        float t_plus = delta <= 0.0f ? R.radius : sqrtf(R2 - delta);
-        t = (tp - t_plus) / length2(V2{d.x, d.z});
-        V3 ip = t * d + o;
-        u = (ip.y - R.pos[1]) / R.height + 0.5f;
-        if (u < 0.0f || u > 1.0f) return false;
-        V2 c = normalize2(V2{ip.x - R.pos[0], ip.z - R.pos[2]});
-        float cx = fminf(fmaxf(c.x, -1.0f), 1.0f);
-        v = (float)((double)acos_r(cx) / (2 * M_PI));
-        if (c.y < 0) v = 1 - v;
-        return true;
-    }
-}
 
 template <int RT, bool REC, bool STATS>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
@@ -201,7 +70,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         int index = -1;
         float energy = 0.0f;
         int flags = 0;
-        if (dda_collision<STATS>(origin, dir, a.grid, own_lo, own_hi, hit, st)) {
+        if (dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
             status = QMC_RAY_SHADOWED;
             index = hit;
         } else {
@@ -211,7 +80,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             const float4 q = a.pert[a2];
             V3 normal = normalize(fr_apply(*pf, V3{q.x, q.y, q.z}));
             dir = normalize(reflect(-dir, normal));
-            if (dda_collision<STATS>(origin, dir, a.grid, own_lo, own_hi, hit, st)) {
+            if (dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
                 status = QMC_RAY_BLOCKED;
                 index = hit;
             } else {
@@ -270,12 +139,6 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
 // ---------------------------------------------------------------------------------------------
 // Sample pools
 // ---------------------------------------------------------------------------------------------
-struct BuieArgs {
-    const float2 *cdf;   // n+1 (theta, cumulative)
-    const float *slope;  // n
-    int n;
-    float A, B, C, I, gamma;
-};
 
 // samplelights::max_less_index (Scene/SceneProcessor.cu:46-59)
 __device__ __forceinline__ int max_less_index(const float2 *cdf, float value, int n) {
@@ -496,129 +359,6 @@ __global__ void k_smooth(float *out, const float *src, int width, int height, in
 // ---------------------------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------------------------
-namespace {
-
-struct GridBuf {
-    DevGrid g{};
-    int32_t *cell_start = nullptr;
-    int32_t *match = nullptr;
-    float4 *quads = nullptr;
-    int n_sub = 0;
-    int receiver = 0;
-};
-
-struct RecvBuf {
-    DevReceiver r{};
-    float *flux = nullptr;
-    int w = 0, h = 0;
-};
-
-}  // namespace
-
-struct qmc_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[16] = {};
-    bool uploaded = false;
-    qmc_config cfg{};
-    std::vector<GridBuf> grids;
-    std::vector<RecvBuf> recvs;
-    std::vector<DevHelio> h_helios;
-    DevHelio *d_helios = nullptr;
-    DevFacet *d_facets = nullptr;
-    int32_t *d_offsets = nullptr;
-    int64_t offsets_total = 0;
-    float4 *d_pool = nullptr;
-    float4 *d_pert = nullptr;
-    int P = 0, L = 0;
-    bool pool_ready = false, offsets_ready = false;
-    int32_t *d_list = nullptr;
-    int list_cap = 0;
-    StatsAcc *d_stats = nullptr;
-    qmc_ray_record *d_rec = nullptr;
-    size_t rec_cap = 0;
-    float *d_scratch = nullptr;
-    size_t scratch_cap = 0;
-    DevFrame sun{};
-    float2 *d_cdf = nullptr;
-    float *d_slope = nullptr;
-    BuieArgs buie{};
-};
-
-namespace {
-
-#define HIP_TRY(expr)                                                                                  \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess)                                                                          \
-            return fail(QMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                 \
-    } while (0)
-
-template <class T>
-qmc_status dev_alloc(T *&p, size_t count) {
-    if (p) {
-        (void)hipFree(p);
-        p = nullptr;
-    }
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
-    if (e != hipSuccess) {
-        p = nullptr;
-        return fail(e == hipErrorOutOfMemory ? QMC_E_OOM : QMC_E_HIP,
-                    std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " B): " + hipGetErrorString(e));
-    }
-    return QMC_OK;
-}
-
-template <class T>
-void dev_free(T *&p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-DevFrame to_dev(const Frame &f) {
-    DevFrame d;
-    d.ux = f.u.x; d.uy = f.u.y; d.uz = f.u.z;
-    d.nx = f.n.x; d.ny = f.n.y; d.nz = f.n.z;
-    d.vx = f.v.x; d.vy = f.v.y; d.vz = f.v.z;
-    d.identity = f.identity;
-    return d;
-}
-
-void free_scene(qmc_ctx *c) {
-    for (auto &g : c->grids) {
-        dev_free(g.cell_start);
-        dev_free(g.match);
-        dev_free(g.quads);
-    }
-    c->grids.clear();
-    for (auto &r : c->recvs) dev_free(r.flux);
-    c->recvs.clear();
-    dev_free(c->d_helios);
-    dev_free(c->d_facets);
-    dev_free(c->d_offsets);
-    c->h_helios.clear();
-    c->uploaded = false;
-    c->offsets_ready = false;
-}
-
-qmc_status set_device(qmc_ctx *c) {
-    HIP_TRY(hipSetDevice(c->device));
-    return QMC_OK;
-}
-
-// Sobol direction numbers of the first two dimensions (dimension 0: van der Corput; dimension 1:
-// primitive polynomial x + 1 with m_1 = 1).
-void sobol_directions(uint32_t v0[32], uint32_t v1[32]) {
-    uint32_t m[33];
-    for (int k = 0; k < 32; ++k) v0[k] = 1u << (31 - k);
-    m[1] = 1;
-    for (int k = 2; k <= 32; ++k) m[k] = (m[k - 1] << 1) ^ m[k - 1];
-    for (int k = 1; k <= 32; ++k) v1[k - 1] = m[k] << (32 - k);
-}
-
-}  // namespace
-
 extern "C" qmc_status qmc_create(int32_t device, qmc_ctx **out) {
     if (!out) return fail(QMC_E_INVALID_ARG, "qmc_create: null out");
     int n = 0;

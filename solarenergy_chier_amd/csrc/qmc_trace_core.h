@@ -1,0 +1,147 @@
+// qmc_trace_core.h — per-ray device functions shared by the plain and the batched trace kernels, so
+// both follow exactly the same arithmetic (and so stay bit-identical to the host oracle):
+//   fr_apply         local2world_rotate with a precomputed frame (Util/global_function.cuh:19-36)
+//   dda_collision    rectGridDDA::collision + intersect (QuasiMonteCarloRayTracing/Grid/rectGridDDA.cu:7-119)
+//   facet_geometry   micro-facet origin / normal in registers (RectangleHelio.cu:54-80,
+//                    FocusFlatRectangleHelio.cu:12-64)
+//   receiver_hit     RectangleReceiver / CylinderReceiver GIntersect
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "qmc_device.h"
+
+namespace qmc {
+
+__device__ __forceinline__ V3 fr_apply(const DevFrame &f, V3 d) {
+    if (f.identity) return d;
+    return V3{d.x * f.ux + d.y * f.nx + d.z * f.vx,
+              d.x * f.uy + d.y * f.ny + d.z * f.vy,
+              d.x * f.uz + d.y * f.nz + d.z * f.vz};
+}
+
+struct RayStats {
+    unsigned cells, candidates, tests;
+};
+
+// 3D-DDA over the uniform grid; true when a non-own sub-heliostat quad is hit (first hit in
+// traversal and cell-list order; its id goes to hit_id).
+template <bool STATS>
+__device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, const float4 *quads, int own_lo,
+                                              int own_hi, int &hit_id, RayStats &st) {
+    const V3 gpos{g.pos[0], g.pos[1], g.pos[2]};
+    const V3 gint{g.interval[0], g.interval[1], g.interval[2]};
+    const V3 cell = (o - gpos) / gint;
+    int ix = (int)cell.x, iy = (int)cell.y, iz = (int)cell.z;
+    const int sx = (d.x >= 0) ? 1 : -1;
+    const int sy = (d.y >= 0) ? 1 : -1;
+    const int sz = (d.z >= 0) ? 1 : -1;
+    float tx = abs_divide(cal_tmax(d.x, gint.x, ix, o.x - gpos.x), d.x);
+    float ty = abs_divide(cal_tmax(d.y, gint.y, iy, o.y - gpos.y), d.y);
+    float tz = abs_divide(cal_tmax(d.z, gint.z, iz, o.z - gpos.z), d.z);
+    const float dx = abs_divide(gint.x, d.x);
+    const float dy = abs_divide(gint.y, d.y);
+    const float dz = abs_divide(gint.z, d.z);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    while (ix >= 0 && ix < nx && iy >= 0 && iy < ny && iz >= 0 && iz < nz) {
+        const int addr = ix * ny * nz + iy * nz + iz;
+        const int b = g.cell_start[addr], e = g.cell_start[addr + 1];
+        if (STATS) {
+            st.cells += 1;
+            st.candidates += (unsigned)(e - b);
+        }
+        for (int i = b; i < e; ++i) {
+            const int id = g.match[i];
+            if (id < own_lo || id >= own_hi) {
+                if (STATS) st.tests += 1;
+                const float4 qa = quads[3 * id];
+                const float4 qb = quads[3 * id + 1];
+                const float4 qc = quads[3 * id + 2];
+                float t, u, v;
+                if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) {
+                    hit_id = id;
+                    return true;
+                }
+            }
+        }
+        bool fx = false, fy = false, fz = false;
+        if (dda_less(tx, ty)) {
+            if (dda_less(tx, tz) || dda_equal(tx, tz)) { ix += sx; fx = true; }
+            if (dda_less(tz, tx) || dda_equal(tx, tz)) { iz += sz; fz = true; }
+        } else if (dda_less(ty, tx)) {
+            if (dda_less(ty, tz) || dda_equal(ty, tz)) { iy += sy; fy = true; }
+            if (dda_less(tz, ty) || dda_equal(ty, tz)) { iz += sz; fz = true; }
+        } else {
+            if (dda_less(tx, tz) || dda_equal(tx, tz)) { ix += sx; iy += sy; fx = fy = true; }
+            if (dda_less(tz, tx) || dda_equal(tx, tz)) { iz += sz; fz = true; }
+        }
+        tx += fx ? dx : 0.0f;
+        ty += fy ? dy : 0.0f;
+        tz += fz ? dz : 0.0f;
+    }
+    return false;
+}
+
+// Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
+// onto the facet normal.
+__device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
+                                               const DevFrame *&pert_frame, V3 &normal) {
+    const int cpr = H.cols * H.sub_cols;
+    int row = m / cpr;
+    int col = m % cpr;
+    const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
+    if (H.type == 0) {
+        // RectangleHelio.cu:64-79
+        const int brow = row / H.sub_rows;
+        const int bcol = col / H.sub_cols;
+        const float pl = H.pl;
+        V3 local{col * pl + bcol * H.gap[0] + pl / 2 - H.size[0] / 2, H.size[1] / 2,
+                 row * pl + brow * H.gap[1] + pl / 2 - H.size[2] / 2};
+        origin = fr_apply(H.fr, local) + pos;
+        pert_frame = &H.fr;
+        normal = V3{H.fr.nx, H.fr.ny, H.fr.nz};
+    } else {
+        // FocusFlatRectangleHelio.cu:21-39,53-64
+        const int block = (row / H.sub_rows) * H.cols + (col / H.sub_cols);
+        const DevFacet &Fc = facets[H.facet_base + block];
+        row %= H.sub_rows;
+        col %= H.sub_cols;
+        const float pl = H.pl;
+        V3 local{(col + 0.5f) * pl - H.len_col / 2, 0.0f, (row + 0.5f) * pl - H.len_row / 2};
+        local = fr_apply(Fc.fr_local, local) + V3{Fc.center[0], Fc.center[1], Fc.center[2]};
+        origin = fr_apply(H.fr_focus, local) + pos;
+        pert_frame = &Fc.fr_pert;
+        normal = V3{Fc.normal[0], Fc.normal[1], Fc.normal[2]};
+    }
+}
+
+// Receiver hit: true with (t, u, v) when the ray reaches the receiver surface.
+template <int RT>
+__device__ __forceinline__ bool receiver_hit(const DevReceiver &R, V3 o, V3 d, float &t, float &u, float &v) {
+    if (RT == 0) {
+        // RectangleReceiver::GIntersect (RectangleReceiver.cuh:32-34)
+        return ray_quad(o, d, V3{R.A[0], R.A[1], R.A[2]}, V3{R.E1[0], R.E1[1], R.E1[2]},
+                        V3{R.E2[0], R.E2[1], R.E2[2]}, t, u, v);
+    } else {
+        // CylinderReceiver::GIntersect (CylinderReceiver.cuh:39-89)
+        const float R2 = R.radius * R.radius;
+        V2 l{o.x - R.pos[0], o.z - R.pos[2]};
+        if (dot2(l, l) <= R2) return false;
+        V2 Ro{R.pos[0] - o.x, R.pos[2] - o.z};
+        float tp = dot2(Ro, normalize2(V2{d.x, d.z}));
+        float delta = dot2(Ro, Ro) - tp * tp;
+        if (tp < -kEps || delta > R2) return false;
+        float t_plus = delta <= 0.0f ? R.radius : sqrtf(R2 - delta);
+        t = (tp - t_plus) / length2(V2{d.x, d.z});
+        V3 ip = t * d + o;
+        u = (ip.y - R.pos[1]) / R.height + 0.5f;
+        if (u < 0.0f || u > 1.0f) return false;
+        V2 c = normalize2(V2{ip.x - R.pos[0], ip.z - R.pos[2]});
+        float cx = fminf(fmaxf(c.x, -1.0f), 1.0f);
+        v = (float)((double)acos_r(cx) / (2 * M_PI));
+        if (c.y < 0) v = 1 - v;
+        return true;
+    }
+}
+
+}  // namespace qmc

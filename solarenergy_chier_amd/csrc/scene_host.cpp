@@ -702,6 +702,25 @@ V3 receiver_focus(const HostReceiver &r, V3 hpos) {
     return V3{x, pos.y, z};
 }
 
+// Aim point with the aim-offset extension: focus centre + off[0]*across + off[1]*up, where for a
+// rectangle across/up are the unit edge directions v3-v0 (width) and v1-v0 (height), and for a
+// cylinder the horizontal tangent at the focus point and the axis (0,1,0). A zero offset returns
+// the reference focus centre unchanged.
+V3 aim_point(const HostReceiver &r, V3 hpos, const float off[2]) {
+    V3 f = receiver_focus(r, hpos);
+    if (off[0] == 0.0f && off[1] == 0.0f) return f;
+    V3 across, up;
+    if (r.d.type == 0) {
+        across = normalize(r.vert[3] - r.vert[0]);
+        up = normalize(r.vert[1] - r.vert[0]);
+    } else {
+        V3 rad = f - f3(r.d.pos);
+        across = (rad.x == 0.0f && rad.z == 0.0f) ? V3{1.0f, 0.0f, 0.0f} : normalize(V3{-rad.z, 0.0f, rad.x});
+        up = V3{0.0f, 1.0f, 0.0f};
+    }
+    return f + across * off[0] + up * off[1];
+}
+
 int sub_size(const HostHeliostat &h) {
     if (h.type == 1) return h.d.rows * h.d.cols;
     return h.d.sub_count_override > 0 ? h.d.sub_count_override : 1;
@@ -853,7 +872,7 @@ extern "C" qmc_status qmc_scene_process(qmc_scene *s) {
         g.sub_verts.clear();
         for (int i = 0; i < g.d.n_heliostats; ++i) {
             HostHeliostat &h = s->heliostats[g.d.start_heliostat + i];
-            V3 aim = receiver_focus(r, f3(h.d.pos));
+            V3 aim = aim_point(r, f3(h.d.pos), h.d.aim_offset);
             heliostat_process(h, aim, sun, c.helio_pixel_length);
             h.sub_begin = sub;
             sub += h.sub_count;

@@ -75,6 +75,7 @@ class HeliostatSpec:
     gap: tuple = (0.0, 0.0)
     surface_property: tuple = (-1.0, -1.0, -1.0, -1.0, -1.0, -1.0)
     sub_count_override: int = 0
+    aim_offset: tuple = (0.0, 0.0)
     local_centers: np.ndarray | None = None
     local_normals: np.ndarray | None = None
     _keep: list = field(default_factory=list, repr=False)
@@ -88,6 +89,7 @@ class HeliostatSpec:
         d.surface_property[:] = _f3(self.surface_property)
         d.grid = self.grid
         d.sub_count_override = self.sub_count_override
+        d.aim_offset[:] = _f3(self.aim_offset)
         self._keep.clear()
         if self.local_centers is not None:
             a = np.ascontiguousarray(self.local_centers, dtype=np.float32).reshape(-1)
