@@ -43,6 +43,9 @@ def main() -> int:
     ap.add_argument("--workload", default="ps10", choices=["ps10", "small"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched sun x aim measurement")
+    ap.add_argument("--batched-suns", type=int, default=4000)
+    ap.add_argument("--loop-instances", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (calibrates HBM byte counters), then exit")
     args = ap.parse_args()
@@ -121,6 +124,9 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, dev, ids, L, P, F, args.cpu_seconds)
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(dev, args.batched_suns, args.loop_instances)
 
     line = {
         "metric": "rays/s on 1 GPU (PS10-like); batched scenes/s; adjoint:forward cost",
@@ -156,6 +162,7 @@ def main() -> int:
             "algorithmic_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
     dev.close()
@@ -195,6 +202,71 @@ def cpu_baseline(scene, dev, ids, L, P, F, seconds):
     log(f"cpu baseline: {st['rays']} rays in {dt:.2f}s = {value / 1e6:.3f} M rays/s")
     return {"value": value, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"every {stride}-th ray of all {len(ids)} heliostats ({st['rays']} rays, {dt:.1f} s)"}
+
+
+def bench_batched(dev, n_suns: int, n_loop: int) -> dict:
+    """Small field (25 heliostats, 2.56M rays per instance, 128x128 map) over n_suns sun positions x 4
+    aim strategies in one persistent launch (schedule from the autotuner), against looping the plain
+    call (re-process the scene for the instance's sun/DNI/aims, refresh the device state, clear, trace,
+    read the map back) on the first n_loop instances."""
+    import numpy as np
+
+    from solarenergy_chier_amd import Scene, workloads
+
+    wl = workloads.small_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    dev.upload(scene)
+    dev.pool_generate(2, 4)
+    dev.offsets_generate(3)
+    nh = scene.counts.n_heliostats
+    aims = workloads.aim_strategies(nh)
+    dev.batch_set_aims(scene, aims)
+    sun, dni, aim = workloads.batched_instances(n_suns=n_suns)
+    inst = dev.instances(sun, dni, aim)
+    n = len(inst)
+    shape = dev.flux_shape(0)
+    rays = sum(scene.heliostat_info(h).n_micro for h in range(nh)) * scene.config.num_sunshape_lights_per_group
+    t = time.perf_counter()
+    sched = dev.batch_autotune(0, inst)
+    tune_s = time.perf_counter() - t
+    log(f"batched: {n} instances x {rays} rays, autotuned schedule {sched} in {tune_s:.2f}s")
+    dev.trace_batched(0, inst, sched)  # warm-up
+    dev.sync()
+    dev.event_record(4)
+    dev.trace_batched(0, inst, sched)
+    dev.event_record(5)
+    dev.sync()
+    ms = dev.event_elapsed_ms(4, 5)
+    batched_sps = n / (ms / 1e3)
+    maps = dev.batch_read(0, min(n, 8), shape)
+    log(f"batched: {ms:.1f} ms for {n} instances = {batched_sps:.0f} scenes/s "
+        f"({n * rays / (ms / 1e3) / 1e9:.2f} G rays/s)")
+    # looped plain calls, timed on the host clock (host re-processing is part of the plain path)
+    cfg = scene.config
+    k = min(n_loop, n)
+    dev.sync()
+    t = time.perf_counter()
+    for i in range(k):
+        cfg.sun_dir[:] = [float(x) for x in sun[i]]
+        cfg.dni = float(dni[i])
+        scene.set_config(cfg)
+        scene.set_aim_offsets(aims[aim[i]])
+        scene.process()
+        dev.update(scene)
+        dev.flux_clear(0)
+        dev.trace(range(nh))
+        m = dev.flux(0)
+        if i < len(maps):
+            d = np.abs(m.astype(np.float64) - maps[i]).sum() / max(np.abs(m).sum(), 1e-30)
+            if d > 1e-5:
+                raise AssertionError(f"batched instance {i} differs from the plain call: rel L1 {d:.2e}")
+    loop_s = time.perf_counter() - t
+    loop_sps = k / loop_s
+    log(f"plain loop: {k} instances in {loop_s:.2f}s = {loop_sps:.0f} scenes/s; speedup {batched_sps / loop_sps:.2f}x")
+    return {"metric": "scenes/s", "workload": f"small field x {n_suns} suns x 4 aims", "instances": n,
+            "rays_per_instance": rays, "batched_ms": ms, "batched_scenes_per_s": batched_sps,
+            "plain_loop_scenes_per_s": loop_sps, "plain_loop_instances": k,
+            "speedup_vs_plain_loop": batched_sps / loop_sps, "schedule": sched, "autotune_s": tune_s}
 
 
 def copy_probe(mib: int) -> int:

@@ -308,6 +308,56 @@ qmc_status qmc_smooth(qmc_ctx *ctx, float *device_image, int32_t w, int32_t h, i
 /* Convenience: smooth a receiver's flux map in place. */
 qmc_status qmc_smooth_receiver(qmc_ctx *ctx, int32_t receiver, int32_t radius, float trimmed_ratio);
 
+/* ------------------------------------------------------------------------------------------
+ * Re-configuration and batched traces (sun positions x aim strategies)
+ * ------------------------------------------------------------------------------------------ */
+/* Set the aim-point offsets of every heliostat (n = 2 * heliostats floats; see
+ * qmc_heliostat_desc.aim_offset). The scene must be processed again afterwards. No reference
+ * counterpart: the reference always aims at the receiver focus centre
+ * (Scene/SceneProcessor.cpp:33-35). */
+qmc_status qmc_scene_set_aim_offsets(qmc_scene *scene, const float *offsets, int32_t n);
+/* Refresh the sun- and aim-dependent device state (heliostat frames and energy factors, blocking
+ * quads, sun frame) of a re-processed scene with the same layout, keeping pools, offsets and flux
+ * maps. QMC_E_STATE if the layout changed (use qmc_upload then). This is the plain path's
+ * equivalent of re-running SceneProcessor::set_grid_receiver_heliostat_content for a new sun
+ * (Scene/SceneProcessor.cpp:13-39). */
+qmc_status qmc_update(qmc_ctx *ctx, const qmc_scene *scene);
+
+/* One instance of a batched trace: a sun direction, its DNI and an aim-point set. */
+typedef struct qmc_instance {
+    float sun_dir[3];
+    float dni;
+    int32_t aim_set;   /* index into the table set by qmc_batch_set_aims */
+    int32_t reserved;
+} qmc_instance;
+
+/* Launch schedule of a batched trace (chosen by the autotuner when NULL is passed). */
+typedef struct qmc_schedule {
+    int32_t block_threads;      /* 64 .. 1024 */
+    int32_t workgroups_per_cu;  /* persistent workgroups per compute unit */
+    int32_t lds_flux_tile;      /* 1: accumulate each instance's map in LDS; 0: global atomics */
+    int32_t reserved;
+} qmc_schedule;
+
+/* Aim-point sets for batched traces: offsets holds n_sets * heliostats * 2 floats (per set, per
+ * heliostat: across, up), turned into aim points with the scene's receiver geometry. No reference
+ * counterpart (batching is an extension). */
+qmc_status qmc_batch_set_aims(qmc_ctx *ctx, const qmc_scene *scene, const float *offsets, int32_t n_sets);
+/* Trace the heliostats of one grid for n instances in one persistent launch; instance i writes its
+ * own flux map (res_x*res_y floats) at d_maps + i*res_x*res_y (device pointer), or into a ctx buffer
+ * read back with qmc_batch_read when d_maps is NULL. Maps are overwritten, not accumulated. Each
+ * instance equals re-processing the scene with that sun, DNI and aim set and calling qmc_trace
+ * (QuasiMonteCarloRayTracer::rayTracing per heliostat, QuasiMonteCarloRayTracer.cpp:15-91) up to
+ * float summation order. Flat heliostats only. Asynchronous. */
+qmc_status qmc_trace_batched(qmc_ctx *ctx, int32_t grid, const qmc_instance *instances, int32_t n,
+                             const qmc_schedule *schedule, float *d_maps);
+qmc_status qmc_batch_read(qmc_ctx *ctx, int32_t first, int32_t count, float *out);
+/* Time the candidate schedules on (a prefix of) the instances and return the fastest; results are
+ * cached per (device, field shape) for the process and, when QMC_TUNE_CACHE names a file, across
+ * processes. */
+qmc_status qmc_batch_autotune(qmc_ctx *ctx, int32_t grid, const qmc_instance *instances, int32_t n,
+                              qmc_schedule *out);
+
 /* HIP-event timing on the ctx stream (slots 0..15). */
 qmc_status qmc_event_record(qmc_ctx *ctx, int32_t slot);
 qmc_status qmc_event_elapsed_ms(qmc_ctx *ctx, int32_t slot_begin, int32_t slot_end, float *ms);

@@ -91,6 +91,18 @@ class PoolParams(C.Structure):
     _fields_ = [("seed_sunshape", C.c_uint64), ("seed_perturbation", C.c_uint64)]
 
 
+class Instance(C.Structure):
+    _fields_ = [("sun_dir", C.c_float * 3), ("dni", C.c_float), ("aim_set", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Schedule(C.Structure):
+    _fields_ = [("block_threads", C.c_int32), ("workgroups_per_cu", C.c_int32), ("lds_flux_tile", C.c_int32),
+                ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_[:3]}
+
+
 RECORD_DTYPE = [("status", "<i4"), ("index", "<i4"), ("energy", "<f4"), ("flags", "<i4")]
 
 _P = C.c_void_p
@@ -150,6 +162,12 @@ SIGNATURES = {
     "qmc_flux_device_ptr": [_P, _I, C.POINTER(_FP), _IP, _IP],
     "qmc_smooth": [_P, _FP, _I, _I, _I, _F],
     "qmc_smooth_receiver": [_P, _I, _I, _F],
+    "qmc_scene_set_aim_offsets": [_P, _FP, _I],
+    "qmc_update": [_P, _P],
+    "qmc_batch_set_aims": [_P, _P, _FP, _I],
+    "qmc_trace_batched": [_P, _I, C.POINTER(Instance), _I, C.POINTER(Schedule), _FP],
+    "qmc_batch_read": [_P, _I, _I, _FP],
+    "qmc_batch_autotune": [_P, _I, C.POINTER(Instance), _I, C.POINTER(Schedule)],
     "qmc_event_record": [_P, _I],
     "qmc_event_elapsed_ms": [_P, _I, _I, _FP],
 }

@@ -37,6 +37,8 @@ struct DevHelio {
     int32_t sub_begin, sub_count;
     int32_t facet_base; // first DevFacet of a focus-flat heliostat
     int32_t grid, receiver;
+    int32_t fac_rule;   // 0: factor = DNI*x*x*rho/L (x = pl_h/pl_r); 1: DNI*x*rho/L (x = pl_h^2/pixel area)
+    float fac_x;
     int32_t pad_;
     int64_t offsets_base;
 };

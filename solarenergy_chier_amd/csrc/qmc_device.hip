@@ -408,6 +408,10 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_scratch);
     dev_free(c->d_cdf);
     dev_free(c->d_slope);
+    dev_free(c->d_aims);
+    dev_free(c->d_batch_maps);
+    if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
+    c->d_batch_inst = nullptr;
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -426,6 +430,82 @@ extern "C" qmc_status qmc_sync(qmc_ctx *c) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     return QMC_OK;
 }
+
+namespace {
+
+// Per-heliostat device records (frames, layout, energy factor, offsets base) and focus-flat facet
+// records of a processed scene; returns the total number of group offsets (F+1 per heliostat).
+int64_t pack_heliostats(const qmc_scene &sc, std::vector<DevHelio> &out, std::vector<DevFacet> &facets) {
+    const qmc_scene *s = &sc;
+    out.resize(s->heliostats.size());
+    facets.clear();
+    int64_t off_base = 0;
+    for (size_t h = 0; h < s->heliostats.size(); ++h) {
+        const HostHeliostat &hh = s->heliostats[h];
+        DevHelio &d = out[h];
+        std::memset(&d, 0, sizeof(d));
+        d.fr = to_dev(make_frame(hh.normal, false));
+        d.fr_focus = to_dev(make_frame(hh.normal, true));
+        for (int k = 0; k < 3; ++k) {
+            d.pos[k] = hh.d.pos[k];
+            d.size[k] = hh.d.size[k];
+        }
+        d.gap[0] = hh.d.gap[0];
+        d.gap[1] = hh.d.gap[1];
+        d.len_row = hh.len_row;
+        d.len_col = hh.len_col;
+        d.pl = hh.pixel_length;
+        d.factor = heliostat_factor(*s, (int)h);
+        heliostat_factor_parts(*s, (int)h, d.fac_rule, d.fac_x);
+        d.rows = hh.d.rows;
+        d.cols = hh.d.cols;
+        d.sub_rows = hh.sub_rows;
+        d.sub_cols = hh.sub_cols;
+        d.F = hh.n_micro;
+        d.type = hh.type;
+        d.sub_begin = hh.sub_begin;
+        d.sub_count = hh.sub_count;
+        d.grid = hh.d.grid;
+        d.receiver = s->grids[hh.d.grid].d.receiver;
+        d.offsets_base = off_base;
+        off_base += (int64_t)hh.n_micro + 1;
+        d.facet_base = (int)facets.size();
+        if (hh.type == 1) {
+            Frame fh = make_frame(hh.normal, true);
+            for (size_t k = 0; k < hh.loc_normals.size(); ++k) {
+                DevFacet f;
+                std::memset(&f, 0, sizeof(f));
+                f.fr_local = to_dev(make_frame(hh.loc_normals[k], true));
+                V3 wn = frame_apply(fh, hh.loc_normals[k]);  // map_microhelio_normals
+                f.fr_pert = to_dev(make_frame(wn, false));
+                f.center[0] = hh.loc_centers[k].x;
+                f.center[1] = hh.loc_centers[k].y;
+                f.center[2] = hh.loc_centers[k].z;
+                f.normal[0] = wn.x;
+                f.normal[1] = wn.y;
+                f.normal[2] = wn.z;
+                facets.push_back(f);
+            }
+        }
+    }
+    return off_base;
+}
+
+// Blocking quads of a grid: (A, E1, E2) = (V[3i+1], V[3i]-V[3i+1], V[3i+2]-V[3i+1]) as in
+// rectGridDDA::intersect (rectGridDDA.cu:15-18).
+std::vector<float4> pack_quads(const HostGrid &hg) {
+    std::vector<float4> quads((size_t)hg.n_sub * 3);
+    for (int i = 0; i < hg.n_sub; ++i) {
+        V3 A = hg.sub_verts[3 * i + 1], B = hg.sub_verts[3 * i], C = hg.sub_verts[3 * i + 2];
+        V3 E1 = B - A, E2 = C - A;
+        quads[3 * i] = make_float4(A.x, A.y, A.z, 0.0f);
+        quads[3 * i + 1] = make_float4(E1.x, E1.y, E1.z, 0.0f);
+        quads[3 * i + 2] = make_float4(E2.x, E2.y, E2.z, 0.0f);
+    }
+    return quads;
+}
+
+}  // namespace
 
 extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     if (!c || !s) return fail(QMC_E_INVALID_ARG, "qmc_upload: null argument");
@@ -476,15 +556,7 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         if ((int)hg.sub_verts.size() != 3 * hg.n_sub)
             return fail(QMC_E_UNSUPPORTED, "qmc_upload: grid " + std::to_string(gi) +
                                                " has a sub-heliostat count override without matching quads");
-        std::vector<float4> quads((size_t)hg.n_sub * 3);
-        for (int i = 0; i < hg.n_sub; ++i) {
-            // quad (A, B, C) = (V[3i+1], V[3i], V[3i+2]) as in rectGridDDA::intersect (rectGridDDA.cu:15-18)
-            V3 A = hg.sub_verts[3 * i + 1], B = hg.sub_verts[3 * i], C = hg.sub_verts[3 * i + 2];
-            V3 E1 = B - A, E2 = C - A;
-            quads[3 * i] = make_float4(A.x, A.y, A.z, 0.0f);
-            quads[3 * i + 1] = make_float4(E1.x, E1.y, E1.z, 0.0f);
-            quads[3 * i + 2] = make_float4(E2.x, E2.y, E2.z, 0.0f);
-        }
+        std::vector<float4> quads = pack_quads(hg);
         if ((st = dev_alloc(gb.cell_start, hg.cell_start.size()))) return st;
         if ((st = dev_alloc(gb.match, hg.match.size()))) return st;
         if ((st = dev_alloc(gb.quads, quads.size()))) return st;
@@ -513,55 +585,7 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     }
     // heliostats and focus-flat facets
     std::vector<DevFacet> facets;
-    c->h_helios.resize(s->heliostats.size());
-    int64_t off_base = 0;
-    for (size_t h = 0; h < s->heliostats.size(); ++h) {
-        const HostHeliostat &hh = s->heliostats[h];
-        DevHelio &d = c->h_helios[h];
-        std::memset(&d, 0, sizeof(d));
-        d.fr = to_dev(make_frame(hh.normal, false));
-        d.fr_focus = to_dev(make_frame(hh.normal, true));
-        for (int k = 0; k < 3; ++k) {
-            d.pos[k] = hh.d.pos[k];
-            d.size[k] = hh.d.size[k];
-        }
-        d.gap[0] = hh.d.gap[0];
-        d.gap[1] = hh.d.gap[1];
-        d.len_row = hh.len_row;
-        d.len_col = hh.len_col;
-        d.pl = hh.pixel_length;
-        d.factor = heliostat_factor(*s, (int)h);
-        d.rows = hh.d.rows;
-        d.cols = hh.d.cols;
-        d.sub_rows = hh.sub_rows;
-        d.sub_cols = hh.sub_cols;
-        d.F = hh.n_micro;
-        d.type = hh.type;
-        d.sub_begin = hh.sub_begin;
-        d.sub_count = hh.sub_count;
-        d.grid = hh.d.grid;
-        d.receiver = s->grids[hh.d.grid].d.receiver;
-        d.offsets_base = off_base;
-        off_base += (int64_t)hh.n_micro + 1;
-        d.facet_base = (int)facets.size();
-        if (hh.type == 1) {
-            Frame fh = make_frame(hh.normal, true);
-            for (size_t k = 0; k < hh.loc_normals.size(); ++k) {
-                DevFacet f;
-                std::memset(&f, 0, sizeof(f));
-                f.fr_local = to_dev(make_frame(hh.loc_normals[k], true));
-                V3 wn = frame_apply(fh, hh.loc_normals[k]);  // map_microhelio_normals
-                f.fr_pert = to_dev(make_frame(wn, false));
-                f.center[0] = hh.loc_centers[k].x;
-                f.center[1] = hh.loc_centers[k].y;
-                f.center[2] = hh.loc_centers[k].z;
-                f.normal[0] = wn.x;
-                f.normal[1] = wn.y;
-                f.normal[2] = wn.z;
-                facets.push_back(f);
-            }
-        }
-    }
+    int64_t off_base = pack_heliostats(*s, c->h_helios, facets);
     c->offsets_total = off_base;
     if ((st = dev_alloc(c->d_helios, c->h_helios.size()))) return st;
     if ((st = dev_alloc(c->d_facets, facets.size()))) return st;
@@ -599,6 +623,50 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     c->offsets_ready = false;
     if (c->pool_ready && c->P != s->cfg.num_sunshape_groups * s->cfg.num_sunshape_lights_per_group)
         c->pool_ready = false;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_update(qmc_ctx *c, const qmc_scene *s) {
+    if (!c || !s) return fail(QMC_E_INVALID_ARG, "qmc_update: null argument");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_update: nothing uploaded (call qmc_upload)");
+    if (!s->processed) return fail(QMC_E_STATE, "qmc_update: scene not processed (call qmc_scene_process)");
+    const qmc_config &o = c->cfg, &n = s->cfg;
+    bool same = s->heliostats.size() == c->h_helios.size() && s->grids.size() == c->grids.size() &&
+                s->receivers.size() == c->recvs.size() &&
+                o.num_sunshape_groups == n.num_sunshape_groups &&
+                o.num_sunshape_lights_per_group == n.num_sunshape_lights_per_group;
+    for (size_t g = 0; same && g < s->grids.size(); ++g)
+        same = s->grids[g].n_sub == c->grids[g].n_sub && (int)s->grids[g].sub_verts.size() == 3 * s->grids[g].n_sub;
+    for (size_t r = 0; same && r < s->receivers.size(); ++r)
+        same = s->receivers[r].res_x * s->receivers[r].res_y == c->recvs[r].w * c->recvs[r].h;
+    std::vector<DevHelio> helios;
+    std::vector<DevFacet> facets;
+    if (same) {
+        pack_heliostats(*s, helios, facets);
+        for (size_t h = 0; same && h < helios.size(); ++h)
+            same = helios[h].F == c->h_helios[h].F && helios[h].type == c->h_helios[h].type &&
+                   helios[h].sub_count == c->h_helios[h].sub_count && helios[h].grid == c->h_helios[h].grid;
+    }
+    if (!same) return fail(QMC_E_STATE, "qmc_update: scene layout differs from the uploaded one (use qmc_upload)");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    c->cfg = n;
+    V3 sun{n.sun_dir[0], n.sun_dir[1], n.sun_dir[2]};
+    c->sun = to_dev(make_frame(-sun, false));
+    c->h_helios = helios;
+    HIP_TRY(hipMemcpyAsync(c->d_helios, helios.data(), helios.size() * sizeof(DevHelio), hipMemcpyHostToDevice,
+                           c->stream));
+    if (!facets.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_facets, facets.data(), facets.size() * sizeof(DevFacet), hipMemcpyHostToDevice,
+                               c->stream));
+    for (size_t g = 0; g < s->grids.size(); ++g) {
+        std::vector<float4> q = pack_quads(s->grids[g]);
+        if (!q.empty())
+            HIP_TRY(hipMemcpyAsync(c->grids[g].quads, q.data(), q.size() * sizeof(float4), hipMemcpyHostToDevice,
+                                   c->stream));
+    }
+    // pageable sources: wait until the copies have consumed them
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return QMC_OK;
 }
 

@@ -82,6 +82,21 @@ __device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, cons
     return false;
 }
 
+// Origin of micro-facet m of a flat heliostat whose normal frame is fr (RectangleHelio.cu:64-79):
+// local centre (col*pl + block_col*gap.x + pl/2 - sx/2, sy/2, row*pl + block_row*gap.y + pl/2 - sz/2)
+// rotated into the frame and translated to the heliostat position.
+__device__ __forceinline__ V3 flat_facet_origin(const DevHelio &H, const DevFrame &fr, int m) {
+    const int cpr = H.cols * H.sub_cols;
+    const int row = m / cpr;
+    const int col = m % cpr;
+    const int brow = row / H.sub_rows;
+    const int bcol = col / H.sub_cols;
+    const float pl = H.pl;
+    V3 local{col * pl + bcol * H.gap[0] + pl / 2 - H.size[0] / 2, H.size[1] / 2,
+             row * pl + brow * H.gap[1] + pl / 2 - H.size[2] / 2};
+    return fr_apply(fr, local) + V3{H.pos[0], H.pos[1], H.pos[2]};
+}
+
 // Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
 // onto the facet normal.
 __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
@@ -91,13 +106,7 @@ __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet
     int col = m % cpr;
     const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
     if (H.type == 0) {
-        // RectangleHelio.cu:64-79
-        const int brow = row / H.sub_rows;
-        const int bcol = col / H.sub_cols;
-        const float pl = H.pl;
-        V3 local{col * pl + bcol * H.gap[0] + pl / 2 - H.size[0] / 2, H.size[1] / 2,
-                 row * pl + brow * H.gap[1] + pl / 2 - H.size[2] / 2};
-        origin = fr_apply(H.fr, local) + pos;
+        origin = flat_facet_origin(H, H.fr, m);
         pert_frame = &H.fr;
         normal = V3{H.fr.nx, H.fr.ny, H.fr.nz};
     } else {

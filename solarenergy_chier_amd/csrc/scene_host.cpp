@@ -897,7 +897,37 @@ float heliostat_factor(const qmc_scene &s, int h) {
     return c.dni * (hh.pixel_length * hh.pixel_length / r.area) * c.helio_reflected_rate /
            float(c.num_sunshape_lights_per_group);
 }
+
+void heliostat_factor_parts(const qmc_scene &s, int h, int &rule, float &x) {
+    const HostHeliostat &hh = s.heliostats[h];
+    const HostReceiver &r = s.receivers[s.grids[hh.d.grid].d.receiver];
+    if (!r.res_override) {
+        rule = 0;
+        x = hh.pixel_length / r.pixel_length;
+    } else {
+        rule = 1;
+        x = hh.pixel_length * hh.pixel_length / r.area;
+    }
+}
+
+V3 heliostat_aim(const qmc_scene &s, int h, const float off[2]) {
+    const HostHeliostat &hh = s.heliostats[h];
+    const HostReceiver &r = s.receivers[s.grids[hh.d.grid].d.receiver];
+    return aim_point(r, f3(hh.d.pos), off);
+}
 }  // namespace qmc
+
+extern "C" qmc_status qmc_scene_set_aim_offsets(qmc_scene *s, const float *off, int32_t n) {
+    if (!s || !off) return fail(QMC_E_INVALID_ARG, "qmc_scene_set_aim_offsets: null argument");
+    if (n != 2 * (int)s->heliostats.size())
+        return fail(QMC_E_INVALID_ARG, "qmc_scene_set_aim_offsets: need 2 floats per heliostat");
+    for (size_t h = 0; h < s->heliostats.size(); ++h) {
+        s->heliostats[h].d.aim_offset[0] = off[2 * h];
+        s->heliostats[h].d.aim_offset[1] = off[2 * h + 1];
+    }
+    s->processed = false;
+    return QMC_OK;
+}
 
 // ---------------------------------------------------------------------------------------------
 // Getters

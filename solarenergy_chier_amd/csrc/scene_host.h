@@ -72,5 +72,9 @@ void set_error(const std::string &msg);
 qmc_status fail(qmc_status s, const std::string &msg);
 // Energy factor of a heliostat (reference rule or resolution-override rule).
 float heliostat_factor(const qmc_scene &s, int h);
+// Factor rule and its scalar: rule 0 -> factor = DNI*x*x*rho/L, rule 1 -> DNI*x*rho/L.
+void heliostat_factor_parts(const qmc_scene &s, int h, int &rule, float &x);
+// Aim point of heliostat h of a processed scene for an aim offset (across, up) in the receiver.
+V3 heliostat_aim(const qmc_scene &s, int h, const float off[2]);
 void sunshape_params(const qmc_config &c, std::vector<float> &cdf_xy, std::vector<float> &slopes, float abcig[5]);
 }  // namespace qmc

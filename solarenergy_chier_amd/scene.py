@@ -241,6 +241,11 @@ class Scene:
         N.call("qmc_scene_get_factor", self.handle, h, C.byref(f))
         return f.value
 
+    def set_aim_offsets(self, offsets: np.ndarray) -> None:
+        """Aim-point offsets (heliostats x 2: across, up) in the receiver surface; re-process after."""
+        a = np.ascontiguousarray(offsets, np.float32).reshape(-1)
+        N.call("qmc_scene_set_aim_offsets", self.handle, N.fptr(a), len(a))
+
     def check_heliostat(self, h: int) -> None:
         N.call("qmc_scene_check_heliostat", self.handle, h)
 
@@ -385,6 +390,45 @@ class Device:
 
     def smooth_device(self, ptr: int, w: int, h: int, radius: int, ratio: float) -> None:
         N.call("qmc_smooth", self.handle, C.cast(C.c_void_p(ptr), N._FP), w, h, radius, C.c_float(ratio))
+
+    # re-configuration and batched traces --------------------------------------------------------------
+    def update(self, scene: Scene) -> None:
+        """Refresh sun/aim-dependent device state of a re-processed scene with the same layout."""
+        N.call("qmc_update", self.handle, scene.handle)
+
+    def batch_set_aims(self, scene: Scene, offsets: np.ndarray) -> None:
+        """offsets: (n_sets, heliostats, 2) aim-point offsets (across, up)."""
+        a = np.ascontiguousarray(offsets, np.float32)
+        N.call("qmc_batch_set_aims", self.handle, scene.handle, N.fptr(a.reshape(-1)), a.shape[0])
+
+    @staticmethod
+    def instances(sun_dirs: np.ndarray, dni: np.ndarray, aim_sets: np.ndarray):
+        n = len(dni)
+        arr = (N.Instance * n)()
+        sd = np.asarray(sun_dirs, np.float32).reshape(n, 3)
+        for i in range(n):
+            arr[i].sun_dir[:] = [float(x) for x in sd[i]]
+            arr[i].dni = float(dni[i])
+            arr[i].aim_set = int(aim_sets[i])
+        return arr
+
+    def trace_batched(self, grid: int, inst, schedule: dict | None = None, device_maps: int | None = None) -> None:
+        sched = None
+        if schedule is not None:
+            sched = C.byref(N.Schedule(schedule["block_threads"], schedule["workgroups_per_cu"],
+                                       schedule["lds_flux_tile"], 0))
+        ptr = C.cast(C.c_void_p(device_maps), N._FP) if device_maps else None
+        N.call("qmc_trace_batched", self.handle, grid, inst, len(inst), sched, ptr)
+
+    def batch_read(self, first: int, count: int, shape: tuple[int, int]) -> np.ndarray:
+        out = np.zeros((count,) + tuple(shape), np.float32)
+        N.call("qmc_batch_read", self.handle, first, count, N.fptr(out))
+        return out
+
+    def batch_autotune(self, grid: int, inst) -> dict:
+        s = N.Schedule()
+        N.call("qmc_batch_autotune", self.handle, grid, inst, len(inst), C.byref(s))
+        return s.as_dict()
 
     # timing -------------------------------------------------------------------------------------------
     def event_record(self, slot: int) -> None:

@@ -18,7 +18,7 @@ from pathlib import Path
 
 import numpy as np
 
-from .scene import GridSpec, HeliostatSpec, ReceiverSpec, make_config
+from .scene import GridSpec, HeliostatSpec, ReceiverSpec, make_config  # noqa: F401  (re-exported)
 
 
 def sun_direction(azimuth_deg: float, elevation_deg: float) -> tuple[float, float, float]:
@@ -152,3 +152,47 @@ def write_json(path: str | Path, cfg) -> None:
         "helio_pixel_length": float(cfg.helio_pixel_length),
     }
     Path(path).write_text(json.dumps(d, indent=2))
+
+
+def sun_sweep(n: int, latitude_deg: float = 37.4, min_elevation_deg: float = 10.0, dni_range=(600.0, 1000.0),
+              seed: int = 5) -> tuple[np.ndarray, np.ndarray]:
+    """Seeded sun positions over a year of daylight: declination uniform on [-23.44, 23.44] deg, hour
+    angle uniform over that day's daylight, rejected below the minimum elevation; DNI uniform on
+    dni_range. Returns (sun_dir (n, 3) float32 unit vectors in the light's travel direction, dni (n,))."""
+    rng = np.random.default_rng(seed)
+    phi = math.radians(latitude_deg)
+    dirs, dnis = [], []
+    while len(dirs) < n:
+        dec = math.radians(rng.uniform(-23.44, 23.44))
+        h0 = math.acos(max(-1.0, min(1.0, -math.tan(phi) * math.tan(dec))))
+        ha = rng.uniform(-h0, h0)
+        east = -math.cos(dec) * math.sin(ha)
+        north = math.sin(dec) * math.cos(phi) - math.cos(dec) * math.sin(phi) * math.cos(ha)
+        up = math.sin(dec) * math.sin(phi) + math.cos(dec) * math.cos(phi) * math.cos(ha)
+        if up <= math.sin(math.radians(min_elevation_deg)):
+            continue
+        v = -np.array([east, up, north])
+        v /= np.linalg.norm(v)
+        dirs.append(v.astype(np.float32))
+        dnis.append(rng.uniform(*dni_range))
+    return np.array(dirs, np.float32), np.array(dnis, np.float32)
+
+
+def aim_strategies(n_heliostats: int, n_random: int = 3, span: float = 3.0, seed: int = 6) -> np.ndarray:
+    """Aim-point offset sets (n_random + 1, heliostats, 2): set 0 aims every heliostat at the receiver
+    centre, the others draw each heliostat's (across, up) offset uniformly in [-span, span] metres."""
+    rng = np.random.default_rng(seed)
+    sets = [np.zeros((n_heliostats, 2), np.float32)]
+    for _ in range(n_random):
+        sets.append(rng.uniform(-span, span, size=(n_heliostats, 2)).astype(np.float32))
+    return np.stack(sets)
+
+
+def batched_instances(n_suns: int = 4000, n_aims: int = 4, seed: int = 5):
+    """The batched workload: n_suns sun positions x n_aims aim strategies, sun-major order.
+    Returns (sun_dirs (n, 3), dni (n,), aim_set (n,))."""
+    d, e = sun_sweep(n_suns, seed=seed)
+    sun = np.repeat(d, n_aims, axis=0)
+    dni = np.repeat(e, n_aims)
+    aim = np.tile(np.arange(n_aims, dtype=np.int32), n_suns)
+    return sun, dni, aim

@@ -74,3 +74,33 @@ def test_reference_scene_files(scn, cfg):
 def test_focus_kat_scene():
     s = Scene.from_specs(*K.focus_helio_scene())
     assert_same_scene(s, O.OracleScene(*K.focus_helio_scene()))
+
+
+@pytest.mark.parametrize("field", ["small", "ps10"])
+def test_aim_offsets_and_new_sun(field):
+    wl = workloads.small_field() if field == "small" else workloads.ps10_field()
+    n = len(wl.heliostats)
+    offs = workloads.aim_strategies(n)[3]
+    sun, dni = workloads.sun_sweep(1, seed=21)
+    cfg = wl.config
+    cfg.sun_dir[:] = [float(x) for x in sun[0]]
+    cfg.dni = float(dni[0])
+    s = Scene.from_specs(cfg, wl.receivers, wl.grids, wl.heliostats, process=False)
+    s.set_aim_offsets(offs)
+    s.process()
+    hs = [workloads.HeliostatSpec(pos=h.pos, size=h.size, grid=0, aim_offset=tuple(offs[k]))
+          for k, h in enumerate(wl.heliostats)]
+    assert_same_scene(s, O.OracleScene(cfg, wl.receivers, wl.grids, hs))
+    # the offset moves the aim point within the receiver plane by exactly |offset|
+    a = np.array(s.heliostat_info(0).aim, np.float64)
+    c = np.array(s.receiver_info(0).focus_center, np.float64)
+    assert np.linalg.norm(a - c) == pytest.approx(np.linalg.norm(offs[0]), rel=1e-5)
+
+
+def test_sun_sweep_and_aim_sets():
+    d, e = workloads.sun_sweep(500)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    assert (-d[:, 1] > np.sin(np.radians(10.0))).all()
+    assert e.min() >= 600 and e.max() <= 1000
+    a = workloads.aim_strategies(25)
+    assert a.shape == (4, 25, 2) and not a[0].any() and np.abs(a[1:]).max() <= 3.0
