@@ -124,6 +124,9 @@ inline void free_scene(qmc_ctx *c) {
     dev_free(c->d_helios);
     dev_free(c->d_facets);
     dev_free(c->d_offsets);
+    dev_free(c->d_aims);  // aim points belong to the scene they were computed for
+    c->n_aim_sets = 0;
+    c->batch_n = 0;
     c->h_helios.clear();
     c->uploaded = false;
     c->offsets_ready = false;
