@@ -358,6 +358,30 @@ qmc_status qmc_batch_read(qmc_ctx *ctx, int32_t first, int32_t count, float *out
 qmc_status qmc_batch_autotune(qmc_ctx *ctx, int32_t grid, const qmc_instance *instances, int32_t n,
                               qmc_schedule *out);
 
+/* ------------------------------------------------------------------------------------------
+ * Differentiable trace (no reference counterpart: the reference renders flux maps only; the
+ * per-ray model is map_raytracing, RectangleReceiverRectangleGridRayTracing.cu:6-41, with a
+ * bilinear instead of a nearest-pixel deposit). Flat heliostats, rectangle receivers.
+ * ------------------------------------------------------------------------------------------ */
+/* Forward of every heliostat of a grid with full visibility (shadowing, blocking, receiver
+ * bounds); each visible ray splats its energy bilinearly at (v*res_x-0.5, u*res_y-0.5) into
+ * d_flux (device, overwritten) and sets its bit of the ctx's visibility mask. */
+qmc_status qmc_adjoint_forward(qmc_ctx *ctx, int32_t grid, float *d_flux);
+/* The same model with the mask of the last qmc_adjoint_forward frozen (no visibility tests),
+ * for the current sun / aim parameters (after qmc_update): finite differences with common random
+ * numbers. */
+qmc_status qmc_adjoint_forward_masked(qmc_ctx *ctx, int32_t grid, float *d_flux);
+/* loss = sum (F - target)^2 + lambda * sum max(0, F - peak)^2 over n pixels of d_flux; dL/dF goes
+ * to d_grad (device, may be NULL: kept in the ctx for qmc_adjoint_backward). Synchronous. */
+qmc_status qmc_flux_loss(qmc_ctx *ctx, const float *d_flux, int32_t n, float target, float peak, float lambda,
+                         float *d_grad, double *loss);
+/* Adjoint of the masked model: given dL/dflux (device, NULL = the one of the last qmc_flux_loss),
+ * writes d(loss)/d(aim offset) for every heliostat (2 floats each: across, up; zero outside the
+ * grid) and d(loss)/d(sun_dir) (3 floats). Rays are regenerated from their sample indices and
+ * recomputed; visibility is the mask of the last qmc_adjoint_forward. Synchronous. */
+qmc_status qmc_adjoint_backward(qmc_ctx *ctx, const qmc_scene *scene, int32_t grid, const float *d_dL_dflux,
+                                float *grad_aim, float *grad_sun);
+
 /* HIP-event timing on the ctx stream (slots 0..15). */
 qmc_status qmc_event_record(qmc_ctx *ctx, int32_t slot);
 qmc_status qmc_event_elapsed_ms(qmc_ctx *ctx, int32_t slot_begin, int32_t slot_end, float *ms);

@@ -1113,3 +1113,211 @@ extern "C" int orc_brute_any_hit(const orc_scene *s, int g, const float o[3], co
     }
     return 0;
 }
+
+// ------------------------------------------------------------------------------------------
+// FP64 host adjoint of the differentiable trace (extension; test infrastructure). Forward-mode
+// dual numbers in double over the masked model: per visible ray the derivatives with respect to
+// its heliostat's aim offset (across, up) and the sun direction (3 components). Visibility is
+// decided by the float reference trace above (shadowing, blocking, receiver bounds) and frozen.
+// ------------------------------------------------------------------------------------------
+namespace {
+
+struct D5 {
+    double v;
+    double d[5];
+};
+inline D5 dc(double v) {
+    D5 r;
+    r.v = v;
+    for (double &x : r.d) x = 0.0;
+    return r;
+}
+inline D5 operator+(const D5 &a, const D5 &b) {
+    D5 r;
+    r.v = a.v + b.v;
+    for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] + b.d[k];
+    return r;
+}
+inline D5 operator-(const D5 &a, const D5 &b) {
+    D5 r;
+    r.v = a.v - b.v;
+    for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] - b.d[k];
+    return r;
+}
+inline D5 operator-(const D5 &a) { return dc(0.0) - a; }
+inline D5 operator*(const D5 &a, const D5 &b) {
+    D5 r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+}
+inline D5 operator*(const D5 &a, double s) {
+    D5 r;
+    r.v = a.v * s;
+    for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * s;
+    return r;
+}
+inline D5 operator/(const D5 &a, const D5 &b) {
+    D5 r;
+    r.v = a.v / b.v;
+    for (int k = 0; k < 5; ++k) r.d[k] = (a.d[k] * b.v - a.v * b.d[k]) / (b.v * b.v);
+    return r;
+}
+inline D5 dsqrt(const D5 &a) {
+    D5 r;
+    r.v = std::sqrt(a.v);
+    for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] / (2.0 * r.v);
+    return r;
+}
+inline D5 dexp(const D5 &a) {
+    D5 r;
+    r.v = std::exp(a.v);
+    for (int k = 0; k < 5; ++k) r.d[k] = a.d[k] * r.v;
+    return r;
+}
+inline D5 dabs(const D5 &a) { return a.v >= 0 ? a : -a; }
+
+struct V5 {
+    D5 x, y, z;
+};
+inline V5 vc(double x, double y, double z) { return V5{dc(x), dc(y), dc(z)}; }
+inline V5 operator+(const V5 &a, const V5 &b) { return V5{a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V5 operator-(const V5 &a, const V5 &b) { return V5{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V5 operator*(const V5 &a, const D5 &s) { return V5{a.x * s, a.y * s, a.z * s}; }
+inline V5 operator*(const V5 &a, double s) { return V5{a.x * s, a.y * s, a.z * s}; }
+inline D5 vdot(const V5 &a, const V5 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V5 vcross(const V5 &a, const V5 &b) {
+    return V5{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline V5 vnorm(const V5 &a) {
+    D5 l = dsqrt(vdot(a, a));
+    return V5{a.x / l, a.y / l, a.z / l};
+}
+
+}  // namespace
+
+extern "C" int orc_adjoint_fp64(const orc_scene *s, int grid, const float *dirs, const float *perts, int P, int L,
+                                const int32_t *offsets_all, float target, float peak, float lambda, int fd_helio,
+                                const double *delta, double *flux_out, double *loss_out, double *grad_aim,
+                                double *grad_sun, int64_t *n_visible) {
+    if (grid < 0 || grid >= (int)s->G.size()) return -1;
+    const Grid &g = s->G[grid];
+    const Receiver &R = s->R[g.d.receiver];
+    if (R.d.type != 0) return -2;
+    std::vector<int64_t> base(s->H.size() + 1, 0);
+    for (size_t h = 0; h < s->H.size(); ++h) base[h + 1] = base[h] + s->H[h].F + 1;
+    const int resx = R.res_x, resy = R.res_y;
+    const size_t npix = (size_t)resx * resy;
+    std::vector<double> flux(npix, 0.0);
+    // pass 1: visibility from the float reference trace
+    std::vector<std::vector<qmc_ray_record>> recs(g.d.n_heliostats);
+    for (int k = 0; k < g.d.n_heliostats; ++k) {
+        int h = g.d.start_heliostat + k;
+        if (s->H[h].type != 0) return -3;
+        std::vector<double> tmp(npix, 0.0);
+        recs[k].resize((size_t)s->H[h].F * L);
+        orc_trace_heliostat(s, h, dirs, perts, P, L, offsets_all + base[h], tmp.data(), recs[k].data(), nullptr);
+    }
+    const f3 sunf = load3(s->cfg.sun_dir);
+    const double Fx = R.focus.x, Fy = R.focus.y, Fz = R.focus.z;
+    f3 acr = normalize(sub(R.v[3], R.v[0])), upv = normalize(sub(R.v[1], R.v[0]));
+    const V5 A0 = vc(R.v[0].x, R.v[0].y, R.v[0].z);
+    f3 e1 = sub(R.v[1], R.v[0]), e2 = sub(R.v[3], R.v[0]);
+    const V5 E1 = vc(e1.x, e1.y, e1.z), E2 = vc(e2.x, e2.y, e2.z);
+    const V5 Y = vc(0.0, 1.0, 0.0);
+    // per-ray model evaluated on duals; mode 0 = forward flux, mode 1 = gradient with G
+    std::vector<double> G(npix, 0.0);
+    int64_t nvis = 0;
+    for (int mode = 0; mode < 2; ++mode) {
+        for (int k = 0; k < g.d.n_heliostats; ++k) {
+            const int h = g.d.start_heliostat + k;
+            const Helio &H = s->H[h];
+            const float factor = factor_of(*s, h);
+            const double dsh = (delta && h == fd_helio) ? 1.0 : 0.0;
+            D5 o0 = dc(H.d.aim_offset[0] + dsh * (delta ? delta[0] : 0.0));
+            D5 o1 = dc(H.d.aim_offset[1] + dsh * (delta ? delta[1] : 0.0));
+            o0.d[0] = 1.0;
+            o1.d[1] = 1.0;
+            V5 sv = vc(sunf.x + (delta ? delta[2] : 0.0), sunf.y + (delta ? delta[3] : 0.0),
+                       sunf.z + (delta ? delta[4] : 0.0));
+            sv.x.d[2] = 1.0;
+            sv.y.d[3] = 1.0;
+            sv.z.d[4] = 1.0;
+            const V5 p = vc(H.d.pos[0], H.d.pos[1], H.d.pos[2]);
+            const V5 A = vc(Fx, Fy, Fz) + vc(acr.x, acr.y, acr.z) * o0 + vc(upv.x, upv.y, upv.z) * o1;
+            const V5 n = vnorm(vnorm(A - p) - sv);
+            const V5 u = vnorm(vcross(Y, n));
+            const V5 v = vnorm(vcross(u, n));
+            const V5 mm = vc(0, 0, 0) - sv;
+            const V5 us = vnorm(vcross(Y, mm));
+            const V5 vs = vnorm(vcross(us, mm));
+            const int32_t *grp = offsets_all + base[h];
+            const int rc_y = H.d.cols;
+            double ga0 = 0.0, ga1 = 0.0;
+            for (int64_t r = 0; r < (int64_t)H.F * L; ++r) {
+                if (recs[k][r].status != QMC_RAY_DEPOSITED) continue;
+                if (mode == 0) ++nvis;
+                const int m = (int)(r / L), j = (int)(r % L);
+                const int row = m / (rc_y * H.sub_cols), col = m % (rc_y * H.sub_cols);
+                const double lx = col * (double)H.pl + (col / H.sub_cols) * (double)H.d.gap[0] + H.pl / 2.0 -
+                                  H.d.size[0] / 2.0;
+                const double ly = H.d.size[1] / 2.0;
+                const double lz = row * (double)H.pl + (row / H.sub_rows) * (double)H.d.gap[1] + H.pl / 2.0 -
+                                  H.d.size[2] / 2.0;
+                const float *qi = dirs + 3 * ((grp[m] + j) % P);
+                const float *qp = perts + 3 * ((grp[(m + 1) % P] + j) % P);
+                const V5 O = p + u * lx + n * ly + v * lz;
+                const V5 din = us * (double)qi[0] + mm * (double)qi[1] + vs * (double)qi[2];
+                const V5 N = vnorm(u * (double)qp[0] + n * (double)qp[1] + v * (double)qp[2]);
+                const V5 ii = vc(0, 0, 0) - din;
+                const V5 rr = ii - N * (vdot(N, ii) * 2.0);
+                const V5 d = vnorm(rr);
+                const V5 pv = vcross(d, E2);
+                const D5 det = vdot(E1, pv);
+                const V5 T = O - A0;
+                const D5 uu = vdot(T, pv) / det;
+                const V5 q = vcross(T, E1);
+                const D5 vv = vdot(d, q) / det;
+                const D5 t = vdot(E2, q) / det;
+                const D5 eta = t.v <= 1000.0 ? dc(0.99331) - t * 0.0001176 + t * t * 1.97e-8 : dexp(t * -0.0001106);
+                const D5 E = dabs(vdot(d, N)) * eta * (double)factor;
+                const D5 x = vv * (double)resx - dc(0.5), yv = uu * (double)resy - dc(0.5);
+                const double fx0 = std::floor(x.v), fy0 = std::floor(yv.v);
+                const D5 fx = x - dc(fx0), fy = yv - dc(fy0);
+                const D5 wx[2] = {dc(1.0) - fx, fx}, wy[2] = {dc(1.0) - fy, fy};
+                for (int b = 0; b < 2; ++b)
+                    for (int a = 0; a < 2; ++a) {
+                        const int pi = (int)fx0 + a, pj = (int)fy0 + b;
+                        if (pi < 0 || pi >= resx || pj < 0 || pj >= resy) continue;
+                        const D5 c = E * wx[a] * wy[b];
+                        const size_t pix = (size_t)pj * resx + pi;
+                        if (mode == 0) {
+                            flux[pix] += c.v;
+                        } else {
+                            ga0 += G[pix] * c.d[0];
+                            ga1 += G[pix] * c.d[1];
+                            for (int kk = 0; kk < 3; ++kk) grad_sun[kk] += G[pix] * c.d[2 + kk];
+                        }
+                    }
+            }
+            if (mode == 1) {
+                grad_aim[2 * h] = ga0;
+                grad_aim[2 * h + 1] = ga1;
+            }
+        }
+        if (mode == 0) {
+            double loss = 0.0;
+            for (size_t i = 0; i < npix; ++i) {
+                const double e = flux[i] - target, over = std::max(flux[i] - (double)peak, 0.0);
+                G[i] = 2.0 * e + 2.0 * lambda * over;
+                loss += e * e + lambda * over * over;
+            }
+            if (loss_out) *loss_out = loss;
+            for (int kk = 0; kk < 3; ++kk) grad_sun[kk] = 0.0;
+            for (size_t hh = 0; hh < s->H.size(); ++hh) grad_aim[2 * hh] = grad_aim[2 * hh + 1] = 0.0;
+        }
+    }
+    if (flux_out) std::copy(flux.begin(), flux.end(), flux_out);
+    if (n_visible) *n_visible = nvis;
+    return 0;
+}

@@ -58,6 +58,15 @@ int orc_trace_many(const orc_scene *s, const int32_t *ids, int n, const float *d
                    int L, const int32_t *offsets_all, double *flux, int nthreads, int64_t ray_stride,
                    qmc_trace_stats *stats);
 
+/* FP64 host adjoint of the differentiable trace of grid `grid` (flat heliostats, rectangle receiver):
+ * bilinear flux (double), loss, d(loss)/d(aim offsets) (2 per heliostat) and d(loss)/d(sun_dir).
+ * delta (5 doubles, may be NULL) shifts heliostat fd_helio's aim offset (across, up) and the sun
+ * direction while visibility stays that of the unshifted scene (finite differences of the model). */
+int orc_adjoint_fp64(const orc_scene *s, int grid, const float *dirs, const float *perts, int P, int L,
+                     const int32_t *offsets_all, float target, float peak, float lambda, int fd_helio,
+                     const double *delta, double *flux_out, double *loss_out, double *grad_aim, double *grad_sun,
+                     int64_t *n_visible);
+
 /* Trimmed-mean smoother, in place (reference two-heap algorithm). */
 void orc_smooth(float *img, int w, int h, int radius, float ratio);
 

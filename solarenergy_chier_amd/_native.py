@@ -168,6 +168,10 @@ SIGNATURES = {
     "qmc_trace_batched": [_P, _I, C.POINTER(Instance), _I, C.POINTER(Schedule), _FP],
     "qmc_batch_read": [_P, _I, _I, _FP],
     "qmc_batch_autotune": [_P, _I, C.POINTER(Instance), _I, C.POINTER(Schedule)],
+    "qmc_adjoint_forward": [_P, _I, _FP],
+    "qmc_adjoint_forward_masked": [_P, _I, _FP],
+    "qmc_flux_loss": [_P, _FP, _I, _F, _F, _F, _FP, C.POINTER(C.c_double)],
+    "qmc_adjoint_backward": [_P, _P, _I, _FP, _FP, _FP],
     "qmc_event_record": [_P, _I],
     "qmc_event_elapsed_ms": [_P, _I, _I, _FP],
 }

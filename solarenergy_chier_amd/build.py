@@ -21,7 +21,7 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libqmcrt.so"
 
-HIP_SOURCES = ["qmc_device.hip", "qmc_batch.hip"]
+HIP_SOURCES = ["qmc_device.hip", "qmc_batch.hip", "qmc_adjoint.hip"]
 HOST_SOURCES = ["scene_host.cpp", "facade.cpp"]
 
 ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")

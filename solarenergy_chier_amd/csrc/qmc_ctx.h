@@ -112,6 +112,8 @@ inline DevFrame to_dev(const Frame &f) {
     return d;
 }
 
+void qmc_adjoint_release(qmc_ctx *c);  // qmc_adjoint.hip
+
 inline void free_scene(qmc_ctx *c) {
     for (auto &g : c->grids) {
         dev_free(g.cell_start);
@@ -127,6 +129,7 @@ inline void free_scene(qmc_ctx *c) {
     dev_free(c->d_aims);  // aim points belong to the scene they were computed for
     c->n_aim_sets = 0;
     c->batch_n = 0;
+    qmc_adjoint_release(c);  // visibility masks belong to the uploaded scene
     c->h_helios.clear();
     c->uploaded = false;
     c->offsets_ready = false;

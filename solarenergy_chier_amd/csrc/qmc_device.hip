@@ -412,6 +412,7 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_batch_maps);
     if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
     c->d_batch_inst = nullptr;
+    qmc_adjoint_release(c);
     for (int k = 0; k < 16; ++k)
         if (c->ev[k]) (void)hipEventDestroy(c->ev[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);

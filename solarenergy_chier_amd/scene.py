@@ -430,6 +430,29 @@ class Device:
         N.call("qmc_batch_autotune", self.handle, grid, inst, len(inst), C.byref(s))
         return s.as_dict()
 
+    # differentiable trace ------------------------------------------------------------------------------
+    def _dptr(self, ptr: int):
+        return C.cast(C.c_void_p(ptr), N._FP)
+
+    def adjoint_forward(self, grid: int, d_flux: int, masked: bool = False) -> None:
+        """Bilinear-deposit forward into the device buffer d_flux (visibility recorded, or frozen)."""
+        name = "qmc_adjoint_forward_masked" if masked else "qmc_adjoint_forward"
+        N.call(name, self.handle, grid, self._dptr(d_flux))
+
+    def flux_loss(self, d_flux: int, n: int, target: float, peak: float, lam: float, d_grad: int | None = None):
+        loss = C.c_double(0.0)
+        N.call("qmc_flux_loss", self.handle, self._dptr(d_flux), n, C.c_float(target), C.c_float(peak),
+               C.c_float(lam), self._dptr(d_grad) if d_grad else None, C.byref(loss))
+        return loss.value
+
+    def adjoint_backward(self, scene: Scene, grid: int, d_grad: int | None = None):
+        nh = scene.counts.n_heliostats
+        ga = np.zeros(2 * nh, np.float32)
+        gs = np.zeros(3, np.float32)
+        N.call("qmc_adjoint_backward", self.handle, scene.handle, grid, self._dptr(d_grad) if d_grad else None,
+               N.fptr(ga), N.fptr(gs))
+        return ga.reshape(nh, 2), gs
+
     # timing -------------------------------------------------------------------------------------------
     def event_record(self, slot: int) -> None:
         N.call("qmc_event_record", self.handle, slot)

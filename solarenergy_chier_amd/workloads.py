@@ -196,3 +196,33 @@ def batched_instances(n_suns: int = 4000, n_aims: int = 4, seed: int = 5):
     dni = np.repeat(e, n_aims)
     aim = np.tile(np.arange(n_aims, dtype=np.int32), n_suns)
     return sun, dni, aim
+
+
+ADJOINT_TARGET = 7.0e5   # W/m2, flat target flux of the aim-point optimisation
+ADJOINT_PEAK = 1.0e6     # W/m2, peak-flux penalty threshold
+ADJOINT_LAMBDA = 1.0
+
+
+def adjoint_small_field(seed: int = 6) -> tuple[Workload, np.ndarray]:
+    """Differentiated form on the small field: 25 heliostats x 1600 micro-facets x 50 rays = 2M rays,
+    128x128 map, aim offsets drawn uniformly in [-2, 2] m (across, up). Returns (workload, offsets)."""
+    wl = small_field(L=50, groups=1024)
+    offs = np.random.default_rng(seed).uniform(-2.0, 2.0, size=(len(wl.heliostats), 2)).astype(np.float32)
+    for h, o in zip(wl.heliostats, offs):
+        h.aim_offset = (float(o[0]), float(o[1]))
+    wl.name = "small field, differentiable: 25 heliostats, 2M rays, 128x128"
+    return wl, offs
+
+
+def adjoint_ps10_field(seed: int = 6) -> tuple[Workload, np.ndarray]:
+    """Differentiated form on the PS10-like field: 624 heliostats, 0.2 m micro-facets (3008 per
+    heliostat) x 1 ray = 1.88M rays per forward pass, 128x128 map, aim offsets in [-2, 2] m."""
+    wl = ps10_field(L=1, groups=65536)
+    wl.config.helio_pixel_length = 0.2
+    r = wl.receivers[0]
+    r.res_x = r.res_y = 128
+    offs = np.random.default_rng(seed).uniform(-2.0, 2.0, size=(len(wl.heliostats), 2)).astype(np.float32)
+    for h, o in zip(wl.heliostats, offs):
+        h.aim_offset = (float(o[0]), float(o[1]))
+    wl.name = "PS10-like field, differentiable: 624 heliostats, 1.88M rays, 128x128"
+    return wl, offs

@@ -263,3 +263,28 @@ def cylinder_resolution(size, ppm):
     res = np.zeros(2, np.int32)
     lib().orc_cylinder_resolution(fp(f32(size)), ppm, ip(res))
     return int(res[0]), int(res[1])
+
+
+def adjoint_fp64(orc: OracleScene, grid, dirs, perts, L, offsets_all, target, peak, lam, fd_helio=-1, delta=None):
+    dirs, perts = f32(dirs).reshape(-1, 3), f32(perts).reshape(-1, 3)
+    P = dirs.shape[0]
+    l = lib()
+    fn = l.orc_adjoint_fp64
+    fn.restype = C.c_int
+    fn.argtypes = [_P, C.c_int, _FP, _FP, C.c_int, C.c_int, _IP, C.c_float, C.c_float, C.c_float, C.c_int, _DP, _DP,
+                   _DP, _DP, _DP, C.POINTER(C.c_int64)]
+    ri = orc.receiver_info(orc.grid_receiver(grid))
+    flux = np.zeros((ri.res_y, ri.res_x), np.float64)
+    loss = C.c_double(0)
+    ga = np.zeros(2 * orc.n_heliostats, np.float64)
+    gs = np.zeros(3, np.float64)
+    nv = C.c_int64(0)
+    dl = None
+    if delta is not None:
+        dl = np.ascontiguousarray(delta, np.float64)
+    off = np.ascontiguousarray(offsets_all, np.int32)
+    rc = fn(orc.handle, grid, fp(dirs), fp(perts), P, L, ip(off), target, peak, lam, fd_helio,
+            dl.ctypes.data_as(_DP) if dl is not None else None, flux.ctypes.data_as(_DP), C.byref(loss),
+            ga.ctypes.data_as(_DP), gs.ctypes.data_as(_DP), C.byref(nv))
+    assert rc == 0, rc
+    return flux, loss.value, ga.reshape(-1, 2), gs, nv.value
