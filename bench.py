@@ -44,6 +44,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched sun x aim measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--batched-suns", type=int, default=4000)
     ap.add_argument("--loop-instances", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -127,6 +128,9 @@ def main() -> int:
     batched = None
     if not args.no_batched:
         batched = bench_batched(dev, args.batched_suns, args.loop_instances)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = bench_adjoint(dev)
 
     line = {
         "metric": "rays/s on 1 GPU (PS10-like); batched scenes/s; adjoint:forward cost",
@@ -163,6 +167,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     dev.close()
@@ -267,6 +272,72 @@ def bench_batched(dev, n_suns: int, n_loop: int) -> dict:
             "rays_per_instance": rays, "batched_ms": ms, "batched_scenes_per_s": batched_sps,
             "plain_loop_scenes_per_s": loop_sps, "plain_loop_instances": k,
             "speedup_vs_plain_loop": batched_sps / loop_sps, "schedule": sched, "autotune_s": tune_s}
+
+
+def bench_adjoint(dev, steps: int = 64) -> dict:
+    """Differentiated form on the PS10-like field (624 heliostats x 2 aim offsets + sun direction,
+    1.88M rays, 128x128 bilinear map): cost of the adjoint against a plain forward trace of the same
+    rays, then `steps` gradient steps of aim-point optimisation."""
+    import numpy as np
+
+    from solarenergy_chier_amd import Scene, workloads
+
+    T, PK, LAM = workloads.ADJOINT_TARGET, workloads.ADJOINT_PEAK, workloads.ADJOINT_LAMBDA
+    wl, offs = workloads.adjoint_ps10_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    dev.upload(scene)
+    dev.pool_generate(2, 4)
+    dev.offsets_generate(3)
+    nh = scene.counts.n_heliostats
+    ids = list(range(nh))
+    rays = sum(scene.heliostat_info(h).n_micro for h in ids) * scene.config.num_sunshape_lights_per_group
+    buf = dev.flux_device_ptr(0)
+    npix = 128 * 128
+    # warm-up of every kernel involved
+    dev.trace(ids)
+    dev.adjoint_forward(0, buf)
+    dev.flux_loss(buf, npix, T, PK, LAM)
+    dev.adjoint_backward(scene, 0)
+    reps = 5
+    t_plain = t_fwd = t_bwd = 0.0
+    for _ in range(reps):
+        dev.sync()
+        dev.event_record(6)
+        dev.trace(ids)
+        dev.event_record(7)
+        dev.adjoint_forward(0, buf)
+        dev.event_record(8)
+        loss = dev.flux_loss(buf, npix, T, PK, LAM)
+        dev.event_record(9)
+        ga, gs = dev.adjoint_backward(scene, 0)
+        dev.event_record(10)
+        dev.sync()
+        t_plain += dev.event_elapsed_ms(6, 7)
+        t_fwd += dev.event_elapsed_ms(7, 8)
+        t_bwd += dev.event_elapsed_ms(9, 10)
+    t_plain, t_fwd, t_bwd = t_plain / reps, t_fwd / reps, t_bwd / reps
+    log(f"adjoint: {rays} rays; plain trace {t_plain:.3f} ms, differentiable forward {t_fwd:.3f} ms, "
+        f"backward {t_bwd:.3f} ms -> adjoint:forward {t_bwd / t_plain:.2f}")
+    # aim-point optimisation: normalised gradient steps, offsets kept within +-2 m
+    o = offs.copy()
+    losses = []
+    t = time.perf_counter()
+    for step in range(steps):
+        scene.set_aim_offsets(o)
+        scene.process()
+        dev.update(scene)
+        dev.adjoint_forward(0, buf)
+        losses.append(dev.flux_loss(buf, npix, T, PK, LAM))
+        ga, gs = dev.adjoint_backward(scene, 0)
+        g = ga / max(float(np.abs(ga).max()), 1e-30)
+        o = np.clip(o - 0.1 * g, -2.0, 2.0).astype(np.float32)
+    opt_s = time.perf_counter() - t
+    log(f"aim-point optimisation: {steps} steps in {opt_s:.2f}s, loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    return {"metric": "adjoint:forward cost", "workload": wl.name, "parameters": 2 * nh + 3, "rays": rays,
+            "plain_forward_ms": t_plain, "differentiable_forward_ms": t_fwd, "backward_ms": t_bwd,
+            "adjoint_to_forward": t_bwd / t_plain, "adjoint_plus_forward_to_forward": (t_bwd + t_fwd) / t_plain,
+            "optimisation_steps": steps, "loss_first": losses[0], "loss_last": losses[-1],
+            "optimisation_s": opt_s}
 
 
 def copy_probe(mib: int) -> int:
