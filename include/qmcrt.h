@@ -364,13 +364,15 @@ qmc_status qmc_batch_autotune(qmc_ctx *ctx, int32_t grid, const qmc_instance *in
  * bilinear instead of a nearest-pixel deposit). Flat heliostats, rectangle receivers.
  * ------------------------------------------------------------------------------------------ */
 /* Forward of every heliostat of a grid with full visibility (shadowing, blocking, receiver
- * bounds); each visible ray splats its energy bilinearly at (v*res_x-0.5, u*res_y-0.5) into
- * d_flux (device, overwritten) and sets its bit of the ctx's visibility mask. */
+ * bounds, decided exactly as qmc_trace decides them); each visible ray splats its energy
+ * bilinearly at (v*res_x-0.5, u*res_y-0.5) into d_flux (device, overwritten) and sets its bit of
+ * the ctx's visibility mask. The splat position and energy are computed in double. */
 qmc_status qmc_adjoint_forward(qmc_ctx *ctx, int32_t grid, float *d_flux);
-/* The same model with the mask of the last qmc_adjoint_forward frozen (no visibility tests),
- * for the current sun / aim parameters (after qmc_update): finite differences with common random
- * numbers. */
-qmc_status qmc_adjoint_forward_masked(qmc_ctx *ctx, int32_t grid, float *d_flux);
+/* The same model with the mask of the last qmc_adjoint_forward frozen (no visibility tests), for
+ * the current sun / aim parameters (after qmc_update), accumulated in double and copied to
+ * flux_out (host, capacity n >= res_x*res_y doubles): finite differences with common random
+ * numbers. Synchronous. */
+qmc_status qmc_adjoint_forward_masked(qmc_ctx *ctx, int32_t grid, double *flux_out, int64_t n);
 /* loss = sum (F - target)^2 + lambda * sum max(0, F - peak)^2 over n pixels of d_flux; dL/dF goes
  * to d_grad (device, may be NULL: kept in the ctx for qmc_adjoint_backward). Synchronous. */
 qmc_status qmc_flux_loss(qmc_ctx *ctx, const float *d_flux, int32_t n, float target, float peak, float lambda,

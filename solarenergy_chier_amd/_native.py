@@ -169,7 +169,7 @@ SIGNATURES = {
     "qmc_batch_read": [_P, _I, _I, _FP],
     "qmc_batch_autotune": [_P, _I, C.POINTER(Instance), _I, C.POINTER(Schedule)],
     "qmc_adjoint_forward": [_P, _I, _FP],
-    "qmc_adjoint_forward_masked": [_P, _I, _FP],
+    "qmc_adjoint_forward_masked": [_P, _I, C.POINTER(C.c_double), C.c_int64],
     "qmc_flux_loss": [_P, _FP, _I, _F, _F, _F, _FP, C.POINTER(C.c_double)],
     "qmc_adjoint_backward": [_P, _P, _I, _FP, _FP, _FP],
     "qmc_event_record": [_P, _I],

@@ -5,14 +5,21 @@
 // 6-41) with the nearest-pixel deposit replaced by a bilinear one.
 //
 // Differentiated model of one visible ray of heliostat h (flat heliostats, rectangle receiver):
-//   n_h   = normalize(normalize(A_h - p_h) - s),  A_h = focus + o0*across + o1*up   (host, double)
+//   n_h   = normalize(normalize(A_h - p_h) - s),  A_h = focus + o0*across + o1*up
 //   (u,n,v) frame of n_h; O = p_h + lf.x*u + lf.y*n + lf.z*v (micro-facet origin, lf fixed)
 //   d_in  = R(-s) q_in  (sun-shape sample), N = normalize(R(n_h) q_p) (slope-error sample)
 //   d     = normalize(reflect(-d_in, N)); (t, uu, vv) = Moeller-Trumbore hit of the receiver plane
 //   E     = |d.N| eta(t) factor;  splat E with bilinear weights at (vv*res_x - 0.5, uu*res_y - 0.5)
-// Visibility (not shadowed, not blocked, inside the receiver) is decided by the forward pass and then
-// frozen, so the gradient is the derivative of the masked sum. Rays are never stored: the backward
-// kernel regenerates each ray from its pool indices (the QMC sample offsets) and recomputes it.
+// Visibility (not shadowed, not blocked, inside the receiver) is decided by the float trace of the
+// plain tracer and then frozen, so the gradient is the derivative of the masked sum. Rays are never
+// stored: the backward kernel regenerates each ray from its pool indices (the QMC sample offsets).
+//
+// Precision: the splat's derivative jumps where a hit crosses a pixel centre, so a gradient is only
+// reproducible if hit positions are: float geometry (about 1e-3 pixel of rounding at a 500 m slant
+// range) flips enough rays across those kinks to move a field-wide gradient by 1e-2. The model
+// (frames, reflection, plane hit, weights, adjoint) therefore runs in double — half the FP32 vector
+// rate on CDNA, a small cost next to the float DDA that decides visibility. The forward deposits
+// float(E*w) with float atomics; the masked forward accumulates in double (finite differences).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -27,15 +34,30 @@ using namespace qmc;
 
 namespace {
 
+struct D3 {
+    double x, y, z;
+};
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ D3 operator-(D3 a) { return D3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ D3 operator*(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ D3 operator/(D3 a, double s) { return D3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 dcross(D3 a, D3 b) {
+    return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ D3 ldd(const double *p) { return D3{p[0], p[1], p[2]}; }
+
 struct AdjHelio {
-    float u[3], n[3], v[3];
-    float nw1, nw2;   // |Y x n|, |u x n|
-    int64_t ray_base; // first mask bit of this heliostat (multiple of 64)
+    double u[3], n[3], v[3];
+    double nw1, nw2;   // |Y x n|, |u x n|
+    double p[3];       // position
+    int64_t ray_base;  // first mask bit of this heliostat (multiple of 64)
 };
 
 struct AdjSun {
-    float us[3], m[3], vs[3];  // frame of m = -s
-    float nw1, nw2;
+    double us[3], m[3], vs[3];  // frame of m = -s
+    double nw1, nw2;
 };
 
 struct AdjArgs {
@@ -51,42 +73,85 @@ struct AdjArgs {
     const int32_t *offsets;
     int32_t P, L;
     float *flux;          // bilinear map (forward)
+    double *flux64;       // bilinear map (masked forward)
     uint32_t *mask;       // visibility bits
     const float *G;       // dL/dflux (backward)
-    float *g_n;           // per heliostat dL/dn (3 floats), indexed by heliostat id
-    float *g_m;           // dL/dm (3 floats), m = -s
+    double *g_n;          // per heliostat dL/dn (3 doubles), indexed by heliostat id
+    double *g_m;          // dL/dm (3 doubles), m = -s
 };
 
-__device__ __forceinline__ V3 ld3(const float *p) { return V3{p[0], p[1], p[2]}; }
+// Forward values of the differentiated model for one ray (kept for the reverse sweep).
+struct RayModel {
+    D3 lf, N, ii, d, pv, T, q;
+    double lN, k, lr, det, uu, vv, t, c, eta, deta, E;
+};
 
-__device__ __forceinline__ void bilinear_splat(float *flux, int resx, int resy, float uu, float vv, float E) {
-    const float x = vv * (float)resx - 0.5f, y = uu * (float)resy - 0.5f;
-    const float fx0 = floorf(x), fy0 = floorf(y);
+__device__ __forceinline__ void ray_model(const AdjArgs &a, const DevHelio &H, const AdjHelio &Ah, int m, float4 qi,
+                                          float4 qp, RayModel &r) {
+    const D3 u = ldd(Ah.u), n = ldd(Ah.n), v = ldd(Ah.v);
+    const D3 us = ldd(a.sun.us), mm = ldd(a.sun.m), vs = ldd(a.sun.vs);
+    // local facet coordinates (RectangleHelio.cu:64-79)
+    const int cpr = H.cols * H.sub_cols;
+    const int row = m / cpr, col = m % cpr;
+    const double pl = H.pl;
+    r.lf = D3{col * pl + (col / H.sub_cols) * (double)H.gap[0] + pl / 2.0 - H.size[0] / 2.0, H.size[1] / 2.0,
+              row * pl + (row / H.sub_rows) * (double)H.gap[1] + pl / 2.0 - H.size[2] / 2.0};
+    const D3 O = ldd(Ah.p) + u * r.lf.x + n * r.lf.y + v * r.lf.z;
+    const D3 din = us * (double)qi.x + mm * (double)qi.y + vs * (double)qi.z;
+    const D3 Nraw = u * (double)qp.x + n * (double)qp.y + v * (double)qp.z;
+    r.lN = sqrt(ddot(Nraw, Nraw));
+    r.N = Nraw / r.lN;
+    r.ii = -din;
+    r.k = ddot(r.N, r.ii);
+    const D3 rr = r.ii - r.N * (r.k * 2.0);
+    r.lr = sqrt(ddot(rr, rr));
+    r.d = rr / r.lr;
+    const DevReceiver &R = a.recv;
+    const D3 A0{R.A[0], R.A[1], R.A[2]}, E1{R.E1[0], R.E1[1], R.E1[2]}, E2{R.E2[0], R.E2[1], R.E2[2]};
+    r.pv = dcross(r.d, E2);
+    r.det = ddot(E1, r.pv);
+    r.T = O - A0;
+    r.uu = ddot(r.T, r.pv) / r.det;
+    r.q = dcross(r.T, E1);
+    r.vv = ddot(r.d, r.q) / r.det;
+    r.t = ddot(E2, r.q) / r.det;
+    // atmospheric attenuation (global_function.cuh eta_aAlpha), with its derivative
+    if (r.t <= 1000.0) {
+        r.eta = 0.99331 - r.t * 0.0001176 + r.t * r.t * 1.97e-8;
+        r.deta = -0.0001176 + 2.0 * 1.97e-8 * r.t;
+    } else {
+        r.eta = exp(r.t * -0.0001106);
+        r.deta = -0.0001106 * r.eta;
+    }
+    r.c = ddot(r.d, r.N);
+    r.E = fabs(r.c) * r.eta * (double)H.factor;
+}
+
+template <bool F64>
+__device__ __forceinline__ void bilinear_splat(const AdjArgs &a, double uu, double vv, double E) {
+    const int resx = a.recv.res_x, resy = a.recv.res_y;
+    const double x = vv * (double)resx - 0.5, y = uu * (double)resy - 0.5;
+    const double fx0 = floor(x), fy0 = floor(y);
     const int i0 = (int)fx0, j0 = (int)fy0;
-    const float fx = x - fx0, fy = y - fy0;
-    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy};
+    const double fx = x - fx0, fy = y - fy0;
+    const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy};
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            const int i = i0 + a, j = j0 + b;
-            if (i >= 0 && i < resx && j >= 0 && j < resy) unsafeAtomicAdd(flux + j * resx + i, E * wx[a] * wy[b]);
+        for (int c = 0; c < 2; ++c) {
+            const int i = i0 + c, j = j0 + b;
+            if (i >= 0 && i < resx && j >= 0 && j < resy) {
+                const double w = E * wx[c] * wy[b];
+                if (F64)
+                    unsafeAtomicAdd(a.flux64 + j * resx + i, w);
+                else
+                    unsafeAtomicAdd(a.flux + j * resx + i, (float)w);
+            }
         }
 }
 
-// Receiver-plane hit without the [0,1] acceptance test (visibility is frozen by the mask).
-__device__ __forceinline__ void plane_hit(const DevReceiver &R, V3 O, V3 d, float &t, float &uu, float &vv) {
-    const V3 A0{R.A[0], R.A[1], R.A[2]}, E1{R.E1[0], R.E1[1], R.E1[2]}, E2{R.E2[0], R.E2[1], R.E2[2]};
-    const V3 pv = cross(d, E2);
-    const float det = dot(E1, pv);
-    const V3 T = O - A0;
-    uu = dot(T, pv) / det;
-    const V3 q = cross(T, E1);
-    vv = dot(d, q) / det;
-    t = dot(E2, q) / det;
-}
-
-// Forward with full visibility: bilinear deposit + visibility bit per ray.
+// Forward with full visibility: float visibility trace (the plain tracer's), double bilinear
+// deposit, one visibility bit per ray.
 __global__ __launch_bounds__(256) void k_adj_forward(AdjArgs a) {
     const int h = a.list[blockIdx.y];
     const DevHelio &H = a.helios[h];
@@ -111,7 +176,9 @@ __global__ __launch_bounds__(256) void k_adj_forward(AdjArgs a) {
                 float t, u, v;
                 if (receiver_hit<0>(a.recv, origin, dir, t, u, v)) {
                     visible = true;
-                    bilinear_splat(a.flux, a.recv.res_x, a.recv.res_y, u, v, ray_energy(t, dir, normal, H.factor));
+                    RayModel rm;
+                    ray_model(a, H, a.ah[h], m, s, q, rm);
+                    bilinear_splat<false>(a, rm.uu, rm.vv, rm.E);
                 }
             }
         }
@@ -125,7 +192,7 @@ __global__ __launch_bounds__(256) void k_adj_forward(AdjArgs a) {
     }
 }
 
-// Forward of the frozen-visibility model (finite differences with common random numbers).
+// Forward of the frozen-visibility model in double (finite differences with common random numbers).
 __global__ __launch_bounds__(256) void k_adj_forward_masked(AdjArgs a) {
     const int h = a.list[blockIdx.y];
     const DevHelio &H = a.helios[h];
@@ -137,41 +204,23 @@ __global__ __launch_bounds__(256) void k_adj_forward_masked(AdjArgs a) {
     if (!((a.mask[bit >> 5] >> (bit & 31)) & 1u)) return;
     const int m = (int)(r / L), j = (int)(r % L);
     const int32_t *grp = a.offsets + H.offsets_base;
-    const float4 s = a.pool[(grp[m] + j) % P];
-    V3 dir = fr_apply(a.sunf, V3{s.x, s.y, s.z});
-    const V3 origin = flat_facet_origin(H, H.fr, m);
-    const float4 q = a.pert[(grp[(m + 1) % P] + j) % P];
-    const V3 normal = normalize(fr_apply(H.fr, V3{q.x, q.y, q.z}));
-    dir = normalize(reflect(-dir, normal));
-    float t, uu, vv;
-    plane_hit(a.recv, origin, dir, t, uu, vv);
-    bilinear_splat(a.flux, a.recv.res_x, a.recv.res_y, uu, vv, ray_energy(t, dir, normal, H.factor));
+    RayModel rm;
+    ray_model(a, H, a.ah[h], m, a.pool[(grp[m] + j) % P], a.pert[(grp[(m + 1) % P] + j) % P], rm);
+    bilinear_splat<true>(a, rm.uu, rm.vv, rm.E);
 }
 
-__device__ __forceinline__ V3 norm_bwd(V3 y, float len, V3 g) {  // y = x/|x|
-    return (g - y * dot(y, g)) * (1.0f / len);
-}
-
-__device__ __forceinline__ float block_sum(float v, float *red) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) red[w] = v;
-    __syncthreads();
-    float s = 0.0f;
-    if (threadIdx.x == 0)
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += red[k];
-    return s;
+__device__ __forceinline__ D3 norm_bwd(D3 y, double len, D3 g) {  // y = x/|x|
+    return (g - y * ddot(y, g)) / len;
 }
 
 __global__ __launch_bounds__(256) void k_adj_backward(AdjArgs a) {
-    __shared__ float red[16];
+    __shared__ double red[4][6];
     const int h = a.list[blockIdx.y];
     const DevHelio &H = a.helios[h];
     const int L = a.L, P = a.P;
     const long long nrays = (long long)H.F * L;
     const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    V3 gn{0, 0, 0}, gm{0, 0, 0};
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // dL/dn (3), dL/dm (3)
     bool vis = false;
     if (r < nrays) {
         const long long bit = a.ah[h].ray_base + r;
@@ -179,119 +228,92 @@ __global__ __launch_bounds__(256) void k_adj_backward(AdjArgs a) {
     }
     if (vis) {
         const AdjHelio &Ah = a.ah[h];
-        const V3 u = ld3(Ah.u), n = ld3(Ah.n), v = ld3(Ah.v);
-        const V3 us = ld3(a.sun.us), mm = ld3(a.sun.m), vs = ld3(a.sun.vs);
         const int m = (int)(r / L), j = (int)(r % L);
         const int32_t *grp = a.offsets + H.offsets_base;
         const float4 qi4 = a.pool[(grp[m] + j) % P];
         const float4 qp4 = a.pert[(grp[(m + 1) % P] + j) % P];
-        const V3 qi{qi4.x, qi4.y, qi4.z}, qp{qp4.x, qp4.y, qp4.z};
-        // local facet coordinates (RectangleHelio.cu:64-79)
-        const int cpr = H.cols * H.sub_cols;
-        const int row = m / cpr, col = m % cpr;
-        const float pl = H.pl;
-        const V3 lf{col * pl + (col / H.sub_cols) * H.gap[0] + pl / 2 - H.size[0] / 2, H.size[1] / 2,
-                    row * pl + (row / H.sub_rows) * H.gap[1] + pl / 2 - H.size[2] / 2};
-        // ---- forward ----
-        const V3 O = ld3(H.pos) + u * lf.x + n * lf.y + v * lf.z;
-        const V3 din = us * qi.x + mm * qi.y + vs * qi.z;
-        const V3 Nraw = u * qp.x + n * qp.y + v * qp.z;
-        const float lN = length(Nraw);
-        const V3 N = Nraw * (1.0f / lN);
-        const V3 ii = -din;
-        const float k = dot(N, ii);
-        const V3 rr = ii - N * (2.0f * k);
-        const float lr = length(rr);
-        const V3 d = rr * (1.0f / lr);
-        const DevReceiver &R = a.recv;
-        const V3 A0{R.A[0], R.A[1], R.A[2]}, E1{R.E1[0], R.E1[1], R.E1[2]}, E2{R.E2[0], R.E2[1], R.E2[2]};
-        const V3 pv = cross(d, E2);
-        const float det = dot(E1, pv);
-        const V3 T = O - A0;
-        const float uu = dot(T, pv) / det;
-        const V3 q = cross(T, E1);
-        const float vv = dot(d, q) / det;
-        const float t = dot(E2, q) / det;
-        const float c = dot(d, N);
-        float eta, deta;
-        if (t <= 1000.0f) {
-            eta = 0.99331f - 0.0001176f * t + 1.97e-8f * t * t;
-            deta = -0.0001176f + 2.0f * 1.97e-8f * t;
-        } else {
-            eta = expf(-0.0001106f * t);
-            deta = -0.0001106f * eta;
-        }
-        const float f = H.factor;
-        const float E = fabsf(c) * eta * f;
-        const int resx = R.res_x, resy = R.res_y;
-        const float x = vv * (float)resx - 0.5f, y = uu * (float)resy - 0.5f;
-        const float fx0 = floorf(x), fy0 = floorf(y);
+        RayModel f;
+        ray_model(a, H, Ah, m, qi4, qp4, f);
+        // d(loss)/d(E), d/d(x), d/d(y) through the four bilinear weights
+        const int resx = a.recv.res_x, resy = a.recv.res_y;
+        const double x = f.vv * (double)resx - 0.5, y = f.uu * (double)resy - 0.5;
+        const double fx0 = floor(x), fy0 = floor(y);
         const int i0 = (int)fx0, j0 = (int)fy0;
-        const float fx = x - fx0, fy = y - fy0;
-        const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, dw[2] = {-1.0f, 1.0f};
-        float gE = 0.0f, gx = 0.0f, gy = 0.0f;
+        const double fx = x - fx0, fy = y - fy0;
+        const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy}, dw[2] = {-1.0, 1.0};
+        double gE = 0.0, gx = 0.0, gy = 0.0;
 #pragma unroll
         for (int bb = 0; bb < 2; ++bb)
 #pragma unroll
             for (int aa = 0; aa < 2; ++aa) {
                 const int pi = i0 + aa, pj = j0 + bb;
                 if (pi >= 0 && pi < resx && pj >= 0 && pj < resy) {
-                    const float Gp = a.G[pj * resx + pi];
+                    const double Gp = a.G[pj * resx + pi];
                     gE += Gp * wx[aa] * wy[bb];
-                    gx += Gp * E * dw[aa] * wy[bb];
-                    gy += Gp * E * wx[aa] * dw[bb];
+                    gx += Gp * f.E * dw[aa] * wy[bb];
+                    gy += Gp * f.E * wx[aa] * dw[bb];
                 }
             }
-        // ---- backward ----
-        const float g_vv = gx * (float)resx, g_uu = gy * (float)resy;
-        const float g_c = gE * (c >= 0.0f ? 1.0f : -1.0f) * eta * f;
-        const float g_t = gE * fabsf(c) * f * deta;
-        V3 g_d = N * g_c;
-        V3 g_N = d * g_c;
-        const float inv_det = 1.0f / det;
-        const float g_a1 = g_uu * inv_det, g_a2 = g_vv * inv_det, g_a3 = g_t * inv_det;
-        const float g_det = -(g_uu * uu + g_vv * vv + g_t * t) * inv_det;
-        V3 g_T = pv * g_a1;
-        const V3 g_pv = T * g_a1 + E1 * g_det;
-        g_d = g_d + cross(E2, g_pv) + q * g_a2;
-        const V3 g_q = d * g_a2 + E2 * g_a3;
-        g_T = g_T + cross(E1, g_q);
-        const V3 g_O = g_T;
-        const V3 g_r = norm_bwd(d, lr, g_d);
-        const float grN = dot(N, g_r);
-        const V3 g_i = g_r - N * (2.0f * grN);
-        g_N = g_N - (g_r * k + ii * grN) * 2.0f;
-        const V3 g_din = -g_i;
-        const V3 g_Nraw = norm_bwd(N, lN, g_N);
-        V3 g_u = g_Nraw * qp.x + g_O * lf.x;
-        gn = g_Nraw * qp.y + g_O * lf.y;
-        const V3 g_v = g_Nraw * qp.z + g_O * lf.z;
+        // reverse sweep
+        const double fac = H.factor;
+        const double g_vv = gx * (double)resx, g_uu = gy * (double)resy;
+        const double g_c = gE * (f.c >= 0.0 ? 1.0 : -1.0) * f.eta * fac;
+        const double g_t = gE * fabs(f.c) * fac * f.deta;
+        D3 g_d = f.N * g_c;
+        D3 g_N = f.d * g_c;
+        const DevReceiver &R = a.recv;
+        const D3 E1{R.E1[0], R.E1[1], R.E1[2]}, E2{R.E2[0], R.E2[1], R.E2[2]};
+        const double g_a1 = g_uu / f.det, g_a2 = g_vv / f.det, g_a3 = g_t / f.det;
+        const double g_det = -(g_uu * f.uu + g_vv * f.vv + g_t * f.t) / f.det;
+        D3 g_T = f.pv * g_a1;
+        const D3 g_pv = f.T * g_a1 + E1 * g_det;
+        g_d = g_d + dcross(E2, g_pv) + f.q * g_a2;
+        const D3 g_q = f.d * g_a2 + E2 * g_a3;
+        g_T = g_T + dcross(E1, g_q);
+        const D3 g_O = g_T;
+        const D3 g_r = norm_bwd(f.d, f.lr, g_d);
+        const double grN = ddot(f.N, g_r);
+        const D3 g_i = g_r - f.N * (2.0 * grN);
+        g_N = g_N - (g_r * f.k + f.ii * grN) * 2.0;
+        const D3 g_din = -g_i;
+        const D3 g_Nraw = norm_bwd(f.N, f.lN, g_N);
+        const D3 u = ldd(Ah.u), n = ldd(Ah.n), v = ldd(Ah.v);
+        D3 g_u = g_Nraw * (double)qp4.x + g_O * f.lf.x;
+        D3 gn = g_Nraw * (double)qp4.y + g_O * f.lf.y;
+        const D3 g_v = g_Nraw * (double)qp4.z + g_O * f.lf.z;
         // heliostat frame: w1 = Y x n, u = w1/|w1|, w2 = u x n, v = w2/|w2|
-        const V3 Y{0.0f, 1.0f, 0.0f};
-        const V3 g_w2 = norm_bwd(v, Ah.nw2, g_v);
-        g_u = g_u + cross(n, g_w2);
-        gn = gn + cross(g_w2, u);
-        const V3 g_w1 = norm_bwd(u, Ah.nw1, g_u);
-        gn = gn + cross(g_w1, Y);
+        const D3 Y{0.0, 1.0, 0.0};
+        const D3 g_w2 = norm_bwd(v, Ah.nw2, g_v);
+        g_u = g_u + dcross(n, g_w2);
+        gn = gn + dcross(g_w2, u);
+        const D3 g_w1 = norm_bwd(u, Ah.nw1, g_u);
+        gn = gn + dcross(g_w1, Y);
         // sun frame (m = -s)
-        const V3 g_us = g_din * qi.x, g_vs = g_din * qi.z;
-        gm = g_din * qi.y;
-        const V3 g_ws2 = norm_bwd(vs, a.sun.nw2, g_vs);
-        const V3 g_us2 = g_us + cross(mm, g_ws2);
-        gm = gm + cross(g_ws2, us);
-        const V3 g_ws1 = norm_bwd(us, a.sun.nw1, g_us2);
-        gm = gm + cross(g_ws1, Y);
+        const D3 us = ldd(a.sun.us), mm = ldd(a.sun.m), vs = ldd(a.sun.vs);
+        const D3 g_us = g_din * (double)qi4.x, g_vs = g_din * (double)qi4.z;
+        D3 gm = g_din * (double)qi4.y;
+        const D3 g_ws2 = norm_bwd(vs, a.sun.nw2, g_vs);
+        const D3 g_us2 = g_us + dcross(mm, g_ws2);
+        gm = gm + dcross(g_ws2, us);
+        const D3 g_ws1 = norm_bwd(us, a.sun.nw1, g_us2);
+        gm = gm + dcross(g_ws1, Y);
+        acc[0] = gn.x; acc[1] = gn.y; acc[2] = gn.z;
+        acc[3] = gm.x; acc[4] = gm.y; acc[5] = gm.z;
     }
-    // one block = one heliostat: reduce, one atomic per component
-    const float s0 = block_sum(gn.x, red), s1 = block_sum(gn.y, red), s2 = block_sum(gn.z, red);
-    const float s3 = block_sum(gm.x, red), s4 = block_sum(gm.y, red), s5 = block_sum(gm.z, red);
-    if (threadIdx.x == 0) {
-        unsafeAtomicAdd(a.g_n + 3 * h, s0);
-        unsafeAtomicAdd(a.g_n + 3 * h + 1, s1);
-        unsafeAtomicAdd(a.g_n + 3 * h + 2, s2);
-        unsafeAtomicAdd(a.g_m, s3);
-        unsafeAtomicAdd(a.g_m + 1, s4);
-        unsafeAtomicAdd(a.g_m + 2, s5);
+    // one block = a slice of one heliostat's rays: wave sums, block sum, six atomics
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) red[w][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        double s = 0.0;
+        for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) s += red[ww][k];
+        unsafeAtomicAdd(k < 3 ? a.g_n + 3 * h + k : a.g_m + (k - 3), s);
     }
 }
 
@@ -328,10 +350,12 @@ struct AdjState {
     int32_t *d_list = nullptr;
     uint32_t *d_mask = nullptr;
     size_t mask_words = 0;
-    float *d_gn = nullptr;
-    float *d_gm = nullptr;
+    double *d_gn = nullptr;
+    double *d_gm = nullptr;
     float *d_G = nullptr;
     double *d_partial = nullptr;
+    double *d_flux64 = nullptr;
+    size_t flux64_cap = 0;
     size_t npix = 0;
     bool mask_ready = false;
     AdjSun sun{};
@@ -343,13 +367,35 @@ AdjState *state(qmc_ctx *c) {
     return static_cast<AdjState *>(c->adj);
 }
 
-void frame_norms(V3 n, float &nw1, float &nw2, V3 &u, V3 &v) {
-    const V3 w1 = cross(V3{0.0f, 1.0f, 0.0f}, n);
-    nw1 = length(w1);
-    u = normalize(w1);
-    const V3 w2 = cross(u, n);
-    nw2 = length(w2);
-    v = normalize(w2);
+struct H3 {
+    double x, y, z;
+};
+H3 hsub(H3 a, H3 b) { return H3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+double hdot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+H3 hcross(H3 a, H3 b) { return H3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+H3 hnorm(H3 a, double &len) {
+    len = std::sqrt(hdot(a, a));
+    return H3{a.x / len, a.y / len, a.z / len};
+}
+void put(double *dst, H3 a) {
+    dst[0] = a.x;
+    dst[1] = a.y;
+    dst[2] = a.z;
+}
+
+// (u, v) of the frame of n with u = normalize(Y x n), v = normalize(u x n); false when n is
+// (anti)parallel to Y, where the frame is not differentiable.
+bool frame_f64(H3 n, H3 &u, H3 &v, double &nw1, double &nw2) {
+    u = hnorm(hcross(H3{0.0, 1.0, 0.0}, n), nw1);
+    v = hnorm(hcross(u, n), nw2);
+    return nw1 > 1e-9 && nw2 > 1e-9;
+}
+
+// Heliostat normal of the differentiable model: normalize(normalize(A - p) - s), in double.
+H3 helio_normal_f64(const double *A, const float *pos, const float *sun, double &la, double &lb, H3 &rr) {
+    const H3 a = hsub(H3{A[0], A[1], A[2]}, H3{pos[0], pos[1], pos[2]});
+    rr = hnorm(a, la);
+    return hnorm(hsub(rr, H3{sun[0], sun[1], sun[2]}), lb);
 }
 
 // Prepare the per-heliostat adjoint records of the current scene state for grid g.
@@ -371,22 +417,26 @@ qmc_status adj_prepare(qmc_ctx *c, int grid) {
         if (H.type != 0 || H.sub_count != 1) return fail(QMC_E_UNSUPPORTED, "adjoint: flat heliostats only");
         s->list.push_back((int32_t)h);
         AdjHelio &A = s->h_ah[h];
-        const V3 n{H.fr.nx, H.fr.ny, H.fr.nz};
-        V3 u, v;
-        frame_norms(n, A.nw1, A.nw2, u, v);
-        A.u[0] = u.x; A.u[1] = u.y; A.u[2] = u.z;
-        A.n[0] = n.x; A.n[1] = n.y; A.n[2] = n.z;
-        A.v[0] = v.x; A.v[1] = v.y; A.v[2] = v.z;
+        double la, lb;
+        H3 rr, u, v;
+        const H3 n = helio_normal_f64(&c->h_aim64[3 * h], H.pos, c->cfg.sun_dir, la, lb, rr);
+        if (!frame_f64(n, u, v, A.nw1, A.nw2))
+            return fail(QMC_E_UNSUPPORTED, "adjoint: heliostat " + std::to_string(h) + " normal parallel to the y axis");
+        put(A.u, u);
+        put(A.n, n);
+        put(A.v, v);
+        for (int k = 0; k < 3; ++k) A.p[k] = H.pos[k];
         A.ray_base = base;
         base += (((int64_t)H.F * c->L + 63) / 64) * 64;
         s->maxF = std::max(s->maxF, H.F);
     }
-    const V3 mvec{-c->cfg.sun_dir[0], -c->cfg.sun_dir[1], -c->cfg.sun_dir[2]};
-    V3 us, vs;
-    frame_norms(mvec, s->sun.nw1, s->sun.nw2, us, vs);
-    s->sun.us[0] = us.x; s->sun.us[1] = us.y; s->sun.us[2] = us.z;
-    s->sun.m[0] = mvec.x; s->sun.m[1] = mvec.y; s->sun.m[2] = mvec.z;
-    s->sun.vs[0] = vs.x; s->sun.vs[1] = vs.y; s->sun.vs[2] = vs.z;
+    const H3 mvec{-(double)c->cfg.sun_dir[0], -(double)c->cfg.sun_dir[1], -(double)c->cfg.sun_dir[2]};
+    H3 us, vs;
+    if (!frame_f64(mvec, us, vs, s->sun.nw1, s->sun.nw2))
+        return fail(QMC_E_UNSUPPORTED, "adjoint: sun direction parallel to the y axis");
+    put(s->sun.us, us);
+    put(s->sun.m, mvec);
+    put(s->sun.vs, vs);
     qmc_status st;
     const size_t words = (size_t)(base / 32) + 2;
     if (words > s->mask_words) {
@@ -429,6 +479,7 @@ AdjArgs adj_args(qmc_ctx *c, AdjState *s, float *flux) {
     a.P = c->P;
     a.L = c->L;
     a.flux = flux;
+    a.flux64 = s->d_flux64;
     a.mask = s->d_mask;
     a.G = s->d_G;
     a.g_n = s->d_gn;
@@ -454,6 +505,7 @@ void qmc_adjoint_release(qmc_ctx *c) {
     dev_free(s->d_gm);
     dev_free(s->d_G);
     dev_free(s->d_partial);
+    dev_free(s->d_flux64);
     delete s;
     c->adj = nullptr;
 }
@@ -475,8 +527,8 @@ extern "C" qmc_status qmc_adjoint_forward(qmc_ctx *c, int32_t grid, float *d_flu
     return QMC_OK;
 }
 
-extern "C" qmc_status qmc_adjoint_forward_masked(qmc_ctx *c, int32_t grid, float *d_flux) {
-    if (!c || !d_flux) return fail(QMC_E_INVALID_ARG, "qmc_adjoint_forward_masked: null argument");
+extern "C" qmc_status qmc_adjoint_forward_masked(qmc_ctx *c, int32_t grid, double *flux_out, int64_t n) {
+    if (!c || !flux_out) return fail(QMC_E_INVALID_ARG, "qmc_adjoint_forward_masked: null argument");
     AdjState *s = state(c);
     if (!s->mask_ready || s->grid != grid)
         return fail(QMC_E_STATE, "qmc_adjoint_forward_masked: run qmc_adjoint_forward on this grid first");
@@ -489,9 +541,19 @@ extern "C" qmc_status qmc_adjoint_forward_masked(qmc_ctx *c, int32_t grid, float
         if (keep[h].ray_base != s->h_ah[h].ray_base)
             return fail(QMC_E_STATE, "qmc_adjoint_forward_masked: field layout changed since the forward pass");
     const RecvBuf &rb = c->recvs[c->grids[grid].receiver];
-    HIP_TRY(hipMemsetAsync(d_flux, 0, sizeof(float) * (size_t)rb.w * rb.h, c->stream));
-    hipLaunchKernelGGL(k_adj_forward_masked, adj_grid(c, s), dim3(256), 0, c->stream, adj_args(c, s, d_flux));
+    const size_t npix = (size_t)rb.w * rb.h;
+    if (n < (int64_t)npix)
+        return fail(QMC_E_RANGE, "qmc_adjoint_forward_masked: output holds " + std::to_string(n) + " of " +
+                                     std::to_string(npix) + " pixels");
+    if (npix > s->flux64_cap) {
+        if ((st = dev_alloc(s->d_flux64, npix))) return st;
+        s->flux64_cap = npix;
+    }
+    HIP_TRY(hipMemsetAsync(s->d_flux64, 0, sizeof(double) * npix, c->stream));
+    hipLaunchKernelGGL(k_adj_forward_masked, adj_grid(c, s), dim3(256), 0, c->stream, adj_args(c, s, nullptr));
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(flux_out, s->d_flux64, sizeof(double) * npix, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return QMC_OK;
 }
 
@@ -534,45 +596,39 @@ extern "C" qmc_status qmc_adjoint_backward(qmc_ctx *c, const qmc_scene *scene, i
     if (st) return st;
     if (d_dL_dflux && d_dL_dflux != s->d_G)
         HIP_TRY(hipMemcpyAsync(s->d_G, d_dL_dflux, sizeof(float) * s->npix, hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(hipMemsetAsync(s->d_gn, 0, sizeof(float) * 3 * s->h_ah.size(), c->stream));
-    HIP_TRY(hipMemsetAsync(s->d_gm, 0, sizeof(float) * 3, c->stream));
+    HIP_TRY(hipMemsetAsync(s->d_gn, 0, sizeof(double) * 3 * s->h_ah.size(), c->stream));
+    HIP_TRY(hipMemsetAsync(s->d_gm, 0, sizeof(double) * 3, c->stream));
     hipLaunchKernelGGL(k_adj_backward, adj_grid(c, s), dim3(256), 0, c->stream, adj_args(c, s, nullptr));
     HIP_TRY(hipGetLastError());
-    std::vector<float> gn(3 * s->h_ah.size());
-    float gm[3];
-    HIP_TRY(hipMemcpyAsync(gn.data(), s->d_gn, sizeof(float) * gn.size(), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(gm, s->d_gm, sizeof(float) * 3, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> gn(3 * s->h_ah.size());
+    double gm[3];
+    HIP_TRY(hipMemcpyAsync(gn.data(), s->d_gn, sizeof(double) * gn.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(gm, s->d_gm, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    // chain n_h = normalize(normalize(A - p) - s), A = focus + o0*across + o1*up (double precision)
-    const double sx = c->cfg.sun_dir[0], sy = c->cfg.sun_dir[1], sz = c->cfg.sun_dir[2];
-    double gs[3] = {-(double)gm[0], -(double)gm[1], -(double)gm[2]};
+    // chain n_h = normalize(normalize(A - p) - s), A = focus + o0*across + o1*up, in double
+    double gs[3] = {-gm[0], -gm[1], -gm[2]};
     const int nh = (int)scene->heliostats.size();
     for (int h = 0; h < nh; ++h) {
         grad_aim[2 * h] = 0.0f;
         grad_aim[2 * h + 1] = 0.0f;
     }
-    const HostReceiver &R = scene->receivers[c->grids[grid].receiver];
-    const V3 across = normalize(R.vert[3] - R.vert[0]), up = normalize(R.vert[1] - R.vert[0]);
+    const DevReceiver &R = c->recvs[c->grids[grid].receiver].r;
+    // unit edge directions (v3 - v0 across, v1 - v0 up), rounded as the aim points were built
+    const V3 across = normalize(V3{R.E2[0], R.E2[1], R.E2[2]}), up = normalize(V3{R.E1[0], R.E1[1], R.E1[2]});
     for (int32_t h : s->list) {
-        const HostHeliostat &hh = scene->heliostats[h];
-        const double p[3] = {hh.d.pos[0], hh.d.pos[1], hh.d.pos[2]};
-        const double A[3] = {hh.aim.x, hh.aim.y, hh.aim.z};
-        double a[3] = {A[0] - p[0], A[1] - p[1], A[2] - p[2]};
-        const double la = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
-        const double rr[3] = {a[0] / la, a[1] / la, a[2] / la};
-        const double b[3] = {rr[0] - sx, rr[1] - sy, rr[2] - sz};
-        const double lb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
-        const double n[3] = {b[0] / lb, b[1] / lb, b[2] / lb};
-        const double g[3] = {gn[3 * h], gn[3 * h + 1], gn[3 * h + 2]};
-        const double ng = n[0] * g[0] + n[1] * g[1] + n[2] * g[2];
-        double gb[3];
-        for (int k = 0; k < 3; ++k) gb[k] = (g[k] - n[k] * ng) / lb;
-        for (int k = 0; k < 3; ++k) gs[k] -= gb[k];
-        const double rg = rr[0] * gb[0] + rr[1] * gb[1] + rr[2] * gb[2];
-        double ga[3];
-        for (int k = 0; k < 3; ++k) ga[k] = (gb[k] - rr[k] * rg) / la;
-        grad_aim[2 * h] = (float)(ga[0] * across.x + ga[1] * across.y + ga[2] * across.z);
-        grad_aim[2 * h + 1] = (float)(ga[0] * up.x + ga[1] * up.y + ga[2] * up.z);
+        double la, lb;
+        H3 rr;
+        const H3 n = helio_normal_f64(&c->h_aim64[3 * h], c->h_helios[h].pos, c->cfg.sun_dir, la, lb, rr);
+        const H3 g{gn[3 * h], gn[3 * h + 1], gn[3 * h + 2]};
+        const double ng = hdot(n, g);
+        const H3 gb{(g.x - n.x * ng) / lb, (g.y - n.y * ng) / lb, (g.z - n.z * ng) / lb};
+        gs[0] -= gb.x;
+        gs[1] -= gb.y;
+        gs[2] -= gb.z;
+        const double rg = hdot(rr, gb);
+        const H3 ga{(gb.x - rr.x * rg) / la, (gb.y - rr.y * rg) / la, (gb.z - rr.z * rg) / la};
+        grad_aim[2 * h] = (float)(ga.x * across.x + ga.y * across.y + ga.z * across.z);
+        grad_aim[2 * h + 1] = (float)(ga.x * up.x + ga.y * up.y + ga.z * up.z);
     }
     for (int k = 0; k < 3; ++k) grad_sun[k] = (float)gs[k];
     return QMC_OK;

@@ -37,6 +37,7 @@ struct qmc_ctx {
     std::vector<qmc::GridBuf> grids;
     std::vector<qmc::RecvBuf> recvs;
     std::vector<qmc::DevHelio> h_helios;
+    std::vector<double> h_aim64;  // aim points in double, 3 per heliostat (differentiable model)
     qmc::DevHelio *d_helios = nullptr;
     qmc::DevFacet *d_facets = nullptr;
     int32_t *d_offsets = nullptr;
@@ -131,6 +132,7 @@ inline void free_scene(qmc_ctx *c) {
     c->batch_n = 0;
     qmc_adjoint_release(c);  // visibility masks belong to the uploaded scene
     c->h_helios.clear();
+    c->h_aim64.clear();
     c->uploaded = false;
     c->offsets_ready = false;
 }

@@ -587,6 +587,7 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     // heliostats and focus-flat facets
     std::vector<DevFacet> facets;
     int64_t off_base = pack_heliostats(*s, c->h_helios, facets);
+    aim_points_f64(*s, c->h_aim64);
     c->offsets_total = off_base;
     if ((st = dev_alloc(c->d_helios, c->h_helios.size()))) return st;
     if ((st = dev_alloc(c->d_facets, facets.size()))) return st;
@@ -655,6 +656,7 @@ extern "C" qmc_status qmc_update(qmc_ctx *c, const qmc_scene *s) {
     V3 sun{n.sun_dir[0], n.sun_dir[1], n.sun_dir[2]};
     c->sun = to_dev(make_frame(-sun, false));
     c->h_helios = helios;
+    aim_points_f64(*s, c->h_aim64);
     HIP_TRY(hipMemcpyAsync(c->d_helios, helios.data(), helios.size() * sizeof(DevHelio), hipMemcpyHostToDevice,
                            c->stream));
     if (!facets.empty())

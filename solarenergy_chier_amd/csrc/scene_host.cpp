@@ -915,6 +915,27 @@ V3 heliostat_aim(const qmc_scene &s, int h, const float off[2]) {
     const HostReceiver &r = s.receivers[s.grids[hh.d.grid].d.receiver];
     return aim_point(r, f3(hh.d.pos), off);
 }
+
+void aim_points_f64(const qmc_scene &s, std::vector<double> &out) {
+    out.assign(3 * s.heliostats.size(), 0.0);
+    for (size_t h = 0; h < s.heliostats.size(); ++h) {
+        const HostHeliostat &hh = s.heliostats[h];
+        const HostReceiver &r = s.receivers[s.grids[hh.d.grid].d.receiver];
+        double *a = &out[3 * h];
+        if (r.d.type != 0) {
+            a[0] = hh.aim.x;
+            a[1] = hh.aim.y;
+            a[2] = hh.aim.z;
+            continue;
+        }
+        const V3 f = receiver_focus(r, f3(hh.d.pos));
+        const V3 across = normalize(r.vert[3] - r.vert[0]), up = normalize(r.vert[1] - r.vert[0]);
+        const double o0 = hh.d.aim_offset[0], o1 = hh.d.aim_offset[1];
+        a[0] = (double)f.x + (double)across.x * o0 + (double)up.x * o1;
+        a[1] = (double)f.y + (double)across.y * o0 + (double)up.y * o1;
+        a[2] = (double)f.z + (double)across.z * o0 + (double)up.z * o1;
+    }
+}
 }  // namespace qmc
 
 extern "C" qmc_status qmc_scene_set_aim_offsets(qmc_scene *s, const float *off, int32_t n) {

@@ -76,5 +76,9 @@ float heliostat_factor(const qmc_scene &s, int h);
 void heliostat_factor_parts(const qmc_scene &s, int h, int &rule, float &x);
 // Aim point of heliostat h of a processed scene for an aim offset (across, up) in the receiver.
 V3 heliostat_aim(const qmc_scene &s, int h, const float off[2]);
+// Aim points in double (3 per heliostat): for rectangle receivers focus + o0*across + o1*up with the
+// float focus, unit edge directions and offsets promoted before the arithmetic (the differentiable
+// model's parameterisation); for cylinder receivers the float aim point.
+void aim_points_f64(const qmc_scene &s, std::vector<double> &out);
 void sunshape_params(const qmc_config &c, std::vector<float> &cdf_xy, std::vector<float> &slopes, float abcig[5]);
 }  // namespace qmc

@@ -434,10 +434,17 @@ class Device:
     def _dptr(self, ptr: int):
         return C.cast(C.c_void_p(ptr), N._FP)
 
-    def adjoint_forward(self, grid: int, d_flux: int, masked: bool = False) -> None:
-        """Bilinear-deposit forward into the device buffer d_flux (visibility recorded, or frozen)."""
-        name = "qmc_adjoint_forward_masked" if masked else "qmc_adjoint_forward"
-        N.call(name, self.handle, grid, self._dptr(d_flux))
+    def adjoint_forward(self, grid: int, d_flux: int) -> None:
+        """Bilinear-deposit forward into the device buffer d_flux; records the visibility mask."""
+        N.call("qmc_adjoint_forward", self.handle, grid, self._dptr(d_flux))
+
+    def adjoint_forward_masked(self, grid: int, receiver: int = 0) -> np.ndarray:
+        """Frozen-visibility forward at the current parameters, accumulated in double; returns the
+        map of `receiver` (the grid's receiver)."""
+        h, w = self.flux_shape(receiver)
+        out = np.zeros((h, w), np.float64)
+        N.call("qmc_adjoint_forward_masked", self.handle, grid, out.ctypes.data_as(C.POINTER(C.c_double)), out.size)
+        return out
 
     def flux_loss(self, d_flux: int, n: int, target: float, peak: float, lam: float, d_grad: int | None = None):
         loss = C.c_double(0.0)
