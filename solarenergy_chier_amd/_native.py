@@ -97,10 +97,10 @@ class Instance(C.Structure):
 
 class Schedule(C.Structure):
     _fields_ = [("block_threads", C.c_int32), ("workgroups_per_cu", C.c_int32), ("lds_flux_tile", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("facet_tiles", C.c_int32)]
 
     def as_dict(self) -> dict:
-        return {name: int(getattr(self, name)) for name, _ in self._fields_[:3]}
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
 RECORD_DTYPE = [("status", "<i4"), ("index", "<i4"), ("energy", "<f4"), ("flags", "<i4")]

@@ -22,7 +22,11 @@ LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libqmcrt.so"
 
 HIP_SOURCES = ["qmc_device.hip", "qmc_batch.hip", "qmc_adjoint.hip"]
-HOST_SOURCES = ["scene_host.cpp", "facade.cpp"]
+HOST_SOURCES = ["scene_host.cpp"]
+# The persistent batched kernel loops over rays; machine-level loop-invariant code motion hoists
+# copies of wave-uniform operands out of that loop into vector registers (about 25 of them), which
+# costs two waves per SIMD of occupancy on a latency-bound traversal.
+EXTRA_FLAGS = {"qmc_batch.hip": ["-mllvm", "-disable-machine-licm"]}
 
 ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")
 
@@ -51,15 +55,13 @@ def build_qmcrt(verbose: bool = False) -> Path:
     for src in HIP_SOURCES:
         obj = objdir / (Path(src).stem + ".o")
         cmd = [hipcc(), "-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-               "-packed-fp32-ops", "-fgpu-rdc" if False else "-fno-gpu-rdc", *common, "-c", str(CSRC / src),
+               "-packed-fp32-ops", "-fno-gpu-rdc", *common, *EXTRA_FLAGS.get(src, []), "-c", str(CSRC / src),
                "-o", str(obj)]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)
         objs.append(obj)
     for src in HOST_SOURCES:
-        if not (CSRC / src).exists():
-            continue
         obj = objdir / (Path(src).stem + ".o")
         cmd = [hipcc(), "-x", "c++", *common, "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:

@@ -1,15 +1,19 @@
-// qmc_batch.hip — batched traces: many sun positions x aim strategies of one heliostat field in one
-// persistent launch, each instance with its own flux map (no counterpart in the reference, which
+// qmc_batch.hip — batched traces: many sun positions x aim strategies of one heliostat field in
+// persistent launches, each instance with its own flux map (no counterpart in the reference, which
 // traces one configuration per process; its per-ray work is map_raytracing,
 // QuasiMonteCarloRayTracing/RectangleReceiverRectangleGridRayTracing.cu:6-41).
 //
-// A workgroup owns whole instances. For each instance it recomputes, in LDS, what depends on the sun
-// and the aim point — heliostat normals and frames (Heliostat::CSetNormal, Heliostat.cu:15-20),
-// blocking quads (CSetWorldVertex, Heliostat.cu:25-40), energy factors
-// (QuasiMonteCarloRayTracer.cpp:58-60) — with the same float operations the host processor uses, then
-// traces every ray of the field with the shared per-ray functions and accumulates the instance's map
-// in LDS (one plain store per pixel at the end) or with global atomics when the map does not fit.
-// The grid cell lists, sample pools and micro-facet offsets are shared by all instances.
+// Two kernels per chunk of instances. k_batch_setup recomputes, for every (instance, heliostat), what
+// depends on the sun and the aim point — heliostat normals and frames (Heliostat::CSetNormal,
+// Heliostat.cu:15-20), blocking quads (CSetWorldVertex, Heliostat.cu:25-40), energy factors
+// (QuasiMonteCarloRayTracer.cpp:58-60) — with the same float operations the host processor uses, into
+// per-instance arrays in HBM. k_trace_batched is persistent: a workgroup owns whole instances, walks
+// the instance's heliostats in order (their frames and factors are wave-uniform scalar loads) and
+// traces every ray with the shared per-ray functions, accumulating the instance's map in LDS (one
+// plain store per pixel at the end) or, when the map does not fit or the schedule says so, with
+// workgroup-scope atomics into the instance's map in HBM (no other workgroup touches it, so the
+// atomics complete in the XCD's L2). Grid cell lists, sample pools and micro-facet offsets are shared
+// by all instances.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,116 +35,135 @@ using namespace qmc;
 
 namespace {
 
+// Instances whose per-heliostat data (frames, factors, quads: ~92 B per heliostat) is resident at
+// once; the chunk is as large as this budget allows, so a sweep normally runs as one launch pair.
+constexpr size_t kWorkBudget = size_t(2) << 30;
+
 struct BatchArgs {
     DevGrid grid;
     DevReceiver recv;
-    const DevHelio *helios;
-    int32_t helio_begin, n_helio, n_total_helio;
-    const float *aims;  // [set][heliostat][3]
-    const qmc_instance *inst;
+    const DevHelio *__restrict__ helios;
+    int32_t helio_begin, n_helio, n_total_helio, n_sub;
+    const float *__restrict__ aims;  // [set][heliostat][3]
+    const qmc_instance *__restrict__ inst;
     int32_t n_inst;
-    const float4 *pool;
-    const float4 *pert;
-    const int32_t *offsets;
+    const float4 *__restrict__ pool;
+    const float4 *__restrict__ pert;
+    const int32_t *__restrict__ offsets;
     int32_t P, L;
     float rho;
     float *maps;
     int32_t npix;
+    // per-instance data of the chunk (written by k_batch_setup, read by k_trace_batched)
+    DevFrame *sun_fr;  // [instance]
+    DevFrame *fr;      // [instance][heliostat]
+    float *fac;        // [instance][heliostat]
+    float4 *quads;     // [instance][sub-heliostat][3]
+    int32_t *queue;    // next instance to take (reset by k_batch_setup)
 };
 
-struct LdsLayout {
-    size_t quads, frames, factors, sun, map, total;
-};
-
-inline LdsLayout lds_layout(int n_helio, int npix, bool lds_map) {
-    LdsLayout l;
-    l.quads = 0;
-    l.frames = l.quads + sizeof(float4) * 3 * (size_t)n_helio;
-    l.factors = l.frames + sizeof(DevFrame) * (size_t)n_helio;
-    l.sun = l.factors + sizeof(float) * (size_t)n_helio;
-    l.map = (l.sun + sizeof(DevFrame) + 15) & ~(size_t)15;
-    l.total = l.map + (lds_map ? sizeof(float) * (size_t)npix : 0);
-    return l;
+__device__ __forceinline__ DevFrame dev_frame(const Frame &f) {
+    DevFrame d;
+    d.ux = f.u.x; d.uy = f.u.y; d.uz = f.u.z;
+    d.nx = f.n.x; d.ny = f.n.y; d.nz = f.n.z;
+    d.vx = f.v.x; d.vy = f.v.y; d.vz = f.v.z;
+    d.identity = f.identity;
+    return d;
 }
 
-template <int RT, bool LDSMAP>
-__global__ void k_trace_batched(BatchArgs a, LdsLayout lay) {
-    extern __shared__ float4 smem4[];
-    char *smem = reinterpret_cast<char *>(smem4);
-    float4 *q_lds = reinterpret_cast<float4 *>(smem + lay.quads);
-    DevFrame *fr_lds = reinterpret_cast<DevFrame *>(smem + lay.frames);
-    float *fac_lds = reinterpret_cast<float *>(smem + lay.factors);
-    DevFrame *sun_lds = reinterpret_cast<DevFrame *>(smem + lay.sun);
-    float *map_lds = reinterpret_cast<float *>(smem + lay.map);
-    const int tid = threadIdx.x, bs = blockDim.x;
+__global__ __launch_bounds__(256) void k_batch_setup(BatchArgs a) {
+    const int ii = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const qmc_instance I = a.inst[ii];
+    const V3 sun{I.sun_dir[0], I.sun_dir[1], I.sun_dir[2]};
+    if (k == 0) a.sun_fr[ii] = dev_frame(make_frame(-sun, false));
+    if (k == 0 && ii == 0) *a.queue = 0;
+    if (k >= a.n_helio) return;
+    const DevHelio &H = a.helios[a.helio_begin + k];
+    const float *ap = a.aims + ((size_t)I.aim_set * a.n_total_helio + a.helio_begin + k) * 3;
+    const V3 aim{ap[0], ap[1], ap[2]};
+    const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
+    const V3 normal = normalize(normalize(aim - pos) - sun);
+    const Frame f = make_frame(normal, false);
+    a.fr[(size_t)ii * a.n_helio + k] = dev_frame(f);
+    // flat heliostat blocking quad: world vertices v0, v1, v2 -> (A, E1, E2) = (v1, v0-v1, v2-v1)
+    const V3 v0l{-H.size[0] / 2, H.size[1] / 2, -H.size[2] / 2};
+    const V3 v1l = V3{0, 0, H.size[2]} + v0l;
+    const V3 v2l = V3{H.size[0], 0, H.size[2]} + v0l;
+    const V3 w0 = frame_apply(f, v0l) + pos, w1 = frame_apply(f, v1l) + pos, w2 = frame_apply(f, v2l) + pos;
+    const V3 E1 = w0 - w1, E2 = w2 - w1;
+    float4 *q = a.quads + ((size_t)ii * a.n_sub + H.sub_begin) * 3;
+    q[0] = make_float4(w1.x, w1.y, w1.z, 0.0f);
+    q[1] = make_float4(E1.x, E1.y, E1.z, 0.0f);
+    q[2] = make_float4(E2.x, E2.y, E2.z, 0.0f);
+    a.fac[(size_t)ii * a.n_helio + k] = H.fac_rule == 0 ? I.dni * H.fac_x * H.fac_x * a.rho / float(a.L)
+                                                          : I.dni * H.fac_x * a.rho / float(a.L);
+}
+
+// Load of data that no kernel of the launch writes (per-heliostat and per-instance records) through
+// the constant address space: with a wave-uniform address it becomes a scalar load into SGPRs.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "records are whole dwords");
+    typedef const __attribute__((address_space(4))) uint32_t cdword;
+    cdword *src = reinterpret_cast<cdword *>(reinterpret_cast<uintptr_t>(p));
+    T out;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
+    return out;
+}
+
+template <int RT, bool LDSMAP, bool TILES, int BS>
+__global__ __launch_bounds__(BS) void k_trace_batched(BatchArgs a) {
+    extern __shared__ float map_lds[];
+    __shared__ int s_next;
+    const int tid = threadIdx.x, bs = BS;
     const int L = a.L, P = a.P;
     RayStats st{0u, 0u, 0u};
 
-    for (int ii = blockIdx.x; ii < a.n_inst; ii += gridDim.x) {
-        const qmc_instance I = a.inst[ii];
-        const V3 sun{I.sun_dir[0], I.sun_dir[1], I.sun_dir[2]};
-        __syncthreads();  // the previous instance is done with LDS
-        if (tid == 0) {
-            const Frame f = make_frame(-sun, false);
-            DevFrame d;
-            d.ux = f.u.x; d.uy = f.u.y; d.uz = f.u.z;
-            d.nx = f.n.x; d.ny = f.n.y; d.nz = f.n.z;
-            d.vx = f.v.x; d.vy = f.v.y; d.vz = f.v.z;
-            d.identity = f.identity;
-            *sun_lds = d;
-        }
-        for (int k = tid; k < a.n_helio; k += bs) {
-            const DevHelio &H = a.helios[a.helio_begin + k];
-            const float *ap = a.aims + ((size_t)I.aim_set * a.n_total_helio + a.helio_begin + k) * 3;
-            const V3 aim{ap[0], ap[1], ap[2]};
-            const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
-            const V3 normal = normalize(normalize(aim - pos) - sun);
-            const Frame f = make_frame(normal, false);
-            DevFrame d;
-            d.ux = f.u.x; d.uy = f.u.y; d.uz = f.u.z;
-            d.nx = f.n.x; d.ny = f.n.y; d.nz = f.n.z;
-            d.vx = f.v.x; d.vy = f.v.y; d.vz = f.v.z;
-            d.identity = f.identity;
-            fr_lds[k] = d;
-            // flat heliostat blocking quad: world vertices v0, v1, v2 -> (A, E1, E2) = (v1, v0-v1, v2-v1)
-            const V3 v0l{-H.size[0] / 2, H.size[1] / 2, -H.size[2] / 2};
-            const V3 v1l = V3{0, 0, H.size[2]} + v0l;
-            const V3 v2l = V3{H.size[0], 0, H.size[2]} + v0l;
-            const V3 w0 = frame_apply(f, v0l) + pos, w1 = frame_apply(f, v1l) + pos, w2 = frame_apply(f, v2l) + pos;
-            const V3 E1 = w0 - w1, E2 = w2 - w1;
-            const int sid = H.sub_begin;
-            q_lds[3 * sid] = make_float4(w1.x, w1.y, w1.z, 0.0f);
-            q_lds[3 * sid + 1] = make_float4(E1.x, E1.y, E1.z, 0.0f);
-            q_lds[3 * sid + 2] = make_float4(E2.x, E2.y, E2.z, 0.0f);
-            fac_lds[k] = H.fac_rule == 0 ? I.dni * H.fac_x * H.fac_x * a.rho / float(L)
-                                         : I.dni * H.fac_x * a.rho / float(L);
-        }
-        if (LDSMAP)
-            for (int p = tid; p < a.npix; p += bs) map_lds[p] = 0.0f;
+    for (;;) {
+        // instances are taken from a queue, so uneven instance costs do not leave workgroups idle;
+        // the index is read from LDS, so every wave of the workgroup leaves the loop together
+        __syncthreads();  // the previous instance (and its map store) is done
+        if (tid == 0) s_next = atomicAdd(a.queue, 1);
         __syncthreads();
-        const DevFrame sunf = *sun_lds;
+        const int ii = s_next;
+        if (ii >= a.n_inst) break;
+        if (LDSMAP) {
+            for (int p = tid; p < a.npix; p += bs) map_lds[p] = 0.0f;
+            __syncthreads();
+        }
+        const DevFrame sunf = ld_uniform(a.sun_fr + ii);
+        const float4 *quads = a.quads + (size_t)ii * a.n_sub * 3;
         float *gmap = a.maps + (size_t)ii * a.npix;
 
+#pragma nounroll
         for (int k = 0; k < a.n_helio; ++k) {
-            const DevHelio &H = a.helios[a.helio_begin + k];
-            const DevFrame fr = fr_lds[k];
-            const float factor = fac_lds[k];
+            const DevHelio H = ld_uniform(a.helios + a.helio_begin + k);
+            const DevFrame fr = ld_uniform(a.fr + (size_t)ii * a.n_helio + k);
+            const float factor = ld_uniform(a.fac + (size_t)ii * a.n_helio + k);
             const int32_t *grp = a.offsets + H.offsets_base;
             const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
-            const int nr = H.F * L;
-            for (int r = tid; r < nr; r += bs) {
+            const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
+            const FacetTiling tl = facet_tiling(R, C, L);
+            const int nslots = TILES ? tl.slots : H.F * L;
+#pragma nounroll
+            for (int slot = tid; slot < nslots; slot += bs) {
+                const int r = TILES ? facet_tiled_ray(tl, R, C, slot) : slot;
+                if (r < 0) continue;
                 const int m = r / L, j = r % L;
                 const int a1 = (grp[m] + j) % P;
                 const float4 s = a.pool[a1];
                 V3 dir = fr_apply(sunf, V3{s.x, s.y, s.z});
                 const V3 origin = flat_facet_origin(H, fr, m);
                 int hit;
-                if (dda_collision<false>(origin, dir, a.grid, q_lds, own_lo, own_hi, hit, st)) continue;
+                if (dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
                 const int a2 = (grp[(m + 1) % P] + j) % P;
                 const float4 q = a.pert[a2];
                 const V3 normal = normalize(fr_apply(fr, V3{q.x, q.y, q.z}));
                 dir = normalize(reflect(-dir, normal));
-                if (dda_collision<false>(origin, dir, a.grid, q_lds, own_lo, own_hi, hit, st)) continue;
+                if (dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
                 float t, u, v;
                 if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v)) continue;
                 const float energy = ray_energy(t, dir, normal, factor);
@@ -152,7 +175,7 @@ __global__ void k_trace_batched(BatchArgs a, LdsLayout lay) {
                 if (LDSMAP)
                     atomicAdd(map_lds + addr, energy);
                 else
-                    unsafeAtomicAdd(gmap + addr, energy);
+                    __hip_atomic_fetch_add(gmap + addr, energy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
         if (LDSMAP) {
@@ -177,7 +200,7 @@ void tune_load_file() {
     std::ifstream f(path);
     std::string key;
     qmc_schedule s;
-    while (f >> key >> s.block_threads >> s.workgroups_per_cu >> s.lds_flux_tile >> s.reserved) g_tune[key] = s;
+    while (f >> key >> s.block_threads >> s.workgroups_per_cu >> s.lds_flux_tile >> s.facet_tiles) g_tune[key] = s;
 }
 
 void tune_store(const std::string &key, const qmc_schedule &s) {
@@ -185,7 +208,7 @@ void tune_store(const std::string &key, const qmc_schedule &s) {
     const char *path = std::getenv("QMC_TUNE_CACHE");
     if (!path) return;
     std::ofstream f(path, std::ios::app);
-    f << key << " " << s.block_threads << " " << s.workgroups_per_cu << " " << s.lds_flux_tile << " " << s.reserved
+    f << key << " " << s.block_threads << " " << s.workgroups_per_cu << " " << s.lds_flux_tile << " " << s.facet_tiles
       << "\n";
 }
 
@@ -216,23 +239,112 @@ struct BatchPlan {
     int rt;
 };
 
+size_t lds_bytes(const BatchArgs &a, bool lds_map) { return lds_map ? sizeof(float) * (size_t)a.npix : 0; }
+
+// Per-instance data of one chunk, carved out of one ctx-owned allocation.
+size_t work_bytes_per_instance(const BatchArgs &a) {
+    return sizeof(DevFrame) * (size_t)(a.n_helio + 1) + sizeof(float) * a.n_helio + sizeof(float4) * 3 * a.n_sub;
+}
+
+qmc_status ensure_work(qmc_ctx *c, BatchArgs &a, int chunk) {
+    const size_t fr = sizeof(DevFrame) * (size_t)chunk * a.n_helio;
+    const size_t fac = sizeof(float) * (size_t)chunk * a.n_helio;
+    const size_t quads = sizeof(float4) * 3 * (size_t)chunk * a.n_sub;
+    const size_t sun = sizeof(DevFrame) * (size_t)chunk;
+    auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t total = up(quads) + up(fr) + up(fac) + up(sun) + 256;
+    if (total > c->batch_work_cap) {
+        qmc_status st = dev_alloc(c->d_batch_work, total);
+        if (st) {
+            c->batch_work_cap = 0;
+            return st;
+        }
+        c->batch_work_cap = total;
+    }
+    char *p = c->d_batch_work;
+    a.quads = reinterpret_cast<float4 *>(p);
+    p += up(quads);
+    a.fr = reinterpret_cast<DevFrame *>(p);
+    p += up(fr);
+    a.fac = reinterpret_cast<float *>(p);
+    p += up(fac);
+    a.sun_fr = reinterpret_cast<DevFrame *>(p);
+    p += up(sun);
+    a.queue = reinterpret_cast<int32_t *>(p);
+    return QMC_OK;
+}
+
+template <int RT, bool LDSMAP, bool TILES>
+void launch_trace_bs(int bs, dim3 g, size_t lds, hipStream_t st, const BatchArgs &b) {
+    if (bs == 256) hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, TILES, 256>), g, dim3(256), lds, st, b);
+    else if (bs == 512) hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, TILES, 512>), g, dim3(512), lds, st, b);
+    else hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, TILES, 1024>), g, dim3(1024), lds, st, b);
+}
+
+template <int RT>
+void launch_trace_rt(bool lds_map, bool tiles, int bs, dim3 g, size_t lds, hipStream_t st, const BatchArgs &b) {
+    if (lds_map) {
+        if (tiles) launch_trace_bs<RT, true, true>(bs, g, lds, st, b);
+        else launch_trace_bs<RT, true, false>(bs, g, lds, st, b);
+    } else {
+        if (tiles) launch_trace_bs<RT, false, true>(bs, g, lds, st, b);
+        else launch_trace_bs<RT, false, false>(bs, g, lds, st, b);
+    }
+}
+
+void launch_trace(int rt, bool lds_map, bool tiles, int bs, dim3 g, size_t lds, hipStream_t st, const BatchArgs &b) {
+    if (rt == 0) launch_trace_rt<0>(lds_map, tiles, bs, g, lds, st, b);
+    else launch_trace_rt<1>(lds_map, tiles, bs, g, lds, st, b);
+}
+
+template <int RT, bool LDSMAP, bool TILES>
+hipError_t occupancy_bs(int bs, size_t lds, int &n) {
+    if (bs == 256)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, TILES, 256>, 256, lds);
+    if (bs == 512)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, TILES, 512>, 512, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, TILES, 1024>, 1024, lds);
+}
+
+template <int RT>
+hipError_t occupancy_rt(bool lds_map, bool tiles, int bs, size_t lds, int &n) {
+    if (lds_map) return tiles ? occupancy_bs<RT, true, true>(bs, lds, n) : occupancy_bs<RT, true, false>(bs, lds, n);
+    return tiles ? occupancy_bs<RT, false, true>(bs, lds, n) : occupancy_bs<RT, false, false>(bs, lds, n);
+}
+
+qmc_status resident_per_cu(int rt, bool lds_map, bool tiles, int bs, size_t lds, int &n) {
+    const hipError_t e = rt == 0 ? occupancy_rt<0>(lds_map, tiles, bs, lds, n) : occupancy_rt<1>(lds_map, tiles, bs, lds, n);
+    if (e != hipSuccess) return fail(QMC_E_HIP, std::string("occupancy query: ") + hipGetErrorString(e));
+    if (n < 1) return fail(QMC_E_UNSUPPORTED, "batched trace: the schedule does not fit on a CU");
+    return QMC_OK;
+}
+
+// Trace instances [0, n_inst) of the plan into plan.args.maps, chunk by chunk.
 qmc_status launch_batched(qmc_ctx *c, const BatchPlan &plan, const qmc_schedule &s, int n_inst) {
     BatchArgs a = plan.args;
-    a.n_inst = n_inst;
     const bool lds_map = s.lds_flux_tile != 0;
-    const LdsLayout lay = lds_layout(a.n_helio, a.npix, lds_map);
-    if (lay.total > c->max_lds) return fail(QMC_E_UNSUPPORTED, "batched trace: LDS layout exceeds the device limit");
+    const size_t lds = lds_bytes(a, lds_map);
+    if (lds > c->max_lds) return fail(QMC_E_UNSUPPORTED, "batched trace: the flux map does not fit in LDS");
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n_inst, kWorkBudget / work_bytes_per_instance(a)));
+    qmc_status st = ensure_work(c, a, chunk);
+    if (st) return st;
     if (!lds_map) HIP_TRY(hipMemsetAsync(a.maps, 0, sizeof(float) * (size_t)n_inst * a.npix, c->stream));
-    const int grid = std::max(1, std::min(n_inst, c->num_cus * std::max(1, s.workgroups_per_cu)));
-    dim3 g(grid), b(s.block_threads);
-    if (plan.rt == 0) {
-        if (lds_map) hipLaunchKernelGGL((k_trace_batched<0, true>), g, b, lay.total, c->stream, a, lay);
-        else hipLaunchKernelGGL((k_trace_batched<0, false>), g, b, lay.total, c->stream, a, lay);
-    } else {
-        if (lds_map) hipLaunchKernelGGL((k_trace_batched<1, true>), g, b, lay.total, c->stream, a, lay);
-        else hipLaunchKernelGGL((k_trace_batched<1, false>), g, b, lay.total, c->stream, a, lay);
+    for (int base = 0; base < n_inst; base += chunk) {
+        BatchArgs b = a;
+        b.n_inst = std::min(chunk, n_inst - base);
+        b.inst = a.inst + base;
+        b.maps = a.maps + (size_t)base * a.npix;
+        hipLaunchKernelGGL(k_batch_setup, dim3((unsigned)((a.n_helio + 255) / 256), (unsigned)b.n_inst), dim3(256), 0,
+                           c->stream, b);
+        // no more workgroups than can be resident: a queue-driven persistent grid
+        int resident = 0;
+        const bool tiles = s.facet_tiles != 0;
+        if ((st = resident_per_cu(plan.rt, lds_map, tiles, s.block_threads, lds, resident))) return st;
+        const int per_cu = std::max(1, std::min(s.workgroups_per_cu, resident));
+        const int grid = std::max(1, std::min(b.n_inst, c->num_cus * per_cu));
+        launch_trace(plan.rt, lds_map, tiles, s.block_threads, dim3(grid), lds, c->stream, b);
+        HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipGetLastError());
     return QMC_OK;
 }
 
@@ -261,6 +373,7 @@ qmc_status plan_batch(qmc_ctx *c, int grid, const qmc_instance *inst, int n, Bat
                 return fail(QMC_E_UNSUPPORTED, "batched trace: flat heliostats only (one blocking quad each)");
         }
     if (count == 0) return fail(QMC_E_INVALID_ARG, "batched trace: grid has no heliostats");
+    if (gb.n_sub != count) return fail(QMC_E_UNSUPPORTED, "batched trace: one blocking quad per heliostat expected");
     BatchArgs &a = plan.args;
     std::memset(&a, 0, sizeof(a));
     a.grid = gb.g;
@@ -269,6 +382,7 @@ qmc_status plan_batch(qmc_ctx *c, int grid, const qmc_instance *inst, int n, Bat
     a.helio_begin = begin;
     a.n_helio = count;
     a.n_total_helio = (int)c->h_helios.size();
+    a.n_sub = gb.n_sub;
     a.aims = c->d_aims;
     a.pool = c->d_pool;
     a.pert = c->d_pert;
@@ -288,6 +402,7 @@ qmc_status plan_batch(qmc_ctx *c, int grid, const qmc_instance *inst, int n, Bat
     }
     HIP_TRY(hipMemcpyAsync(c->d_batch_inst, inst, sizeof(qmc_instance) * n, hipMemcpyHostToDevice, c->stream));
     a.inst = reinterpret_cast<const qmc_instance *>(c->d_batch_inst);
+    a.n_inst = n;
     return QMC_OK;
 }
 
@@ -316,35 +431,57 @@ qmc_status autotune(qmc_ctx *c, const BatchPlan &plan, int n, qmc_schedule &best
             return QMC_OK;
         }
     }
+    // Candidates: block size x LDS map x facet tiles, each with as many workgroups per CU as are
+    // resident (the occupancy query). A first round screens them on one instance per workgroup; the
+    // two fastest are then timed on the same number of instances (three rounds of the larger grid),
+    // which ranks them in the steady state the sweep runs in.
     const size_t lds_per_cu = 160 * 1024;
-    const int trial = std::min(n, c->num_cus * 4);
-    qmc_status st = ensure_maps(c, (size_t)trial * plan.args.npix);
-    if (st) return st;
-    BatchPlan p = plan;
-    p.args.maps = c->d_batch_maps;
-    float best_ms = 1e30f;
-    bool found = false;
+    struct Cand {
+        qmc_schedule s;
+        float rate;
+    };
+    std::vector<Cand> cands;
+    int max_grid = 0;
     for (int bs : {256, 512, 1024})
         for (int lds : {1, 0})
-            for (int wpc : {1, 2, 4}) {
-                const LdsLayout lay = lds_layout(p.args.n_helio, p.args.npix, lds != 0);
-                if (lay.total > c->max_lds || lay.total * wpc > lds_per_cu) continue;
-                if (bs * wpc > 2048) continue;
-                qmc_schedule s{bs, wpc, lds, 0};
-                HIP_TRY(hipEventRecord(c->ev[14], c->stream));
-                if ((st = launch_batched(c, p, s, trial))) return st;
-                HIP_TRY(hipEventRecord(c->ev[15], c->stream));
-                HIP_TRY(hipEventSynchronize(c->ev[15]));
-                float ms = 0;
-                HIP_TRY(hipEventElapsedTime(&ms, c->ev[14], c->ev[15]));
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = s;
-                    found = true;
-                }
+            for (int tiles : {0, 1}) {
+                const size_t bytes = lds_bytes(plan.args, lds != 0);
+                if (bytes > c->max_lds) continue;
+                int resident = 0;
+                qmc_status st = resident_per_cu(plan.rt, lds != 0, tiles != 0, bs, bytes, resident);
+                if (st) return st;
+                resident = std::min(resident, 2048 / bs);  // 32 waves per CU
+                if (bytes > 0) resident = std::min<int>(resident, (int)(lds_per_cu / bytes));
+                if (resident < 1) continue;
+                cands.push_back(Cand{qmc_schedule{bs, resident, lds, tiles}, 0.0f});
+                max_grid = std::max(max_grid, c->num_cus * resident);
             }
-    if (!found) return fail(QMC_E_UNSUPPORTED, "batched trace: no schedule fits this device");
-    best.reserved = 0;
+    if (cands.empty()) return fail(QMC_E_UNSUPPORTED, "batched trace: no schedule fits this device");
+    const int long_trial = std::min(n, 3 * max_grid);
+    qmc_status st = ensure_maps(c, (size_t)std::max(long_trial, std::min(n, max_grid)) * plan.args.npix);
+    if (st) return st;
+    c->batch_n = 0;  // the trial traces overwrite the ctx-owned maps
+    BatchPlan p = plan;
+    p.args.maps = c->d_batch_maps;
+    auto time_it = [&](const qmc_schedule &s, int trial, float &rate) -> qmc_status {
+        HIP_TRY(hipEventRecord(c->ev[14], c->stream));
+        qmc_status e = launch_batched(c, p, s, trial);
+        if (e) return e;
+        HIP_TRY(hipEventRecord(c->ev[15], c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev[15]));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[14], c->ev[15]));
+        rate = ms > 0.0f ? (float)trial / ms : 0.0f;
+        return QMC_OK;
+    };
+    for (Cand &cd : cands)
+        if ((st = time_it(cd.s, std::min(n, c->num_cus * cd.s.workgroups_per_cu), cd.rate))) return st;
+    std::sort(cands.begin(), cands.end(), [](const Cand &x, const Cand &y) { return x.rate > y.rate; });
+    const size_t finalists = std::min<size_t>(2, cands.size());
+    for (size_t i = 0; i < finalists; ++i)
+        if ((st = time_it(cands[i].s, long_trial, cands[i].rate))) return st;
+    std::sort(cands.begin(), cands.begin() + finalists, [](const Cand &x, const Cand &y) { return x.rate > y.rate; });
+    best = cands[0].s;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     tune_store(key, best);
     return QMC_OK;
@@ -399,14 +536,18 @@ extern "C" qmc_status qmc_trace_batched(qmc_ctx *c, int32_t grid, const qmc_inst
     qmc_schedule s;
     if (sched) {
         s = *sched;
-        if (s.block_threads != 64 && s.block_threads != 128 && s.block_threads != 256 && s.block_threads != 512 &&
-            s.block_threads != 1024)
-            return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: block_threads must be 64..1024, a power of two");
+        if (s.block_threads != 256 && s.block_threads != 512 && s.block_threads != 1024)
+            return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: block_threads must be 256, 512 or 1024");
         if (s.workgroups_per_cu < 1 || s.workgroups_per_cu > 16)
             return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: workgroups_per_cu must be 1..16");
+        if (s.lds_flux_tile != 0 && s.lds_flux_tile != 1)
+            return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: lds_flux_tile must be 0 or 1");
+        if (s.facet_tiles != 0 && s.facet_tiles != 1)
+            return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: facet_tiles must be 0 or 1");
     } else if ((st = autotune(c, plan, n, s))) {
         return st;
     }
+    c->batch_n = 0;
     if (d_maps) {
         plan.args.maps = d_maps;
     } else {
@@ -414,6 +555,7 @@ extern "C" qmc_status qmc_trace_batched(qmc_ctx *c, int32_t grid, const qmc_inst
         plan.args.maps = c->d_batch_maps;
     }
     if ((st = launch_batched(c, plan, s, n))) return st;
+    c->batch_maps_caller = d_maps != nullptr;
     c->batch_n = n;
     c->batch_npix = plan.args.npix;
     return QMC_OK;
@@ -421,6 +563,8 @@ extern "C" qmc_status qmc_trace_batched(qmc_ctx *c, int32_t grid, const qmc_inst
 
 extern "C" qmc_status qmc_batch_read(qmc_ctx *c, int32_t first, int32_t count, float *out) {
     if (!c || !out) return fail(QMC_E_INVALID_ARG, "qmc_batch_read: null argument");
+    if (c->batch_maps_caller)
+        return fail(QMC_E_STATE, "qmc_batch_read: the last batched trace wrote into the caller's buffer");
     if (first < 0 || count < 0 || first + count > c->batch_n)
         return fail(QMC_E_RANGE, "qmc_batch_read: instances out of range of the last batched trace");
     qmc_status st = set_device(c);

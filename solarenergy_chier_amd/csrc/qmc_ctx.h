@@ -64,6 +64,9 @@ struct qmc_ctx {
     size_t batch_cap = 0;          // floats
     void *d_batch_inst = nullptr;  // instance table
     size_t batch_inst_cap = 0;
+    char *d_batch_work = nullptr;  // per-instance frames, factors and quads of a chunk
+    size_t batch_work_cap = 0;     // bytes
+    bool batch_maps_caller = false; // the last batched trace wrote into the caller's buffer
     int32_t batch_n = 0;           // instances of the last batched trace
     int32_t batch_npix = 0;
     int32_t batch_grid = 0;

@@ -39,7 +39,16 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const DevHelio &H = a.helios[h];
     const int L = a.L;
     const long long nrays = (long long)H.F * L;
-    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // flat heliostats take their rays in facet tiles (a wave's origins form a compact patch)
+    const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long r = -1;
+    if (H.type == 0) {
+        const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
+        const FacetTiling tl = facet_tiling(R, C, L);
+        if (slot < tl.slots) r = facet_tiled_ray(tl, R, C, (int)slot);
+    } else if (slot < nrays) {
+        r = slot;
+    }
 
     __shared__ unsigned long long s_acc[9];
     if (STATS) {
@@ -48,7 +57,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     }
     RayStats st{0u, 0u, 0u};
     unsigned c_unshadowed = 0, c_unblocked = 0, c_dep = 0, c_edge = 0;
-    const bool active = r < nrays;
+    const bool active = r >= 0;
     if (!STATS && !active) return;
 
     if (active) {
@@ -410,6 +419,7 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_slope);
     dev_free(c->d_aims);
     dev_free(c->d_batch_maps);
+    dev_free(c->d_batch_work);
     if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
     c->d_batch_inst = nullptr;
     qmc_adjoint_release(c);
@@ -861,8 +871,8 @@ extern "C" qmc_status qmc_facets_get(qmc_ctx *c, int32_t h, float *origins, floa
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, int maxF, qmc_ray_record *rec,
-                        StatsAcc *stats) {
+qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
+                        qmc_ray_record *rec, StatsAcc *stats) {
     const GridBuf &gb = c->grids[grid];
     const RecvBuf &rb = c->recvs[gb.receiver];
     TraceArgs a;
@@ -882,8 +892,7 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, int 
     a.P = c->P;
     a.L = c->L;
     a.n_list = n;
-    const long long rays = (long long)maxF * c->L;
-    const long long bx = (rays + 255) / 256;
+    const long long bx = (max_slots + 255) / 256;
     if (bx > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
     if (n > 65535) return fail(QMC_E_UNSUPPORTED, "trace: more than 65535 heliostats in one grid launch");
     a.blocks_per_helio = (int)bx;
@@ -931,9 +940,16 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
     int at = 0;
     for (auto &sp : spans) {
         if (sp.second == 0) continue;
-        int maxF = 0;
-        for (int i = at; i < at + sp.second; ++i) maxF = std::max(maxF, c->h_helios[sorted[i]].F);
-        if ((st = launch_trace(c, sp.first, c->d_list + at, sp.second, maxF, rec, stats))) return st;
+        long long max_slots = 0;  // rays, or tiled ray slots of flat heliostats (see k_trace)
+        for (int i = at; i < at + sp.second; ++i) {
+            const DevHelio &H = c->h_helios[sorted[i]];
+            const long long slots = H.type == 0 ? (long long)facet_tiling(H.rows * H.sub_rows, H.cols * H.sub_cols,
+                                                                          c->L).slots
+                                                : (long long)H.F * c->L;
+            max_slots = std::max(max_slots, slots);
+        }
+        if (max_slots > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
+        if ((st = launch_trace(c, sp.first, c->d_list + at, sp.second, max_slots, rec, stats))) return st;
         at += sp.second;
     }
     return QMC_OK;

@@ -97,6 +97,47 @@ __device__ __forceinline__ V3 flat_facet_origin(const DevHelio &H, const DevFram
     return fr_apply(fr, local) + V3{H.pos[0], H.pos[1], H.pos[2]};
 }
 
+// Wave-tiled ray order of a flat heliostat with R x C micro-facets (row-major, m = row*C + col) and
+// L rays per facet, L in {1, 2, 4, 8}: slot t of a wave of 64 takes ray j of facet (row, col) inside
+// a 2^ltw x 2^lth tile of facets (8x8, 8x4, 4x4 or 4x2; tile area * L = 64), tiles row-major over
+// the padded facet grid. A wave's ray origins then span a compact patch instead of a long facet row,
+// so its lanes walk similar grid cells. Slots past the grid's edge hold no ray. Other L keep facet
+// order (with L >= 16 a wave holds at most four facets of one row already).
+struct FacetTiling {
+    int ltw, lth, lL, ntx, slots;
+    bool tiled;
+};
+
+__host__ __device__ inline FacetTiling facet_tiling(int R, int C, int L) {
+    FacetTiling t;
+    t.lL = L == 1 ? 0 : L == 2 ? 1 : L == 4 ? 2 : L == 8 ? 3 : -1;
+    t.tiled = t.lL >= 0;
+    if (!t.tiled) {
+        t.ltw = t.lth = t.ntx = 0;
+        t.slots = R * C * L;
+        return t;
+    }
+    const int lf = 6 - t.lL;  // log2 of facets per wave
+    t.ltw = (lf + 1) / 2;
+    t.lth = lf - t.ltw;
+    t.ntx = (C + (1 << t.ltw) - 1) >> t.ltw;
+    const int nty = (R + (1 << t.lth) - 1) >> t.lth;
+    t.slots = t.ntx * nty * 64;
+    return t;
+}
+
+// Ray id of slot t, or -1 for a slot outside the facet grid.
+__host__ __device__ inline int facet_tiled_ray(const FacetTiling &tl, int R, int C, int t) {
+    if (!tl.tiled) return t;
+    const int tile = t >> 6, lane = t & 63;
+    const int f = lane >> tl.lL, j = lane & ((1 << tl.lL) - 1);
+    const int ty = tile / tl.ntx, tx = tile - ty * tl.ntx;
+    const int col = (tx << tl.ltw) + (f & ((1 << tl.ltw) - 1));
+    const int row = (ty << tl.lth) + (f >> tl.ltw);
+    if (col >= C || row >= R) return -1;
+    return ((row * C + col) << tl.lL) + j;
+}
+
 // Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
 // onto the facet normal.
 __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,

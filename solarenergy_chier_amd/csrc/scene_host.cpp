@@ -733,30 +733,40 @@ void grid_match(HostGrid &g, const std::vector<HostHeliostat> &hs) {
     g.ny = (int)std::ceil(gsize.y / gint.y);
     g.nz = (int)std::ceil(gsize.z / gint.z);
     const size_t ncell = (size_t)g.nx * g.ny * g.nz;
-    std::vector<std::vector<int>> lists(ncell);
+    // two passes (count, then fill) over the cell ranges of every heliostat's bounding sphere; each
+    // cell lists sub-heliostat ids in heliostat order, as the reference's per-cell appends do
+    struct Range {
+        int x0, x1, y0, y1, z0, z1, first, n;
+    };
+    std::vector<Range> ranges;
+    ranges.reserve(g.d.n_heliostats);
     int start_sub = 0;
     for (int i = g.d.start_heliostat; i < g.d.start_heliostat + g.d.n_heliostats; ++i) {
         V3 pos = f3(hs[i].d.pos);
         float radius = length(f3(hs[i].d.size)) / 2;
         V3 mn = pos - radius, mx = pos + radius;
         V3 a = (mn - gpos) / gint, b = (mx - gpos) / gint;
-        int ax = (int)a.x, ay = (int)a.y, az = (int)a.z;
-        int bx = (int)b.x, by = (int)b.y, bz = (int)b.z;
-        int ss = sub_size(hs[i]);
-        for (int x = std::max(0, ax); x <= std::min(g.nx - 1, bx); ++x)
-            for (int y = std::max(0, ay); y <= std::min(g.ny - 1, by); ++y)
-                for (int z = std::max(0, az); z <= std::min(g.nz - 1, bz); ++z) {
-                    size_t id = (size_t)x * g.ny * g.nz + (size_t)y * g.nz + z;
-                    for (int k = 0; k < ss; ++k) lists[id].push_back(start_sub + k);
-                }
+        const int ss = sub_size(hs[i]);
+        ranges.push_back(Range{std::max(0, (int)a.x), std::min(g.nx - 1, (int)b.x), std::max(0, (int)a.y),
+                               std::min(g.ny - 1, (int)b.y), std::max(0, (int)a.z), std::min(g.nz - 1, (int)b.z),
+                               start_sub, ss});
         start_sub += ss;
     }
     g.cell_start.assign(ncell + 1, 0);
-    g.match.clear();
-    for (size_t c = 0; c < ncell; ++c) {
-        g.cell_start[c + 1] = g.cell_start[c] + (int)lists[c].size();
-        g.match.insert(g.match.end(), lists[c].begin(), lists[c].end());
-    }
+    for (const Range &r : ranges)
+        for (int x = r.x0; x <= r.x1; ++x)
+            for (int y = r.y0; y <= r.y1; ++y)
+                for (int z = r.z0; z <= r.z1; ++z) g.cell_start[(size_t)x * g.ny * g.nz + (size_t)y * g.nz + z + 1] += r.n;
+    for (size_t c = 0; c < ncell; ++c) g.cell_start[c + 1] += g.cell_start[c];
+    g.match.assign((size_t)g.cell_start[ncell], 0);
+    std::vector<int> fill(g.cell_start.begin(), g.cell_start.end() - 1);
+    for (const Range &r : ranges)
+        for (int x = r.x0; x <= r.x1; ++x)
+            for (int y = r.y0; y <= r.y1; ++y)
+                for (int z = r.z0; z <= r.z1; ++z) {
+                    int &f = fill[(size_t)x * g.ny * g.nz + (size_t)y * g.nz + z];
+                    for (int k = 0; k < r.n; ++k) g.match[f++] = r.first + k;
+                }
 }
 
 void heliostat_process(HostHeliostat &h, V3 aim, V3 sun_dir, float pixel_length) {

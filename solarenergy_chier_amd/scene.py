@@ -416,7 +416,7 @@ class Device:
         sched = None
         if schedule is not None:
             sched = C.byref(N.Schedule(schedule["block_threads"], schedule["workgroups_per_cu"],
-                                       schedule["lds_flux_tile"], 0))
+                                       schedule["lds_flux_tile"], schedule.get("facet_tiles", 0)))
         ptr = C.cast(C.c_void_p(device_maps), N._FP) if device_maps else None
         N.call("qmc_trace_batched", self.handle, grid, inst, len(inst), sched, ptr)
 

@@ -52,7 +52,11 @@ def batched_setup(device):
 
 @pytest.mark.parametrize("schedule", [None, {"block_threads": 256, "workgroups_per_cu": 1, "lds_flux_tile": 1},
                                       {"block_threads": 1024, "workgroups_per_cu": 2, "lds_flux_tile": 1},
-                                      {"block_threads": 512, "workgroups_per_cu": 4, "lds_flux_tile": 0}])
+                                      {"block_threads": 512, "workgroups_per_cu": 4, "lds_flux_tile": 0},
+                                      {"block_threads": 256, "workgroups_per_cu": 8, "lds_flux_tile": 0,
+                                       "facet_tiles": 1},
+                                      {"block_threads": 1024, "workgroups_per_cu": 1, "lds_flux_tile": 1,
+                                       "facet_tiles": 1}])
 def test_batched_equals_looped_plain_calls(device, batched_setup, schedule):
     wl, scene, aims, sun, dni, aim = batched_setup
     inst = device.instances(sun, dni, aim)
@@ -132,3 +136,29 @@ def test_batched_errors(device):
     assert e.value.status == N.QMC_E_RANGE
     device.trace_batched(0, inst)
     assert device.batch_read(0, 1, device.flux_shape(0)).sum() > 0
+
+
+@pytest.mark.parametrize("schedule", [None, {"block_threads": 512, "workgroups_per_cu": 1, "lds_flux_tile": 1},
+                                      {"block_threads": 256, "workgroups_per_cu": 2, "lds_flux_tile": 0},
+                                      {"block_threads": 256, "workgroups_per_cu": 8, "lds_flux_tile": 0,
+                                       "facet_tiles": 1}])
+def test_ps10_sampled_batched_equals_looped_plain_calls(device, schedule):
+    """The batched sweep's own field: 624 heliostats at instance sampling, shared cell lists, every
+    instance's quads and frames rebuilt in LDS."""
+    wl = workloads.ps10_sampled_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    device.upload(scene)
+    device.pool_generate(2, 4)
+    device.offsets_generate(3)
+    aims = workloads.aim_strategies(scene.counts.n_heliostats)
+    device.batch_set_aims(scene, aims)
+    sun, dni, aim = workloads.batched_instances(n_suns=2)
+    inst = device.instances(sun, dni, aim)
+    device.trace_batched(0, inst, schedule)
+    maps = device.batch_read(0, len(inst), device.flux_shape(0))
+    for i in range(len(inst)):
+        ref = plain_map(device, scene, wl.config, sun[i], dni[i], aims[aim[i]])
+        err = rel_l1(maps[i], ref)
+        print(f"ps10 instance {i} (aim set {aim[i]}): rel L1 {err:.2e}, sum {ref.sum():.4e}", flush=True)
+        assert ref.sum() > 0
+        assert err < TOL
