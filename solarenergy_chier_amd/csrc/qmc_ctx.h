@@ -13,6 +13,7 @@ namespace qmc {
 
 struct GridBuf {
     DevGrid g{};
+    int4 *cells = nullptr;
     int32_t *cell_start = nullptr;
     int32_t *match = nullptr;
     float4 *quads = nullptr;
@@ -120,6 +121,7 @@ void qmc_adjoint_release(qmc_ctx *c);  // qmc_adjoint.hip
 
 inline void free_scene(qmc_ctx *c) {
     for (auto &g : c->grids) {
+        dev_free(g.cells);
         dev_free(g.cell_start);
         dev_free(g.match);
         dev_free(g.quads);

@@ -2,6 +2,7 @@
 //
 // HBM layout (one qmc_ctx):
 //   grid      cell_start[nx*ny*nz+1], match[n_match]          (CSR of grid-relative sub ids)
+//             cells[nx*ny*nz] int4: count, first three ids      (one 16 B load per visited cell)
 //             quads[3*n_sub] float4: A, E1=B-A, E2=C-A         (48 B per blocking quad)
 //   helios    DevHelio[n_heliostats]                           (wave-uniform, scalar-loaded)
 //   facets    DevFacet[sum rows*cols of focus-flat heliostats]
@@ -66,6 +67,7 @@ struct DevGrid {
     float interval[3];
     int32_t nx, ny, nz;
     int32_t pad_;
+    const int4 *cells;         // per cell: candidate count and the first three candidate ids
     const int32_t *cell_start;
     const int32_t *match;
     const float4 *quads;

@@ -568,6 +568,18 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
             return fail(QMC_E_UNSUPPORTED, "qmc_upload: grid " + std::to_string(gi) +
                                                " has a sub-heliostat count override without matching quads");
         std::vector<float4> quads = pack_quads(hg);
+        // cell records: the candidate count and the first three candidates of each cell in list
+        // order (further candidates stay in the CSR lists)
+        const size_t ncell = hg.cell_start.size() - 1;
+        std::vector<int4> cells(ncell);
+        for (size_t cc = 0; cc < ncell; ++cc) {
+            const int b = hg.cell_start[cc], n = hg.cell_start[cc + 1] - b;
+            cells[cc] = make_int4(n, n > 0 ? hg.match[b] : -1, n > 1 ? hg.match[b + 1] : -1, n > 2 ? hg.match[b + 2] : -1);
+        }
+        if ((st = dev_alloc(gb.cells, cells.size()))) return st;
+        if (!cells.empty())
+            HIP_TRY(hipMemcpyAsync(gb.cells, cells.data(), cells.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                   c->stream));
         if ((st = dev_alloc(gb.cell_start, hg.cell_start.size()))) return st;
         if ((st = dev_alloc(gb.match, hg.match.size()))) return st;
         if ((st = dev_alloc(gb.quads, quads.size()))) return st;
@@ -588,6 +600,7 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         g.nx = hg.nx;
         g.ny = hg.ny;
         g.nz = hg.nz;
+        g.cells = gb.cells;
         g.cell_start = gb.cell_start;
         g.match = gb.match;
         g.quads = gb.quads;

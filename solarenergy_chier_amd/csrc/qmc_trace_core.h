@@ -43,15 +43,19 @@ __device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, cons
     const float dy = abs_divide(gint.y, d.y);
     const float dz = abs_divide(gint.z, d.z);
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    while (ix >= 0 && ix < nx && iy >= 0 && iy < ny && iz >= 0 && iz < nz) {
-        const int addr = ix * ny * nz + iy * nz + iz;
-        const int b = g.cell_start[addr], e = g.cell_start[addr + 1];
+    // cell address kept incrementally (x-major, then y, then z); one unsigned compare per axis
+    // covers both ends of the grid
+    const int stride_x = sx * ny * nz, stride_y = sy * nz;
+    int addr = ix * ny * nz + iy * nz + iz;
+    while (((unsigned)ix < (unsigned)nx) & ((unsigned)iy < (unsigned)ny) & ((unsigned)iz < (unsigned)nz)) {
+        const int4 rec = g.cells[addr];  // count, then the cell's first three candidates
+        const int n = rec.x;
         if (STATS) {
             st.cells += 1;
-            st.candidates += (unsigned)(e - b);
+            st.candidates += (unsigned)n;
         }
-        for (int i = b; i < e; ++i) {
-            const int id = g.match[i];
+        for (int i = 0; i < n; ++i) {
+            const int id = i == 0 ? rec.y : i == 1 ? rec.z : i == 2 ? rec.w : g.match[g.cell_start[addr] + i];
             if (id < own_lo || id >= own_hi) {
                 if (STATS) st.tests += 1;
                 const float4 qa = quads[3 * id];
@@ -64,17 +68,25 @@ __device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, cons
                 }
             }
         }
-        bool fx = false, fy = false, fz = false;
-        if (dda_less(tx, ty)) {
-            if (dda_less(tx, tz) || dda_equal(tx, tz)) { ix += sx; fx = true; }
-            if (dda_less(tz, tx) || dda_equal(tx, tz)) { iz += sz; fz = true; }
-        } else if (dda_less(ty, tx)) {
-            if (dda_less(ty, tz) || dda_equal(ty, tz)) { iy += sy; fy = true; }
-            if (dda_less(tz, ty) || dda_equal(ty, tz)) { iz += sz; fz = true; }
-        } else {
-            if (dda_less(tx, tz) || dda_equal(tx, tz)) { ix += sx; iy += sy; fx = fy = true; }
-            if (dda_less(tz, tx) || dda_equal(tx, tz)) { iz += sz; fz = true; }
-        }
+        // Step rule of rectGridDDA (tie tolerance kEps, compared in double): if tx is clearly below
+        // ty, step x and/or z by comparing tx with tz; else if ty is clearly below tx, step y and/or
+        // z by comparing ty with tz; else step x and y together and/or z by comparing tx with tz.
+        // Branch-free form of dda_less/dda_equal on the same double values: each bound t -/+ kEps
+        // is formed once and every comparison is made once.
+        const double dtx = tx, dty = ty, dtz = tz;
+        const double xm = dtx - kEps, ym = dty - kEps, zm = dtz - kEps, zp = dtz + kEps;
+        const bool ltxy = dtx < ym;               // dda_less(tx, ty)
+        const bool use_y = !ltxy && dty < xm;     // dda_less(ty, tx) in the else branch
+        const double ta = use_y ? dty : dtx, tam = use_y ? ym : xm;
+        const bool eq_az = ta < zp && ta > zm;    // dda_equal(ta, tz)
+        const bool step_a = ta < zm || eq_az;     // dda_less(ta, tz) || dda_equal(ta, tz)
+        const bool fz = dtz < tam || eq_az;       // dda_less(tz, ta) || dda_equal(ta, tz)
+        const bool fx = step_a && !use_y;
+        const bool fy = step_a && !ltxy;
+        ix += fx ? sx : 0;
+        iy += fy ? sy : 0;
+        iz += fz ? sz : 0;
+        addr += (fx ? stride_x : 0) + (fy ? stride_y : 0) + (fz ? sz : 0);
         tx += fx ? dx : 0.0f;
         ty += fy ? dy : 0.0f;
         tz += fz ? dz : 0.0f;
