@@ -72,6 +72,7 @@ struct AdjArgs {
     const float4 *pert;
     const int32_t *offsets;
     int32_t P, L;
+    FastDiv fdP, fdL;
     float *flux;          // bilinear map (forward)
     double *flux64;       // bilinear map (masked forward)
     uint32_t *mask;       // visibility bits
@@ -92,10 +93,11 @@ __device__ __forceinline__ void ray_model(const AdjArgs &a, const DevHelio &H, c
     const D3 us = ldd(a.sun.us), mm = ldd(a.sun.m), vs = ldd(a.sun.vs);
     // local facet coordinates (RectangleHelio.cu:64-79)
     const int cpr = H.cols * H.sub_cols;
-    const int row = m / cpr, col = m % cpr;
+    const int row = (int)fast_div((uint32_t)m, H.fd_cpr), col = m - row * cpr;
+    const int bcol = (int)fast_div((uint32_t)col, H.fd_sub_cols), brow = (int)fast_div((uint32_t)row, H.fd_sub_rows);
     const double pl = H.pl;
-    r.lf = D3{col * pl + (col / H.sub_cols) * (double)H.gap[0] + pl / 2.0 - H.size[0] / 2.0, H.size[1] / 2.0,
-              row * pl + (row / H.sub_rows) * (double)H.gap[1] + pl / 2.0 - H.size[2] / 2.0};
+    r.lf = D3{col * pl + bcol * (double)H.gap[0] + pl / 2.0 - H.size[0] / 2.0, H.size[1] / 2.0,
+              row * pl + brow * (double)H.gap[1] + pl / 2.0 - H.size[2] / 2.0};
     const D3 O = ldd(Ah.p) + u * r.lf.x + n * r.lf.y + v * r.lf.z;
     const D3 din = us * (double)qi.x + mm * (double)qi.y + vs * (double)qi.z;
     const D3 Nraw = u * (double)qp.x + n * (double)qp.y + v * (double)qp.z;
@@ -160,16 +162,16 @@ __global__ __launch_bounds__(256) void k_adj_forward(AdjArgs a) {
     const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     bool visible = false;
     if (r < nrays) {
-        const int m = (int)(r / L), j = (int)(r % L);
+        const int m = (int)fast_div((uint32_t)r, a.fdL), j = (int)r - m * L;
         const int32_t *grp = a.offsets + H.offsets_base;
-        const float4 s = a.pool[(grp[m] + j) % P];
+        const float4 s = a.pool[fast_mod((uint32_t)(grp[m] + j), P, a.fdP)];
         V3 dir = fr_apply(a.sunf, V3{s.x, s.y, s.z});
         const V3 origin = flat_facet_origin(H, H.fr, m);
         RayStats st{0u, 0u, 0u};
         int hit;
         const int lo = H.sub_begin, hi = H.sub_begin + H.sub_count;
         if (!dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st)) {
-            const float4 q = a.pert[(grp[(m + 1) % P] + j) % P];
+            const float4 q = a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)];
             const V3 normal = normalize(fr_apply(H.fr, V3{q.x, q.y, q.z}));
             dir = normalize(reflect(-dir, normal));
             if (!dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st)) {
@@ -202,10 +204,11 @@ __global__ __launch_bounds__(256) void k_adj_forward_masked(AdjArgs a) {
     if (r >= nrays) return;
     const long long bit = a.ah[h].ray_base + r;
     if (!((a.mask[bit >> 5] >> (bit & 31)) & 1u)) return;
-    const int m = (int)(r / L), j = (int)(r % L);
+    const int m = (int)fast_div((uint32_t)r, a.fdL), j = (int)r - m * L;
     const int32_t *grp = a.offsets + H.offsets_base;
     RayModel rm;
-    ray_model(a, H, a.ah[h], m, a.pool[(grp[m] + j) % P], a.pert[(grp[(m + 1) % P] + j) % P], rm);
+    ray_model(a, H, a.ah[h], m, a.pool[fast_mod((uint32_t)(grp[m] + j), P, a.fdP)],
+              a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)], rm);
     bilinear_splat<true>(a, rm.uu, rm.vv, rm.E);
 }
 
@@ -228,10 +231,10 @@ __global__ __launch_bounds__(256) void k_adj_backward(AdjArgs a) {
     }
     if (vis) {
         const AdjHelio &Ah = a.ah[h];
-        const int m = (int)(r / L), j = (int)(r % L);
+        const int m = (int)fast_div((uint32_t)r, a.fdL), j = (int)r - m * L;
         const int32_t *grp = a.offsets + H.offsets_base;
-        const float4 qi4 = a.pool[(grp[m] + j) % P];
-        const float4 qp4 = a.pert[(grp[(m + 1) % P] + j) % P];
+        const float4 qi4 = a.pool[fast_mod((uint32_t)(grp[m] + j), P, a.fdP)];
+        const float4 qp4 = a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)];
         RayModel f;
         ray_model(a, H, Ah, m, qi4, qp4, f);
         // d(loss)/d(E), d/d(x), d/d(y) through the four bilinear weights
@@ -478,6 +481,8 @@ AdjArgs adj_args(qmc_ctx *c, AdjState *s, float *flux) {
     a.offsets = c->d_offsets;
     a.P = c->P;
     a.L = c->L;
+    a.fdP = make_fastdiv((uint32_t)c->P);
+    a.fdL = make_fastdiv((uint32_t)c->L);
     a.flux = flux;
     a.flux64 = s->d_flux64;
     a.mask = s->d_mask;

@@ -51,6 +51,7 @@ struct BatchArgs {
     const float4 *__restrict__ pert;
     const int32_t *__restrict__ offsets;
     int32_t P, L;
+    FastDiv fdP, fdL;
     float rho;
     float *maps;
     int32_t npix;
@@ -150,16 +151,16 @@ __global__ __launch_bounds__(BS) void k_trace_batched(BatchArgs a) {
             const int nslots = TILES ? tl.slots : H.F * L;
 #pragma nounroll
             for (int slot = tid; slot < nslots; slot += bs) {
-                const int r = TILES ? facet_tiled_ray(tl, R, C, slot) : slot;
+                const int r = TILES ? facet_tiled_ray(tl, tile_divisor(H, tl), R, C, slot) : slot;
                 if (r < 0) continue;
-                const int m = r / L, j = r % L;
-                const int a1 = (grp[m] + j) % P;
+                const int m = (int)fast_div((uint32_t)r, a.fdL), j = r - m * L;
+                const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
                 const float4 s = a.pool[a1];
                 V3 dir = fr_apply(sunf, V3{s.x, s.y, s.z});
                 const V3 origin = flat_facet_origin(H, fr, m);
                 int hit;
                 if (dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
-                const int a2 = (grp[(m + 1) % P] + j) % P;
+                const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
                 const float4 q = a.pert[a2];
                 const V3 normal = normalize(fr_apply(fr, V3{q.x, q.y, q.z}));
                 dir = normalize(reflect(-dir, normal));
@@ -389,6 +390,8 @@ qmc_status plan_batch(qmc_ctx *c, int grid, const qmc_instance *inst, int n, Bat
     a.offsets = c->d_offsets;
     a.P = c->P;
     a.L = c->L;
+    a.fdP = make_fastdiv((uint32_t)c->P);
+    a.fdL = make_fastdiv((uint32_t)c->L);
     a.rho = c->cfg.helio_reflected_rate;
     a.npix = rb.w * rb.h;
     plan.rt = rb.r.type;

@@ -42,6 +42,9 @@ struct DevHelio {
     float fac_x;
     int32_t pad_;
     int64_t offsets_base;
+    // divisions by per-heliostat constants: micro-facets per row (cols*sub_cols), per-facet
+    // rows/cols, and the facet-tile counts across a row for 4- and 8-wide tiles
+    FastDiv fd_cpr, fd_sub_rows, fd_sub_cols, fd_ntx4, fd_ntx8;
 };
 
 struct DevFacet {
@@ -99,6 +102,7 @@ struct TraceArgs {
     qmc_ray_record *records;   // nullable (one heliostat per launch when set)
     StatsAcc *stats;           // nullable
     int32_t P, L;
+    FastDiv fdP, fdL;
     int32_t n_list;
     int32_t blocks_per_helio;  // grid.x extent per heliostat
 };

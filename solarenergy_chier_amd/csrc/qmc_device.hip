@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     if (H.type == 0) {
         const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
         const FacetTiling tl = facet_tiling(R, C, L);
-        if (slot < tl.slots) r = facet_tiled_ray(tl, R, C, (int)slot);
+        if (slot < tl.slots) r = facet_tiled_ray(tl, tile_divisor(H, tl), R, C, (int)slot);
     } else if (slot < nrays) {
         r = slot;
     }
@@ -61,14 +61,14 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     if (!STATS && !active) return;
 
     if (active) {
-        const int m = (int)(r / L);
-        const int j = (int)(r % L);
+        const int m = (int)fast_div((uint32_t)r, a.fdL);
+        const int j = (int)r - m * L;
         const int32_t *grp = a.offsets + H.offsets_base;
         const int P = a.P;
         const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
 
         // Step 1: incident direction and shadowing (RectangleReceiverRectangleGridRayTracing.cu:14-22)
-        const int a1 = (grp[m] + j) % P;
+        const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
         const float4 s = a.pool[a1];
         V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
         V3 origin, fnormal;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         } else {
             c_unshadowed = 1;
             // Step 2: perturbed facet normal, reflection, blocking (:24-36)
-            const int a2 = (grp[(m + 1) % P] + j) % P;
+            const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
             const float4 q = a.pert[a2];
             V3 normal = normalize(fr_apply(*pf, V3{q.x, q.y, q.z}));
             dir = normalize(reflect(-dir, normal));
@@ -480,6 +480,12 @@ int64_t pack_heliostats(const qmc_scene &sc, std::vector<DevHelio> &out, std::ve
         d.receiver = s->grids[hh.d.grid].d.receiver;
         d.offsets_base = off_base;
         off_base += (int64_t)hh.n_micro + 1;
+        const int cpr = d.cols * d.sub_cols;
+        d.fd_cpr = make_fastdiv((uint32_t)std::max(1, cpr));
+        d.fd_sub_rows = make_fastdiv((uint32_t)std::max(1, d.sub_rows));
+        d.fd_sub_cols = make_fastdiv((uint32_t)std::max(1, d.sub_cols));
+        d.fd_ntx4 = make_fastdiv((uint32_t)std::max(1, (cpr + 3) / 4));
+        d.fd_ntx8 = make_fastdiv((uint32_t)std::max(1, (cpr + 7) / 8));
         d.facet_base = (int)facets.size();
         if (hh.type == 1) {
             Frame fh = make_frame(hh.normal, true);
@@ -701,6 +707,8 @@ extern "C" qmc_status qmc_update(qmc_ctx *c, const qmc_scene *s) {
 // ---------------------------------------------------------------------------------------------
 static qmc_status ensure_pool(qmc_ctx *c, int P, int L) {
     if (P <= 0 || L <= 0) return fail(QMC_E_INVALID_ARG, "pool size and lights per group must be positive");
+    if (P > (1 << 30) || L > P) return fail(QMC_E_RANGE, "pool size above 2^30 or fewer entries than lights per group");
+    if (c->P != P) c->offsets_ready = false;  // group offsets index the pool: [0, P]
     if (c->P != P || !c->d_pool) {
         qmc_status st;
         if ((st = dev_alloc(c->d_pool, (size_t)P))) return st;
@@ -836,6 +844,11 @@ extern "C" qmc_status qmc_offsets_set(qmc_ctx *c, int32_t h, const int32_t *off,
     if (h < 0 || h >= (int)c->h_helios.size()) return fail(QMC_E_RANGE, "qmc_offsets_set: heliostat out of range");
     const DevHelio &H = c->h_helios[h];
     if (n != H.F + 1) return fail(QMC_E_INVALID_ARG, "qmc_offsets_set: need F+1 = " + std::to_string(H.F + 1) + " offsets");
+    if (!c->pool_ready) return fail(QMC_E_STATE, "qmc_offsets_set: set or generate the pool first (offsets are in [0, P])");
+    for (int32_t i = 0; i < n; ++i)
+        if (off[i] < 0 || off[i] > c->P)
+            return fail(QMC_E_RANGE, "qmc_offsets_set: offset " + std::to_string(i) + " = " + std::to_string(off[i]) +
+                                         " outside [0, " + std::to_string(c->P) + "]");
     qmc_status st = set_device(c);
     if (st) return st;
     HIP_TRY(hipMemcpyAsync(c->d_offsets + H.offsets_base, off, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
@@ -904,6 +917,8 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     a.stats = stats;
     a.P = c->P;
     a.L = c->L;
+    a.fdP = make_fastdiv((uint32_t)c->P);
+    a.fdL = make_fastdiv((uint32_t)c->L);
     a.n_list = n;
     const long long bx = (max_slots + 255) / 256;
     if (bx > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");

@@ -155,6 +155,23 @@ QMC_HD bool ray_quad(V3 o, V3 d, V3 A, V3 E1, V3 E2, float &t, float &u, float &
 }
 
 // DDA helpers (QuasiMonteCarloRayTracing/Grid/rectGridDDA.cuh:15-32).
+// Exact quotient of 0 <= n < 2^31 by a divisor d >= 1 fixed ahead of the kernel, as a multiply-high:
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, q = (mulhi(n, m) + n) >> l (the round-up
+// multiplier of Granlund and Montgomery, "Division by invariant integers using multiplication",
+// PLDI 1994); mulhi(n, m) < n keeps the sum below 2^32.
+struct FastDiv {
+    uint32_t m, l;
+};
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    return FastDiv{(uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1), l};
+}
+QMC_HD uint32_t fast_div(uint32_t n, FastDiv f) {
+    return (uint32_t)((((uint64_t)n * f.m) >> 32) + n) >> f.l;
+}
+QMC_HD uint32_t fast_mod(uint32_t n, uint32_t d, FastDiv f) { return n - fast_div(n, f) * d; }
+
 QMC_HD float abs_divide(float den, float num) {
     if (num <= kEps && num >= -kEps) return kIntMaxF;
     return fabsf(den / num);

@@ -99,10 +99,10 @@ __device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, cons
 // rotated into the frame and translated to the heliostat position.
 __device__ __forceinline__ V3 flat_facet_origin(const DevHelio &H, const DevFrame &fr, int m) {
     const int cpr = H.cols * H.sub_cols;
-    const int row = m / cpr;
-    const int col = m % cpr;
-    const int brow = row / H.sub_rows;
-    const int bcol = col / H.sub_cols;
+    const int row = (int)fast_div((uint32_t)m, H.fd_cpr);
+    const int col = m - row * cpr;
+    const int brow = (int)fast_div((uint32_t)row, H.fd_sub_rows);
+    const int bcol = (int)fast_div((uint32_t)col, H.fd_sub_cols);
     const float pl = H.pl;
     V3 local{col * pl + bcol * H.gap[0] + pl / 2 - H.size[0] / 2, H.size[1] / 2,
              row * pl + brow * H.gap[1] + pl / 2 - H.size[2] / 2};
@@ -138,16 +138,21 @@ __host__ __device__ inline FacetTiling facet_tiling(int R, int C, int L) {
     return t;
 }
 
-// Ray id of slot t, or -1 for a slot outside the facet grid.
-__host__ __device__ inline int facet_tiled_ray(const FacetTiling &tl, int R, int C, int t) {
+// Ray id of slot t, or -1 for a slot outside the facet grid; fd_ntx divides by tl.ntx.
+__host__ __device__ inline int facet_tiled_ray(const FacetTiling &tl, FastDiv fd_ntx, int R, int C, int t) {
     if (!tl.tiled) return t;
     const int tile = t >> 6, lane = t & 63;
     const int f = lane >> tl.lL, j = lane & ((1 << tl.lL) - 1);
-    const int ty = tile / tl.ntx, tx = tile - ty * tl.ntx;
+    const int ty = (int)fast_div((uint32_t)tile, fd_ntx), tx = tile - ty * tl.ntx;
     const int col = (tx << tl.ltw) + (f & ((1 << tl.ltw) - 1));
     const int row = (ty << tl.lth) + (f >> tl.ltw);
     if (col >= C || row >= R) return -1;
     return ((row * C + col) << tl.lL) + j;
+}
+
+// The multiply-high divisor of a heliostat for tiles of 2^ltw facets across.
+__device__ __forceinline__ FastDiv tile_divisor(const DevHelio &H, const FacetTiling &tl) {
+    return tl.ltw == 3 ? H.fd_ntx8 : H.fd_ntx4;
 }
 
 // Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples

@@ -272,3 +272,25 @@ def test_ps10_full_size_properties(device):
     assert rel_l1(device.flux(0) / 2.0, full) < FLUX_RTOL_L1
     st2 = device.trace_stats(range(n))
     assert st2 == st
+
+
+def test_offsets_set_validates_range(device):
+    """Group offsets index the sample pool, so values outside [0, P] are refused before any launch."""
+    wl = workloads.small_field(L=8, groups=16)
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    device.upload(scene)
+    F = scene.heliostat_info(0).n_micro
+    with pytest.raises(N.QmcError) as e:
+        device.offsets_set(0, np.zeros(F + 1, np.int32))
+    assert e.value.status == N.QMC_E_STATE
+    device.pool_generate(2, 4)
+    P = 16 * 8
+    good = np.full(F + 1, P, np.int32)
+    device.offsets_set(0, good)
+    assert np.array_equal(device.offsets_get(0, F), good)
+    for bad_value in (-1, P + 1):
+        bad = good.copy()
+        bad[F // 2] = bad_value
+        with pytest.raises(N.QmcError) as e:
+            device.offsets_set(0, bad)
+        assert e.value.status == N.QMC_E_RANGE
