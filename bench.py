@@ -114,13 +114,15 @@ def main() -> int:
     log(f"flux map sum {float(flux.sum(dtype=np.float64)):.6e}, max {float(flux.max()):.3f}")
 
     achieved = alg_bytes / (kernel_avg_ms / 1e3) / 1e9
-    traffic = None
+    # HBM bytes per launch and VALU busy fraction of this kernel, from the committed rocprofv3 PMC
+    # summaries of the same workload (profiles/trace_<workload>_{hbm_traffic,valu}.md)
+    traffic = valu_busy = None
     tfile = ROOT / "profiles" / f"trace_{args.workload}_hbm_traffic.json"
+    vfile = ROOT / "profiles" / f"trace_{args.workload}_valu.json"
     if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+    if vfile.exists():
+        valu_busy = json.loads(vfile.read_text()).get("valu_busy")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -165,6 +167,9 @@ def main() -> int:
             "kernel": "k_trace (rectangle receiver)",
             "kernel_avg_ms": kernel_avg_ms,
             "algorithmic_bytes_per_launch": alg_bytes,
+            "hbm_gbs_measured": traffic / (kernel_avg_ms / 1e3) / 1e9 if traffic else None,
+            "binding_resource": "VALU issue",
+            "valu_busy": valu_busy,
         },
         "cpu_baseline": cpu,
         "batched": batched,
