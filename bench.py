@@ -45,6 +45,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched sun x aim measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
+    ap.add_argument("--no-cylinder", action="store_true", help="skip the config-3 (cylinder) field measurement")
     ap.add_argument("--batched-suns", type=int, default=4000)
     ap.add_argument("--loop-instances", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -136,6 +137,9 @@ def main() -> int:
     adjoint = None
     if not args.no_adjoint:
         adjoint = bench_adjoint(dev)
+    cylinder = None
+    if not args.no_cylinder:
+        cylinder = bench_cylinder(dev)
 
     line = {
         "metric": "rays/s on 1 GPU (PS10-like); batched scenes/s; adjoint:forward cost",
@@ -177,6 +181,7 @@ def main() -> int:
         "batched": batched,
         "batched_ps10_sampled": batched_ps10,
         "adjoint": adjoint,
+        "config3_cylinder_field": cylinder,
     }
     print(json.dumps(line), flush=True)
     dev.close()
@@ -290,6 +295,44 @@ def bench_batched(dev, field: str, n_suns: int, n_loop: int) -> dict:
             "speedup_vs_plain_loop": batched_sps / loop_sps,
             "plain_kernel_only_scenes_per_s": kernel_sps, "speedup_vs_plain_kernels": batched_sps / kernel_sps,
             "schedule": sched, "autotune_s": tune_s}
+
+
+def bench_cylinder(dev, steps: int = 3) -> dict:
+    """Config-3 field: 2650 focus-flat heliostats x 9016 micro-facets x 8 rays = 191M rays per launch
+    onto a 512x256 cylinder map, then the pipeline's trimmed-mean smoothing (radius 5, ratio 0.05)."""
+    from solarenergy_chier_amd import Scene, workloads
+
+    wl = workloads.cylinder_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    dev.upload(scene)
+    dev.pool_generate(2, 4)
+    dev.offsets_generate(3)
+    ids = list(range(scene.counts.n_heliostats))
+    rays = sum(scene.heliostat_info(h).n_micro for h in ids) * scene.config.num_sunshape_lights_per_group
+    st = dev.trace_stats(ids)
+    dev.flux_clear(0)
+    dev.trace(ids)
+    dev.sync()
+    trace_ms = 0.0
+    for _ in range(steps):
+        dev.flux_clear(0)
+        dev.event_record(12)
+        dev.trace(ids)
+        dev.event_record(13)
+        dev.sync()
+        trace_ms += dev.event_elapsed_ms(12, 13)
+    trace_ms /= steps
+    dev.event_record(12)
+    dev.smooth(0, wl.smooth_radius, wl.smooth_ratio)
+    dev.event_record(13)
+    dev.sync()
+    smooth_ms = dev.event_elapsed_ms(12, 13)
+    alg = algorithmic_bytes(st)
+    log(f"config-3 field: {rays} rays in {trace_ms:.3f} ms = {rays / trace_ms / 1e6:.2f} G rays/s; "
+        f"smoothing 512x256 r=5 {smooth_ms:.3f} ms; counted {st}")
+    return {"workload": wl.name, "rays": rays, "trace_ms": trace_ms, "rays_per_s": rays / (trace_ms / 1e3),
+            "smooth_ms": smooth_ms, "algorithmic_bytes_per_launch": alg,
+            "algorithmic_gbs": alg / (trace_ms / 1e3) / 1e9, "counted_work": st}
 
 
 def bench_adjoint(dev, steps: int = 64) -> dict:

@@ -279,6 +279,7 @@ class Device:
         h = C.c_void_p()
         N.call("qmc_create", device, C.byref(h))
         self.handle = h
+        self.device_id = device
         self.scene: Scene | None = None
 
     def close(self):

@@ -59,12 +59,11 @@ def small_field(L: int = 64, groups: int = 1024) -> Workload:
 
 
 def ps10_layout(n: int = 624, r_min: float = 100.0, r_max: float = 750.0, span_deg: float = 40.0,
-                seed: int = 1) -> np.ndarray:
+                seed: int = 1, n_rows: int = 33) -> np.ndarray:
     """Radial-staggered north field: rows every dR metres from r_min, azimuthal pitch chosen so that
     n heliostats fill the rows up to about r_max; odd rows are shifted by half a pitch; positions get
     a seeded +-0.5 m jitter. Returns (n, 2) float64 (x, z)."""
     rng = np.random.default_rng(seed)
-    n_rows = 33
     d_r = (r_max - r_min - 10.0) / (n_rows - 1)
     radii = r_min + d_r * np.arange(n_rows)
     span = math.radians(span_deg)
@@ -107,6 +106,29 @@ def ps10_field(L: int = 16, groups: int = 4096, n: int = 624, seed: int = 1) -> 
     grid = GridSpec(pos=(lo_x, 0.0, lo_z), size=(hi_x - lo_x, 15.0, hi_z - lo_z), interval=(cell, cell, cell),
                     start_heliostat=0, n_heliostats=len(hs), heliostat_type=0, receiver=0)
     return Workload("PS10-like 624 heliostats, 483M rays, 256x256", cfg, [recv], [grid], hs, 1, 0.15)
+
+
+def cylinder_field(L: int = 8, groups: int = 8192, n: int = 2650, seed: int = 2) -> Workload:
+    """Config-3 field: 2650 focus-flat heliostats (10.95 x 9.75 m, 4 x 7 canted facets with 0.1 m gaps,
+    focal length = slant range) in a seeded radial-staggered surround field from 90 m out to 1.2 km,
+    cylinder receiver (radius 4 m, height 10.5 m) centred 140 m up, 512 x 256 map (circumference x
+    height), sun at 35 deg elevation, CSR 0.1, 0.1 m micro-facets x 8 rays."""
+    cfg = make_config(sun_dir=sun_direction(180.0, 35.0), dni=1000.0, csr=0.1, num_sunshape_groups=groups,
+                      num_sunshape_lights_per_group=L, inverse_transform_sampling_groups=64,
+                      receiver_pixel_length=2 * math.pi * 4.0 / 512.0, helio_disturb_std=0.0015,
+                      helio_pixel_length=0.1, helio_reflected_rate=0.88)
+    recv = ReceiverSpec(type=1, pos=(0.0, 140.0, 0.0), size=(4.0, 10.5, 4.0), normal=(0.0, 0.0, 1.0), face=0,
+                        res_x=512, res_y=256)
+    xz = ps10_layout(n=n, r_min=90.0, r_max=1200.0, span_deg=180.0, seed=seed, n_rows=45)
+    hs = [HeliostatSpec(pos=(float(np.float32(x)), 5.5, float(np.float32(z))), size=(10.95, 0.1, 9.75), grid=0,
+                        rows=4, cols=7, gap=(0.1, 0.1), surface_property=(-1.0, -1.0, -1.0, -1.0, -1.0, -1.0))
+          for x, z in xz]
+    cell = 10.0
+    lo = -1220.0
+    grid = GridSpec(pos=(lo, 0.0, lo), size=(-2 * lo, 15.0, -2 * lo), interval=(cell, 7.5, cell),
+                    start_heliostat=0, n_heliostats=len(hs), heliostat_type=1, receiver=0)
+    return Workload("config-3 field: 2650 focus-flat heliostats, cylinder receiver, 512x256", cfg, [recv], [grid],
+                    hs, 5, 0.05)
 
 
 def _fmt(x: float) -> str:

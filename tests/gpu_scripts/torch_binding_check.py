@@ -1,0 +1,52 @@
+"""Child process of tests/test_adjoint.py::test_torch_autograd_binding_matches_adjoint: torch (and its
+GPU) first, then the library, as the binding requires. Compares the torch.autograd binding's
+gradients, for the library's own loss written in torch, with qmc_adjoint_backward's. Prints one
+result line and exits non-zero on a mismatch."""
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+from solarenergy_chier_amd import Device, Scene, workloads  # noqa: E402
+from solarenergy_chier_amd.autograd import FluxMap  # noqa: E402
+
+T, PK, LAM = workloads.ADJOINT_TARGET, workloads.ADJOINT_PEAK, workloads.ADJOINT_LAMBDA
+
+
+def main() -> int:
+    print("torch binding check: small differentiable field", flush=True)
+    wl, offs0 = workloads.adjoint_small_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    dev = Device(0)
+    dev.upload(scene)
+    dev.pool_generate(2, 4)
+    dev.offsets_generate(3)
+    buf = dev.flux_device_ptr(0)
+    dev.adjoint_forward(0, buf)
+    ref_flux = dev.flux(0)
+    dev.flux_loss(buf, ref_flux.size, T, PK, LAM)
+    ga_ref, gs_ref = dev.adjoint_backward(scene, 0)
+
+    fm = FluxMap(dev, scene, 0)
+    offsets = torch.tensor(offs0, dtype=torch.float32, requires_grad=True)
+    sun = torch.tensor(list(scene.config.sun_dir), dtype=torch.float32, requires_grad=True)
+    flux = fm(offsets, sun)
+    e_flux = float(np.abs(flux.detach().cpu().numpy() - ref_flux).sum() / np.abs(ref_flux).sum())
+    loss = ((flux - T) ** 2).sum() + LAM * (torch.clamp(flux - PK, min=0.0) ** 2).sum()
+    loss.backward()
+    ga, gs = offsets.grad.numpy(), sun.grad.numpy()
+    e_aim = float(np.linalg.norm(ga - ga_ref) / np.linalg.norm(ga_ref))
+    e_sun = float(np.linalg.norm(gs - gs_ref) / np.linalg.norm(gs_ref))
+    ok = flux.shape == ref_flux.shape and e_flux < 1e-6 and e_aim < 1e-5 and e_sun < 1e-5
+    print(f"torch binding vs adjoint: flux {e_flux:.2e}, aim {e_aim:.2e}, sun {e_sun:.2e} -> {'ok' if ok else 'MISMATCH'}",
+          flush=True)
+    dev.close()
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -196,3 +196,19 @@ class TestGpuAdjoint:
             o = np.clip(o - lr * ga, -2.0, 2.0).astype(np.float32)
         print("loss first/last", losses[0], losses[-1])
         assert losses[-1] < losses[0]
+
+
+@pytest.mark.gpu
+def test_torch_autograd_binding_matches_adjoint():
+    """The torch.autograd binding composes a loss written in torch with the hand-written adjoint: for
+    the library's own loss its gradients equal qmc_adjoint_backward's. Runs in a child process, which
+    initialises torch's GPU before loading the library (this session already holds the library)."""
+    import subprocess
+    import sys
+    from pathlib import Path
+
+    script = Path(__file__).resolve().parent / "gpu_scripts" / "torch_binding_check.py"
+    out = subprocess.run([sys.executable, "-u", str(script)], capture_output=True, text=True, timeout=240)
+    print(out.stdout[-2000:], out.stderr[-2000:], flush=True)
+    assert out.returncode == 0
+    assert "-> ok" in out.stdout

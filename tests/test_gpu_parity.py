@@ -179,6 +179,13 @@ def test_ps10_field_sampled_heliostats(device):
     assert parity_on_scene(device, scene, [0, 17, 300, 623]) == 0
 
 
+def test_cylinder_field_sampled_heliostats(device):
+    """Config-3 field: canted focus-flat facets, 28 blocking quads per heliostat, cylinder receiver."""
+    wl = workloads.cylinder_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    assert parity_on_scene(device, scene, [0, 1000, 2649]) == 0
+
+
 def test_facets_kernel_bit_exact(device):
     import kat_scenes as K
     for specs in (K.rectangle_helio_scene(), K.focus_helio_scene()):

@@ -55,6 +55,19 @@ def test_ps10_field():
     assert_same_scene(s, O.OracleScene.from_workload(wl))
 
 
+def test_cylinder_field():
+    """Config-3 field: focus-flat heliostats with slant-range focal lengths, cylinder receiver with a
+    512 x 256 resolution override."""
+    wl = workloads.cylinder_field()
+    s = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    ri = s.receiver_info(0)
+    assert (ri.type, ri.res_x, ri.res_y) == (1, 512, 256)
+    assert s.counts.n_heliostats == 2650
+    hi = s.heliostat_info(0)
+    assert (hi.type, hi.sub_count, hi.n_micro) == (1, 28, 4 * 23 * 7 * 14)
+    assert_same_scene(s, O.OracleScene.from_workload(wl))
+
+
 @pytest.mark.parametrize("scn,cfg", [
     ("qmcrt_test_scene.scn", "qmcrt_test_configuration.json"),
     ("qmcrt_test_scene_cylinder_receiver.scn", "qmcrt_test_configuration2.json"),
