@@ -384,6 +384,21 @@ qmc_status qmc_flux_loss(qmc_ctx *ctx, const float *d_flux, int32_t n, float tar
 qmc_status qmc_adjoint_backward(qmc_ctx *ctx, const qmc_scene *scene, int32_t grid, const float *d_dL_dflux,
                                 float *grad_aim, float *grad_sun);
 
+/* ------------------------------------------------------------------------------------------
+ * Output (RayTracingPipeline, RayTracingPipeline.cpp:145-175)
+ * ------------------------------------------------------------------------------------------ */
+typedef struct qmc_image_stats {
+    float max_value;   /* running float maximum of the written values >= 1e-5 */
+    float sum;         /* float sum of those values in file order */
+    int32_t max_row;   /* file row / column of the last occurrence of max_value (-1 if none) */
+    int32_t max_col;
+} qmc_image_stats;
+/* ImageSaver::saveText (Tool/ImageSaver/ImageSaver.cpp:13-57, defaults precision 2): writes an
+ * height x width row-major map as text, rows bottom-up, values < 1e-5 as "0", others with `precision`
+ * decimals, comma-separated. Host only; stats may be NULL. */
+qmc_status qmc_image_save_text(const char *path, int32_t height, int32_t width, const float *data,
+                               int32_t precision, qmc_image_stats *stats);
+
 /* HIP-event timing on the ctx stream (slots 0..15). */
 qmc_status qmc_event_record(qmc_ctx *ctx, int32_t slot);
 qmc_status qmc_event_elapsed_ms(qmc_ctx *ctx, int32_t slot_begin, int32_t slot_end, float *ms);

@@ -103,6 +103,10 @@ class Schedule(C.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
+class ImageStats(C.Structure):
+    _fields_ = [("max_value", C.c_float), ("sum", C.c_float), ("max_row", C.c_int32), ("max_col", C.c_int32)]
+
+
 RECORD_DTYPE = [("status", "<i4"), ("index", "<i4"), ("energy", "<f4"), ("flags", "<i4")]
 
 _P = C.c_void_p
@@ -172,6 +176,7 @@ SIGNATURES = {
     "qmc_adjoint_forward_masked": [_P, _I, C.POINTER(C.c_double), C.c_int64],
     "qmc_flux_loss": [_P, _FP, _I, _F, _F, _F, _FP, C.POINTER(C.c_double)],
     "qmc_adjoint_backward": [_P, _P, _I, _FP, _FP, _FP],
+    "qmc_image_save_text": [C.c_char_p, _I, _I, _FP, _I, C.POINTER(ImageStats)],
     "qmc_event_record": [_P, _I],
     "qmc_event_elapsed_ms": [_P, _I, _I, _FP],
 }

@@ -20,9 +20,11 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 LIB = LIBDIR / "libqmcrt.so"
+BINDIR = PKG / "bin"
+PIPELINE = BINDIR / "qmc_pipeline"
 
 HIP_SOURCES = ["qmc_device.hip", "qmc_batch.hip", "qmc_adjoint.hip"]
-HOST_SOURCES = ["scene_host.cpp"]
+HOST_SOURCES = ["scene_host.cpp", "image_saver.cpp"]
 # The persistent batched kernel loops over rays; machine-level loop-invariant code motion hoists
 # copies of wave-uniform operands out of that loop into vector registers (about 25 of them), which
 # costs two waves per SIMD of occupancy on a latency-bound traversal.
@@ -74,7 +76,20 @@ def build_qmcrt(verbose: bool = False) -> Path:
         print(" ".join(cmd))
     _run(cmd)
     os.replace(tmp, LIB)
+    build_pipeline(verbose)
     return LIB
+
+
+def build_pipeline(verbose: bool = False) -> Path:
+    """The `qmc_pipeline` command-line program (the reference's RayTracingPipeline executable) on the
+    C ABI: host C++ linked against libqmcrt.so, found through an $ORIGIN-relative run path."""
+    BINDIR.mkdir(parents=True, exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"), str(CSRC / "pipeline_main.cpp"),
+           "-L", str(LIBDIR), "-lqmcrt", "-Wl,-rpath,$ORIGIN/../lib", "-o", str(PIPELINE)]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    return PIPELINE
 
 
 def build_oracle(verbose: bool = False) -> Path:

@@ -1,0 +1,76 @@
+"""End to end: the qmc_pipeline program on the reference's example inputs (InputFiles/example: 3
+heliostats, 737M rays, 320x200 map) writes, per receiver, the before- and after-smoothing text maps
+of RayTracingPipeline.cpp:145-175; they equal the library's own trace and smoothing of the same
+scene with the same seeds, up to the 2-decimal text rounding and float atomic order."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from solarenergy_chier_amd import Scene
+from solarenergy_chier_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+PIPELINE = ROOT / "solarenergy_chier_amd" / "bin" / "qmc_pipeline"
+EX = ROOT / "tests" / "golden" / "ref_fixtures" / "inputs" / "example"
+
+
+def read_text_map(path):
+    rows = [[float(v) for v in line.split(",")] for line in Path(path).read_text().splitlines()]
+    return np.array(rows[::-1], np.float64)  # file rows are bottom-up
+
+
+def library_maps(device):
+    s = Scene()
+    s.load_config(str(EX / "example_configuration.json"))
+    N.call("qmc_scene_load_file", s.handle, str(EX / "example_scene.scn").encode())
+    s.process()
+    device.upload(s)
+    device.pool_generate(2, 4)
+    device.offsets_generate(3)
+    device.flux_clear(0)
+    device.trace(range(s.counts.n_heliostats))
+    before = device.flux(0).astype(np.float64)
+    device.smooth(0, 5, 0.05)
+    after = device.flux(0).astype(np.float64)
+    return before, after
+
+
+def close_to_text(text_map, ref, name):
+    """Fraction of pixels whose text value differs from the library's map by more than the text
+    rounding (0.005) plus the float-atomic order of the two runs. A hot pixel sums ~1.5e5 deposits of
+    ~0.011 into ~1.7e3, where each float add rounds at 1.2e-4 (about 1% of the increment), so two
+    orders differ by up to ~3e-4 relative there; 5e-4 relative per pixel, and 1e-4 relative L1 over
+    the map, bound it."""
+    expect = np.where(ref.astype(np.float32) < 1e-5, 0.0, ref)
+    err = np.abs(text_map - expect)
+    tol = 0.005 + 5e-4 * np.abs(ref)
+    rel_l1 = float(err.sum() / np.abs(ref).sum())
+    print(f"{name}: max |diff| {err.max():.4f} at value {ref.flat[err.argmax()]:.2f}, rel L1 {rel_l1:.2e}, "
+          f"pixels over tolerance {(err > tol).mean():.2e}", flush=True)
+    return float((err > tol).mean()), rel_l1
+
+
+@pytest.mark.parametrize("style", ["long", "short"])
+def test_pipeline_program_matches_library(device, tmp_path, style):
+    out = str(tmp_path) + "/"
+    files = [str(EX / "example_configuration.json"), str(EX / "example_scene.scn"),
+             str(EX / "task_heliostats_index.txt"), out]
+    opts = (["--configuration_path", "--scene_path", "--heliostat_index_path", "--output_path"] if style == "long"
+            else ["-c", "-s", "-h", "-o"])
+    args = [x for pair in zip(opts, files) for x in pair]
+    print("running qmc_pipeline", style, flush=True)
+    r = subprocess.run([str(PIPELINE), *args], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    before_txt = read_text_map(tmp_path / "0_receiver_before_smooth.txt")
+    after_txt = read_text_map(tmp_path / "0_receiver_after_smooth.txt")
+    assert before_txt.shape == (200, 320) and after_txt.shape == (200, 320)
+    before, after = library_maps(device)
+    assert before.sum() > 0
+    for txt, ref, name in ((before_txt, before, "before smoothing"), (after_txt, after, "after smoothing")):
+        over, rel_l1 = close_to_text(txt, ref, name)
+        assert over == 0.0 and rel_l1 < 1e-4
