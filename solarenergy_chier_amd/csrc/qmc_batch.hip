@@ -115,8 +115,10 @@ __device__ __forceinline__ T ld_uniform(const T *p) {
     return out;
 }
 
+// 1024-thread workgroups are built for two per CU (8 waves per SIMD, at most 64 VGPRs): with a 64 KB LDS
+// map each, that is the only way an LDS-accumulated instance runs at full occupancy.
 template <int RT, bool LDSMAP, bool TILES, int BS>
-__global__ __launch_bounds__(BS) void k_trace_batched(BatchArgs a) {
+__global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchArgs a) {
     extern __shared__ float map_lds[];
     __shared__ int s_next;
     const int tid = threadIdx.x, bs = BS;

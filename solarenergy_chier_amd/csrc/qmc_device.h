@@ -88,6 +88,17 @@ struct StatsAcc {
     unsigned long long rays, unshadowed, unblocked, deposits, cells, candidates, tests, facets, edge;
 };
 
+struct CullArgs {
+    const DevHelio *helios;
+    const int32_t *list;
+    const float4 *quads;  // the grid's blocking quads (A, E1, E2)
+    int32_t n_sub;
+    DevFrame sun;         // frame of -sun_dir: shadow rays leave along its n axis
+    double tan_sun, tan_block;
+    double margin;
+    uint8_t *flags;
+};
+
 struct TraceArgs {
     DevGrid grid;
     DevReceiver recv;
@@ -101,6 +112,7 @@ struct TraceArgs {
     float *flux;
     qmc_ray_record *records;   // nullable (one heliostat per launch when set)
     StatsAcc *stats;           // nullable
+    const uint8_t *cull;       // per heliostat: bit 0 shadow traversal needed, bit 1 blocking traversal needed
     int32_t P, L;
     FastDiv fdP, fdL;
     int32_t n_list;

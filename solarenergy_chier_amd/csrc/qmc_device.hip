@@ -33,6 +33,39 @@ using namespace qmc;
 // ---------------------------------------------------------------------------------------------
 namespace {
 
+// Per heliostat of the launch: whether any ray of its shadow cone (directions within the sun-shape
+// pool's half-angle of the sun axis) or of its blocking cone (within sun shape + twice the slope
+// error of the ideal reflection) can meet another quad of the grid (cone_may_hit, conservative).
+// When none can, every test of that traversal is a miss, so skipping it returns the same result for
+// every ray. Flat heliostats only; others always traverse.
+__global__ __launch_bounds__(256) void k_cull(CullArgs a) {
+    const int h = a.list[blockIdx.x];
+    const DevHelio &H = a.helios[h];
+    bool may_shadow = H.type != 0 || a.tan_sun < 0.0;
+    bool may_block = H.type != 0 || a.tan_block < 0.0;
+    if (!(may_shadow && may_block)) {
+        const DevFrame &f = H.fr;
+        const CD3 u = f.identity ? CD3{1, 0, 0} : CD3{f.ux, f.uy, f.uz};
+        const CD3 n = f.identity ? CD3{0, 1, 0} : CD3{f.nx, f.ny, f.nz};
+        const CD3 v = f.identity ? CD3{0, 0, 1} : CD3{f.vx, f.vy, f.vz};
+        const CD3 c = cadd(CD3{H.pos[0], H.pos[1], H.pos[2]}, cmul(n, 0.5 * H.size[1]));
+        const CD3 hu = cmul(u, 0.5 * H.size[0]), hv = cmul(v, 0.5 * H.size[2]);
+        const CD3 ash = a.sun.identity ? CD3{0, 1, 0} : CD3{a.sun.nx, a.sun.ny, a.sun.nz};
+        CD3 r0 = cadd(cmul(ash, -1.0), cmul(n, 2.0 * cdot(ash, n)));  // reflect(-ash, n)
+        r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
+        for (int q = threadIdx.x; q < a.n_sub; q += blockDim.x) {
+            if (q >= H.sub_begin && q < H.sub_begin + H.sub_count) continue;
+            const float4 qa = a.quads[3 * q], qb = a.quads[3 * q + 1], qc = a.quads[3 * q + 2];
+            const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
+            if (!may_shadow && cone_may_hit(c, hu, hv, ash, a.tan_sun, A, E1, E2, a.margin)) may_shadow = true;
+            if (!may_block && cone_may_hit(c, hu, hv, r0, a.tan_block, A, E1, E2, a.margin)) may_block = true;
+        }
+    }
+    const int any_shadow = __syncthreads_or(may_shadow ? 1 : 0);
+    const int any_block = __syncthreads_or(may_block ? 1 : 0);
+    if (threadIdx.x == 0) a.flags[h] = (uint8_t)((any_shadow ? 1 : 0) | (any_block ? 2 : 0));
+}
+
 template <int RT, bool REC, bool STATS>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const int h = a.list[blockIdx.y];
@@ -79,7 +112,9 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         int index = -1;
         float energy = 0.0f;
         int flags = 0;
-        if (dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
+        // traversals the cull kernel proved hit-free are skipped (never while counting work)
+        const unsigned need = (STATS || !a.cull) ? 3u : (unsigned)a.cull[h];
+        if ((need & 1u) && dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
             status = QMC_RAY_SHADOWED;
             index = hit;
         } else {
@@ -89,7 +124,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             const float4 q = a.pert[a2];
             V3 normal = normalize(fr_apply(*pf, V3{q.x, q.y, q.z}));
             dir = normalize(reflect(-dir, normal));
-            if (dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
+            if ((need & 2u) && dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
                 status = QMC_RAY_BLOCKED;
                 index = hit;
             } else {
@@ -420,6 +455,7 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_aims);
     dev_free(c->d_batch_maps);
     dev_free(c->d_batch_work);
+    dev_free(c->d_cull);
     if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
     c->d_batch_inst = nullptr;
     qmc_adjoint_release(c);
@@ -719,6 +755,32 @@ static qmc_status ensure_pool(qmc_ctx *c, int P, int L) {
     return QMC_OK;
 }
 
+// Cone half-angles of the current pools for traversal culling: alpha = largest angle of a sun-shape
+// sample from the local y axis, beta = the same for the slope-error samples; shadow rays lie within
+// alpha of the sun axis, reflected rays within alpha + 2 beta of the ideal reflection. Computed in
+// double from the device pools, widened by 1e-4 rad; culling is off beyond 80 degrees.
+static qmc_status update_cull_angles(qmc_ctx *c) {
+    std::vector<float4> a((size_t)c->P), b((size_t)c->P);
+    HIP_TRY(hipMemcpyAsync(a.data(), c->d_pool, sizeof(float4) * a.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(b.data(), c->d_pert, sizeof(float4) * b.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    auto max_angle = [](const std::vector<float4> &v) {
+        double worst = 0.0;
+        for (const float4 &e : v) {
+            const double len = std::sqrt((double)e.x * e.x + (double)e.y * e.y + (double)e.z * e.z);
+            if (!(len > 0.0)) return 10.0;
+            const double cs = std::fmax(-1.0, std::fmin(1.0, e.y / len));
+            worst = std::fmax(worst, std::acos(cs));
+        }
+        return worst;
+    };
+    const double alpha = max_angle(a) + 1e-4, beta = max_angle(b) + 1e-4;
+    const double limit = 80.0 * M_PI / 180.0;
+    c->cull_tan_sun = alpha < limit ? std::tan(alpha) : -1.0;
+    c->cull_tan_block = alpha + 2.0 * beta < limit ? std::tan(alpha + 2.0 * beta) : -1.0;
+    return QMC_OK;
+}
+
 extern "C" qmc_status qmc_pool_generate(qmc_ctx *c, const qmc_pool_params *p) {
     if (!c || !p) return fail(QMC_E_INVALID_ARG, "qmc_pool_generate: null argument");
     if (!c->uploaded) return fail(QMC_E_STATE, "qmc_pool_generate: upload a scene first");
@@ -738,6 +800,7 @@ extern "C" qmc_status qmc_pool_generate(qmc_ctx *c, const qmc_pool_params *p) {
     hipLaunchKernelGGL(k_pool_perturbation, dim3(blocks), dim3(256), 0, c->stream, c->d_pert, P,
                        mix64(p->seed_perturbation), c->cfg.helio_disturb_std);
     HIP_TRY(hipGetLastError());
+    if ((st = update_cull_angles(c))) return st;
     c->pool_ready = true;
     return QMC_OK;
 }
@@ -767,6 +830,7 @@ extern "C" qmc_status qmc_pool_from_uniforms(qmc_ctx *c, const float *ut, const 
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dev_free(tmp);
     if (e != hipSuccess) return fail(QMC_E_HIP, std::string("k_pool_from_uniforms: ") + hipGetErrorString(e));
+    if ((st = update_cull_angles(c))) return st;
     c->pool_ready = true;
     return QMC_OK;
 }
@@ -784,6 +848,7 @@ extern "C" qmc_status qmc_pool_set(qmc_ctx *c, const float *dirs, const float *p
     HIP_TRY(hipMemcpyAsync(c->d_pool, a.data(), sizeof(float4) * P, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_pert, b.data(), sizeof(float4) * P, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if ((st = update_cull_angles(c))) return st;
     c->pool_ready = true;
     return QMC_OK;
 }
@@ -925,6 +990,26 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     if (n > 65535) return fail(QMC_E_UNSUPPORTED, "trace: more than 65535 heliostats in one grid launch");
     a.blocks_per_helio = (int)bx;
     if (bx == 0 || n == 0) return QMC_OK;
+    if (!stats) {
+        qmc_status st;
+        if (c->cull_cap < c->h_helios.size()) {
+            if ((st = dev_alloc(c->d_cull, c->h_helios.size()))) return st;
+            c->cull_cap = c->h_helios.size();
+        }
+        CullArgs ca;
+        ca.helios = c->d_helios;
+        ca.list = d_list;
+        ca.quads = gb.quads;
+        ca.n_sub = gb.n_sub;
+        ca.sun = c->sun;
+        ca.tan_sun = c->cull_tan_sun;
+        ca.tan_block = c->cull_tan_block;
+        ca.margin = 0.01;  // metres: far above the float rounding of origins, directions and the hit test
+        ca.flags = c->d_cull;
+        hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
+        HIP_TRY(hipGetLastError());
+        a.cull = c->d_cull;
+    }
     dim3 g((unsigned)bx, (unsigned)n), b(256);
     const int rt = rb.r.type;
     if (rec) {

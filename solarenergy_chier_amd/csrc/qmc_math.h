@@ -172,6 +172,59 @@ QMC_HD uint32_t fast_div(uint32_t n, FastDiv f) {
 }
 QMC_HD uint32_t fast_mod(uint32_t n, uint32_t d, FastDiv f) { return n - fast_div(n, f) * d; }
 
+// ---------------------------------------------------------------------------------------------
+// Conservative ray-cone culling (double precision). cone_may_hit returns false only when no ray
+// p + t*d with p in the rectangle c +- hu +- hv, t >= 0 and d within angle theta (tan_theta, theta
+// below 90 deg) of the unit axis a can meet the parallelogram A + s*E1 + r*E2 (s, r in [0, 1]) —
+// then every ray/parallelogram test of such a ray against it is a miss.
+// Bound: along the axis a ray point x = p + t*d has (x-p).a = t*cos(phi) <= Tmax (the largest axial
+// gap between a quad vertex and a rectangle vertex) and a sideways offset |(x-p).w| = t*|d.w| <=
+// Tmax*tan(theta) for any unit w orthogonal to a. So if, along some such w, the projections of the
+// rectangle and the quad are further apart than Tmax*tan(theta) + margin, or the quad lies wholly
+// behind the rectangle along a, no ray reaches it. Candidate w: a x (each edge of both shapes) and the
+// in-plane direction between their centres. `margin` (metres) absorbs the float rounding of traced
+// origins, directions and the intersection test.
+struct CD3 {
+    double x, y, z;
+};
+QMC_HD CD3 cd3(V3 v) { return CD3{(double)v.x, (double)v.y, (double)v.z}; }
+QMC_HD CD3 cadd(CD3 a, CD3 b) { return CD3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+QMC_HD CD3 csub(CD3 a, CD3 b) { return CD3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+QMC_HD CD3 cmul(CD3 a, double s) { return CD3{a.x * s, a.y * s, a.z * s}; }
+QMC_HD double cdot(CD3 a, CD3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+QMC_HD CD3 ccross(CD3 a, CD3 b) { return CD3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+QMC_HD bool cone_may_hit(CD3 c, CD3 hu, CD3 hv, CD3 a, double tan_theta, CD3 A, CD3 E1, CD3 E2, double margin) {
+    CD3 P[4] = {cadd(cadd(c, hu), hv), csub(cadd(c, hu), hv), cadd(csub(c, hu), hv), csub(csub(c, hu), hv)};
+    CD3 Q[4] = {A, cadd(A, E1), cadd(A, E2), cadd(cadd(A, E1), E2)};
+    double pa_min = 1e300, qa_max = -1e300;
+    for (int i = 0; i < 4; ++i) {
+        pa_min = fmin(pa_min, cdot(P[i], a));
+        qa_max = fmax(qa_max, cdot(Q[i], a));
+    }
+    const double tmax = qa_max - pa_min;
+    if (tmax < -margin) return false;  // the quad lies wholly behind every origin
+    const double reach = fmax(tmax, 0.0) * tan_theta + margin;
+    const CD3 qc = cadd(A, cmul(cadd(E1, E2), 0.5));
+    const CD3 dc = csub(qc, c);
+    CD3 axes[5] = {ccross(a, hu), ccross(a, hv), ccross(a, E1), ccross(a, E2), csub(dc, cmul(a, cdot(dc, a)))};
+    for (int k = 0; k < 5; ++k) {
+        const double len = sqrt(cdot(axes[k], axes[k]));
+        if (!(len > 1e-12)) continue;
+        const CD3 w = cmul(axes[k], 1.0 / len);
+        double p_lo = 1e300, p_hi = -1e300, q_lo = 1e300, q_hi = -1e300;
+        for (int i = 0; i < 4; ++i) {
+            const double pw = cdot(P[i], w), qw = cdot(Q[i], w);
+            p_lo = fmin(p_lo, pw);
+            p_hi = fmax(p_hi, pw);
+            q_lo = fmin(q_lo, qw);
+            q_hi = fmax(q_hi, qw);
+        }
+        if (q_lo > p_hi + reach || q_hi < p_lo - reach) return false;
+    }
+    return true;
+}
+
 QMC_HD float abs_divide(float den, float num) {
     if (num <= kEps && num >= -kEps) return kIntMaxF;
     return fabsf(den / num);
