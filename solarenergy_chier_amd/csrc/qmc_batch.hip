@@ -52,6 +52,7 @@ struct BatchArgs {
     const int32_t *__restrict__ offsets;
     int32_t P, L;
     FastDiv fdP, fdL;
+    double tan_sun, tan_block;  // traversal-culling cone tangents (negative: no culling)
     float rho;
     float *maps;
     int32_t npix;
@@ -60,6 +61,7 @@ struct BatchArgs {
     DevFrame *fr;      // [instance][heliostat]
     float *fac;        // [instance][heliostat]
     float4 *quads;     // [instance][sub-heliostat][3]
+    uint32_t *cull;    // [instance][heliostat]: bit 0 shadow, bit 1 blocking traversal needed (null: both)
     int32_t *queue;    // next instance to take (reset by k_batch_setup)
 };
 
@@ -99,6 +101,22 @@ __global__ __launch_bounds__(256) void k_batch_setup(BatchArgs a) {
     q[2] = make_float4(E2.x, E2.y, E2.z, 0.0f);
     a.fac[(size_t)ii * a.n_helio + k] = H.fac_rule == 0 ? I.dni * H.fac_x * H.fac_x * a.rho / float(a.L)
                                                           : I.dni * H.fac_x * a.rho / float(a.L);
+}
+
+// Traversal culling of every (instance, heliostat), one wavefront each, from the frames and quads
+// k_batch_setup wrote: the per-instance counterpart of the plain trace's k_cull. Bits left clear mark
+// shadow / blocking traversals that can only miss for that instance's sun and aim points.
+__global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
+    const int ii = blockIdx.y;
+    const int k = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (k >= a.n_helio) return;
+    const DevHelio &H = a.helios[a.helio_begin + k];
+    const DevFrame fr = a.fr[(size_t)ii * a.n_helio + k];
+    bool may_shadow = false, may_block = false;
+    cull_scan(H, fr, a.sun_fr[ii], a.quads + (size_t)ii * a.n_sub * 3, a.n_sub, a.tan_sun, a.tan_block,
+              kCullMargin, lane, 64, may_shadow, may_block);
+    const bool any_shadow = __ballot(may_shadow) != 0, any_block = __ballot(may_block) != 0;
+    if (lane == 0) a.cull[(size_t)ii * a.n_helio + k] = (any_shadow ? 1u : 0u) | (any_block ? 2u : 0u);
 }
 
 // Load of data that no kernel of the launch writes (per-heliostat and per-instance records) through
@@ -148,6 +166,8 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
             const float factor = ld_uniform(a.fac + (size_t)ii * a.n_helio + k);
             const int32_t *grp = a.offsets + H.offsets_base;
             const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
+            const uint32_t need = a.cull ? ld_uniform(a.cull + (size_t)ii * a.n_helio + k) : 3u;
+            const bool shadow_pass = (need & 1u) != 0, block_pass = (need & 2u) != 0;
             const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
             const FacetTiling tl = facet_tiling(R, C, L);
             const int nslots = TILES ? tl.slots : H.F * L;
@@ -161,12 +181,12 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
                 V3 dir = fr_apply(sunf, V3{s.x, s.y, s.z});
                 const V3 origin = flat_facet_origin(H, fr, m);
                 int hit;
-                if (dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
+                if (shadow_pass && dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
                 const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
                 const float4 q = a.pert[a2];
                 const V3 normal = normalize(fr_apply(fr, V3{q.x, q.y, q.z}));
                 dir = normalize(reflect(-dir, normal));
-                if (dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
+                if (block_pass && dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
                 float t, u, v;
                 if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v)) continue;
                 const float energy = ray_energy(t, dir, normal, factor);
@@ -246,7 +266,8 @@ size_t lds_bytes(const BatchArgs &a, bool lds_map) { return lds_map ? sizeof(flo
 
 // Per-instance data of one chunk, carved out of one ctx-owned allocation.
 size_t work_bytes_per_instance(const BatchArgs &a) {
-    return sizeof(DevFrame) * (size_t)(a.n_helio + 1) + sizeof(float) * a.n_helio + sizeof(float4) * 3 * a.n_sub;
+    return sizeof(DevFrame) * (size_t)(a.n_helio + 1) + (sizeof(float) + sizeof(uint32_t)) * a.n_helio +
+           sizeof(float4) * 3 * a.n_sub;
 }
 
 qmc_status ensure_work(qmc_ctx *c, BatchArgs &a, int chunk) {
@@ -254,8 +275,9 @@ qmc_status ensure_work(qmc_ctx *c, BatchArgs &a, int chunk) {
     const size_t fac = sizeof(float) * (size_t)chunk * a.n_helio;
     const size_t quads = sizeof(float4) * 3 * (size_t)chunk * a.n_sub;
     const size_t sun = sizeof(DevFrame) * (size_t)chunk;
+    const size_t cull = sizeof(uint32_t) * (size_t)chunk * a.n_helio;
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t total = up(quads) + up(fr) + up(fac) + up(sun) + 256;
+    const size_t total = up(quads) + up(fr) + up(fac) + up(sun) + up(cull) + 256;
     if (total > c->batch_work_cap) {
         qmc_status st = dev_alloc(c->d_batch_work, total);
         if (st) {
@@ -273,6 +295,9 @@ qmc_status ensure_work(qmc_ctx *c, BatchArgs &a, int chunk) {
     p += up(fac);
     a.sun_fr = reinterpret_cast<DevFrame *>(p);
     p += up(sun);
+    // culling needs both cone tangents; without them every traversal runs
+    a.cull = (a.tan_sun >= 0.0 && a.tan_block >= 0.0) ? reinterpret_cast<uint32_t *>(p) : nullptr;
+    p += up(cull);
     a.queue = reinterpret_cast<int32_t *>(p);
     return QMC_OK;
 }
@@ -339,6 +364,9 @@ qmc_status launch_batched(qmc_ctx *c, const BatchPlan &plan, const qmc_schedule 
         b.maps = a.maps + (size_t)base * a.npix;
         hipLaunchKernelGGL(k_batch_setup, dim3((unsigned)((a.n_helio + 255) / 256), (unsigned)b.n_inst), dim3(256), 0,
                            c->stream, b);
+        if (b.cull)
+            hipLaunchKernelGGL(k_batch_cull, dim3((unsigned)((a.n_helio + 3) / 4), (unsigned)b.n_inst), dim3(256), 0,
+                               c->stream, b);
         // no more workgroups than can be resident: a queue-driven persistent grid
         int resident = 0;
         const bool tiles = s.facet_tiles != 0;
@@ -394,6 +422,8 @@ qmc_status plan_batch(qmc_ctx *c, int grid, const qmc_instance *inst, int n, Bat
     a.L = c->L;
     a.fdP = make_fastdiv((uint32_t)c->P);
     a.fdL = make_fastdiv((uint32_t)c->L);
+    a.tan_sun = c->cull_tan_sun;
+    a.tan_block = c->cull_tan_block;
     a.rho = c->cfg.helio_reflected_rate;
     a.npix = rb.w * rb.h;
     plan.rt = rb.r.type;

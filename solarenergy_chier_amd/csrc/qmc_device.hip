@@ -43,24 +43,8 @@ __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
     const DevHelio &H = a.helios[h];
     bool may_shadow = H.type != 0 || a.tan_sun < 0.0;
     bool may_block = H.type != 0 || a.tan_block < 0.0;
-    if (!(may_shadow && may_block)) {
-        const DevFrame &f = H.fr;
-        const CD3 u = f.identity ? CD3{1, 0, 0} : CD3{f.ux, f.uy, f.uz};
-        const CD3 n = f.identity ? CD3{0, 1, 0} : CD3{f.nx, f.ny, f.nz};
-        const CD3 v = f.identity ? CD3{0, 0, 1} : CD3{f.vx, f.vy, f.vz};
-        const CD3 c = cadd(CD3{H.pos[0], H.pos[1], H.pos[2]}, cmul(n, 0.5 * H.size[1]));
-        const CD3 hu = cmul(u, 0.5 * H.size[0]), hv = cmul(v, 0.5 * H.size[2]);
-        const CD3 ash = a.sun.identity ? CD3{0, 1, 0} : CD3{a.sun.nx, a.sun.ny, a.sun.nz};
-        CD3 r0 = cadd(cmul(ash, -1.0), cmul(n, 2.0 * cdot(ash, n)));  // reflect(-ash, n)
-        r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
-        for (int q = threadIdx.x; q < a.n_sub; q += blockDim.x) {
-            if (q >= H.sub_begin && q < H.sub_begin + H.sub_count) continue;
-            const float4 qa = a.quads[3 * q], qb = a.quads[3 * q + 1], qc = a.quads[3 * q + 2];
-            const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
-            if (!may_shadow && cone_may_hit(c, hu, hv, ash, a.tan_sun, A, E1, E2, a.margin)) may_shadow = true;
-            if (!may_block && cone_may_hit(c, hu, hv, r0, a.tan_block, A, E1, E2, a.margin)) may_block = true;
-        }
-    }
+    cull_scan(H, H.fr, a.sun, a.quads, a.n_sub, a.tan_sun, a.tan_block, a.margin, threadIdx.x, blockDim.x,
+              may_shadow, may_block);
     const int any_shadow = __syncthreads_or(may_shadow ? 1 : 0);
     const int any_block = __syncthreads_or(may_block ? 1 : 0);
     if (threadIdx.x == 0) a.flags[h] = (uint8_t)((any_shadow ? 1 : 0) | (any_block ? 2 : 0));
@@ -1004,7 +988,7 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
         ca.sun = c->sun;
         ca.tan_sun = c->cull_tan_sun;
         ca.tan_block = c->cull_tan_block;
-        ca.margin = 0.01;  // metres: far above the float rounding of origins, directions and the hit test
+        ca.margin = kCullMargin;
         ca.flags = c->d_cull;
         hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
         HIP_TRY(hipGetLastError());

@@ -155,6 +155,36 @@ __device__ __forceinline__ FastDiv tile_divisor(const DevHelio &H, const FacetTi
     return tl.ltw == 3 ? H.fd_ntx8 : H.fd_ntx4;
 }
 
+// One thread's share of the traversal-culling scan of flat heliostat H (frame fr) against the quads
+// q = first, first + step, ... of a grid: sets may_shadow / may_block when some quad other than H's
+// own can meet its shadow cone (tan_sun around the sun axis) or blocking cone (tan_block around the
+// ideal reflection), per cone_may_hit. The caller ORs the flags over its threads; a traversal whose
+// flag stays clear can only miss.
+__device__ __forceinline__ void cull_scan(const DevHelio &H, const DevFrame &f, const DevFrame &sunf,
+                                          const float4 *quads, int n_sub, double tan_sun, double tan_block,
+                                          double margin, int first, int step, bool &may_shadow, bool &may_block) {
+    if (may_shadow && may_block) return;
+    const CD3 u = f.identity ? CD3{1, 0, 0} : CD3{f.ux, f.uy, f.uz};
+    const CD3 n = f.identity ? CD3{0, 1, 0} : CD3{f.nx, f.ny, f.nz};
+    const CD3 v = f.identity ? CD3{0, 0, 1} : CD3{f.vx, f.vy, f.vz};
+    const CD3 c = cadd(CD3{H.pos[0], H.pos[1], H.pos[2]}, cmul(n, 0.5 * H.size[1]));
+    const CD3 hu = cmul(u, 0.5 * H.size[0]), hv = cmul(v, 0.5 * H.size[2]);
+    const CD3 ash = sunf.identity ? CD3{0, 1, 0} : CD3{sunf.nx, sunf.ny, sunf.nz};
+    CD3 r0 = cadd(cmul(ash, -1.0), cmul(n, 2.0 * cdot(ash, n)));  // reflect(-sun axis, n)
+    r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
+    for (int q = first; q < n_sub; q += step) {
+        if (q >= H.sub_begin && q < H.sub_begin + H.sub_count) continue;
+        const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
+        const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
+        if (!may_shadow && cone_may_hit(c, hu, hv, ash, tan_sun, A, E1, E2, margin)) may_shadow = true;
+        if (!may_block && cone_may_hit(c, hu, hv, r0, tan_block, A, E1, E2, margin)) may_block = true;
+    }
+}
+
+// Margin (metres) of the culling bound: far above the float rounding of traced origins, directions
+// and the intersection test.
+constexpr double kCullMargin = 0.01;
+
 // Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
 // onto the facet normal.
 __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
