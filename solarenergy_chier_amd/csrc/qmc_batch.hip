@@ -111,10 +111,10 @@ __global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
     const int k = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
     if (k >= a.n_helio) return;
     const DevHelio &H = a.helios[a.helio_begin + k];
-    const DevFrame fr = a.fr[(size_t)ii * a.n_helio + k];
+    const DevCullShape S = flat_cull_shape(H, a.fr[(size_t)ii * a.n_helio + k]);
     bool may_shadow = false, may_block = false;
-    cull_scan(H, fr, a.sun_fr[ii], a.quads + (size_t)ii * a.n_sub * 3, a.n_sub, a.tan_sun, a.tan_block,
-              kCullMargin, lane, 64, may_shadow, may_block);
+    cull_scan(S, a.sun_fr[ii], a.quads + (size_t)ii * a.n_sub * 3, a.n_sub, H.sub_begin, H.sub_begin + H.sub_count,
+              a.tan_sun, a.tan_block, kCullMargin, lane, 64, may_shadow, may_block);
     const bool any_shadow = __ballot(may_shadow) != 0, any_block = __ballot(may_block) != 0;
     if (lane == 0) a.cull[(size_t)ii * a.n_helio + k] = (any_shadow ? 1u : 0u) | (any_block ? 2u : 0u);
 }

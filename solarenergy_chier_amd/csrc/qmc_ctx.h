@@ -75,8 +75,9 @@ struct qmc_ctx {
     size_t max_lds = 0;
     // traversal culling: tangents of the sample pools' cone half-angles (sun shape; sun shape plus
     // twice the slope error), negative when culling does not apply, and a flag byte per heliostat
-    double cull_tan_sun = -1.0, cull_tan_block = -1.0;
+    double cull_tan_sun = -1.0, cull_tan_block = -1.0, cull_ang_block = -1.0;
     uint8_t *d_cull = nullptr;
+    qmc::DevCullShape *d_cull_shapes = nullptr;
     size_t cull_cap = 0;
     // adjoint
     void *adj = nullptr;

@@ -88,13 +88,23 @@ struct StatsAcc {
     unsigned long long rays, unshadowed, unblocked, deposits, cells, candidates, tests, facets, edge;
 };
 
+// Where a heliostat's traced rays start and how far their reflections turn, for traversal culling:
+// every micro-facet origin lies in the box c +- hu +- hn +- hv (world, metres), and every facet
+// normal is within `cant` radians of the heliostat normal n (0 for a flat heliostat).
+struct DevCullShape {
+    double c[3], hu[3], hn[3], hv[3], n[3];
+    double cant;
+};
+
 struct CullArgs {
     const DevHelio *helios;
+    const DevCullShape *shapes;
     const int32_t *list;
     const float4 *quads;  // the grid's blocking quads (A, E1, E2)
     int32_t n_sub;
     DevFrame sun;         // frame of -sun_dir: shadow rays leave along its n axis
-    double tan_sun, tan_block;
+    double tan_sun;       // cone of shadow rays (negative: no culling)
+    double ang_block;     // half-angle of the blocking cone of a flat heliostat (negative: no culling)
     double margin;
     uint8_t *flags;
 };

@@ -41,10 +41,13 @@ namespace {
 __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
     const int h = a.list[blockIdx.x];
     const DevHelio &H = a.helios[h];
-    bool may_shadow = H.type != 0 || a.tan_sun < 0.0;
-    bool may_block = H.type != 0 || a.tan_block < 0.0;
-    cull_scan(H, H.fr, a.sun, a.quads, a.n_sub, a.tan_sun, a.tan_block, a.margin, threadIdx.x, blockDim.x,
-              may_shadow, may_block);
+    const DevCullShape S = a.shapes[h];
+    // canted facets turn reflections further from the ideal one: twice the canting angle
+    const double ang_block = a.ang_block < 0.0 ? -1.0 : a.ang_block + 2.0 * S.cant;
+    const double tan_block = (ang_block >= 0.0 && ang_block < kCullMaxAngle) ? tan(ang_block) : -1.0;
+    bool may_shadow = a.tan_sun < 0.0, may_block = tan_block < 0.0;
+    cull_scan(S, a.sun, a.quads, a.n_sub, H.sub_begin, H.sub_begin + H.sub_count, a.tan_sun, tan_block, a.margin,
+              threadIdx.x, blockDim.x, may_shadow, may_block);
     const int any_shadow = __syncthreads_or(may_shadow ? 1 : 0);
     const int any_block = __syncthreads_or(may_block ? 1 : 0);
     if (threadIdx.x == 0) a.flags[h] = (uint8_t)((any_shadow ? 1 : 0) | (any_block ? 2 : 0));
@@ -440,6 +443,7 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_batch_maps);
     dev_free(c->d_batch_work);
     dev_free(c->d_cull);
+    dev_free(c->d_cull_shapes);
     if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
     c->d_batch_inst = nullptr;
     qmc_adjoint_release(c);
@@ -466,9 +470,53 @@ namespace {
 
 // Per-heliostat device records (frames, layout, energy factor, offsets base) and focus-flat facet
 // records of a processed scene; returns the total number of group offsets (F+1 per heliostat).
-int64_t pack_heliostats(const qmc_scene &sc, std::vector<DevHelio> &out, std::vector<DevFacet> &facets) {
+// Culling shape of a focus-flat heliostat (facet records [f0, f0 + nf) in the heliostat frame): the
+// box, in the focus-variant heliostat frame, around every facet's four corners, and the largest angle
+// between a facet's world normal and the heliostat normal.
+DevCullShape focus_cull_shape(const DevHelio &H, const DevFacet *fc, int nf) {
+    auto axes = [](const DevFrame &f, CD3 &u, CD3 &n, CD3 &v) {
+        u = f.identity ? CD3{1, 0, 0} : CD3{f.ux, f.uy, f.uz};
+        n = f.identity ? CD3{0, 1, 0} : CD3{f.nx, f.ny, f.nz};
+        v = f.identity ? CD3{0, 0, 1} : CD3{f.vx, f.vy, f.vz};
+    };
+    CD3 lo{1e300, 1e300, 1e300}, hi{-1e300, -1e300, -1e300};
+    for (int k = 0; k < nf; ++k) {
+        CD3 u, n, v;
+        axes(fc[k].fr_local, u, n, v);
+        for (int i = 0; i < 4; ++i) {
+            const double x = (i & 1 ? 0.5 : -0.5) * H.len_col, z = (i & 2 ? 0.5 : -0.5) * H.len_row;
+            const CD3 p = cadd(CD3{fc[k].center[0], fc[k].center[1], fc[k].center[2]}, cadd(cmul(u, x), cmul(v, z)));
+            lo = CD3{std::fmin(lo.x, p.x), std::fmin(lo.y, p.y), std::fmin(lo.z, p.z)};
+            hi = CD3{std::fmax(hi.x, p.x), std::fmax(hi.y, p.y), std::fmax(hi.z, p.z)};
+        }
+    }
+    CD3 u, n, v;
+    axes(H.fr_focus, u, n, v);
+    const CD3 mid = cmul(cadd(lo, hi), 0.5), half = cmul(csub(hi, lo), 0.5);
+    const CD3 c = cadd(CD3{H.pos[0], H.pos[1], H.pos[2]}, cadd(cadd(cmul(u, mid.x), cmul(n, mid.y)), cmul(v, mid.z)));
+    const CD3 hu = cmul(u, half.x), hn = cmul(n, half.y), hv = cmul(v, half.z);
+    CD3 hu0, nh, hv0;
+    axes(H.fr, hu0, nh, hv0);
+    double cant = 0.0;
+    for (int k = 0; k < nf; ++k) {
+        const CD3 w{fc[k].normal[0], fc[k].normal[1], fc[k].normal[2]};
+        cant = std::fmax(cant, std::atan2(std::sqrt(cdot(ccross(w, nh), ccross(w, nh))), cdot(w, nh)));
+    }
+    DevCullShape s;
+    s.c[0] = c.x; s.c[1] = c.y; s.c[2] = c.z;
+    s.hu[0] = hu.x; s.hu[1] = hu.y; s.hu[2] = hu.z;
+    s.hn[0] = hn.x; s.hn[1] = hn.y; s.hn[2] = hn.z;
+    s.hv[0] = hv.x; s.hv[1] = hv.y; s.hv[2] = hv.z;
+    s.n[0] = nh.x; s.n[1] = nh.y; s.n[2] = nh.z;
+    s.cant = cant + 1e-5;  // the float rounding of the traced normals
+    return s;
+}
+
+int64_t pack_heliostats(const qmc_scene &sc, std::vector<DevHelio> &out, std::vector<DevFacet> &facets,
+                        std::vector<DevCullShape> &shapes) {
     const qmc_scene *s = &sc;
     out.resize(s->heliostats.size());
+    shapes.resize(s->heliostats.size());
     facets.clear();
     int64_t off_base = 0;
     for (size_t h = 0; h < s->heliostats.size(); ++h) {
@@ -523,6 +571,9 @@ int64_t pack_heliostats(const qmc_scene &sc, std::vector<DevHelio> &out, std::ve
                 f.normal[2] = wn.z;
                 facets.push_back(f);
             }
+            shapes[h] = focus_cull_shape(d, facets.data() + d.facet_base, (int)hh.loc_normals.size());
+        } else {
+            shapes[h] = flat_cull_shape(d, d.fr);
         }
     }
     return off_base;
@@ -635,7 +686,8 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     }
     // heliostats and focus-flat facets
     std::vector<DevFacet> facets;
-    int64_t off_base = pack_heliostats(*s, c->h_helios, facets);
+    std::vector<DevCullShape> shapes;
+    int64_t off_base = pack_heliostats(*s, c->h_helios, facets, shapes);
     aim_points_f64(*s, c->h_aim64);
     c->offsets_total = off_base;
     if ((st = dev_alloc(c->d_helios, c->h_helios.size()))) return st;
@@ -647,6 +699,10 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     if (!facets.empty())
         HIP_TRY(hipMemcpyAsync(c->d_facets, facets.data(), facets.size() * sizeof(DevFacet), hipMemcpyHostToDevice,
                                c->stream));
+    if ((st = dev_alloc(c->d_cull_shapes, shapes.size()))) return st;
+    if (!shapes.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_cull_shapes, shapes.data(), shapes.size() * sizeof(DevCullShape),
+                               hipMemcpyHostToDevice, c->stream));
     if ((int)s->heliostats.size() > c->list_cap) {
         if ((st = dev_alloc(c->d_list, s->heliostats.size()))) return st;
         c->list_cap = (int)s->heliostats.size();
@@ -692,8 +748,9 @@ extern "C" qmc_status qmc_update(qmc_ctx *c, const qmc_scene *s) {
         same = s->receivers[r].res_x * s->receivers[r].res_y == c->recvs[r].w * c->recvs[r].h;
     std::vector<DevHelio> helios;
     std::vector<DevFacet> facets;
+    std::vector<DevCullShape> shapes;
     if (same) {
-        pack_heliostats(*s, helios, facets);
+        pack_heliostats(*s, helios, facets, shapes);
         for (size_t h = 0; same && h < helios.size(); ++h)
             same = helios[h].F == c->h_helios[h].F && helios[h].type == c->h_helios[h].type &&
                    helios[h].sub_count == c->h_helios[h].sub_count && helios[h].grid == c->h_helios[h].grid;
@@ -711,6 +768,9 @@ extern "C" qmc_status qmc_update(qmc_ctx *c, const qmc_scene *s) {
     if (!facets.empty())
         HIP_TRY(hipMemcpyAsync(c->d_facets, facets.data(), facets.size() * sizeof(DevFacet), hipMemcpyHostToDevice,
                                c->stream));
+    if (!shapes.empty())
+        HIP_TRY(hipMemcpyAsync(c->d_cull_shapes, shapes.data(), shapes.size() * sizeof(DevCullShape),
+                               hipMemcpyHostToDevice, c->stream));
     for (size_t g = 0; g < s->grids.size(); ++g) {
         std::vector<float4> q = pack_quads(s->grids[g]);
         if (!q.empty())
@@ -759,9 +819,9 @@ static qmc_status update_cull_angles(qmc_ctx *c) {
         return worst;
     };
     const double alpha = max_angle(a) + 1e-4, beta = max_angle(b) + 1e-4;
-    const double limit = 80.0 * M_PI / 180.0;
-    c->cull_tan_sun = alpha < limit ? std::tan(alpha) : -1.0;
-    c->cull_tan_block = alpha + 2.0 * beta < limit ? std::tan(alpha + 2.0 * beta) : -1.0;
+    c->cull_tan_sun = alpha < kCullMaxAngle ? std::tan(alpha) : -1.0;
+    c->cull_ang_block = alpha + 2.0 * beta < kCullMaxAngle ? alpha + 2.0 * beta : -1.0;
+    c->cull_tan_block = c->cull_ang_block >= 0.0 ? std::tan(c->cull_ang_block) : -1.0;
     return QMC_OK;
 }
 
@@ -986,8 +1046,9 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
         ca.quads = gb.quads;
         ca.n_sub = gb.n_sub;
         ca.sun = c->sun;
+        ca.shapes = c->d_cull_shapes;
         ca.tan_sun = c->cull_tan_sun;
-        ca.tan_block = c->cull_tan_block;
+        ca.ang_block = c->cull_ang_block;
         ca.margin = kCullMargin;
         ca.flags = c->d_cull;
         hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);

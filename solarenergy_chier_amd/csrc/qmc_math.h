@@ -174,15 +174,15 @@ QMC_HD uint32_t fast_mod(uint32_t n, uint32_t d, FastDiv f) { return n - fast_di
 
 // ---------------------------------------------------------------------------------------------
 // Conservative ray-cone culling (double precision). cone_may_hit returns false only when no ray
-// p + t*d with p in the rectangle c +- hu +- hv, t >= 0 and d within angle theta (tan_theta, theta
-// below 90 deg) of the unit axis a can meet the parallelogram A + s*E1 + r*E2 (s, r in [0, 1]) —
-// then every ray/parallelogram test of such a ray against it is a miss.
+// p + t*d with p in the box c +- hu +- hn +- hv (hn = 0: a rectangle), t >= 0 and d within angle
+// theta (tan_theta, theta below 90 deg) of the unit axis a can meet the parallelogram A + s*E1 + r*E2
+// (s, r in [0, 1]) — then every ray/parallelogram test of such a ray against it is a miss.
 // Bound: along the axis a ray point x = p + t*d has (x-p).a = t*cos(phi) <= Tmax (the largest axial
-// gap between a quad vertex and a rectangle vertex) and a sideways offset |(x-p).w| = t*|d.w| <=
+// gap between a quad vertex and a box vertex) and a sideways offset |(x-p).w| = t*|d.w| <=
 // Tmax*tan(theta) for any unit w orthogonal to a. So if, along some such w, the projections of the
-// rectangle and the quad are further apart than Tmax*tan(theta) + margin, or the quad lies wholly
-// behind the rectangle along a, no ray reaches it. Candidate w: a x (each edge of both shapes) and the
-// in-plane direction between their centres. `margin` (metres) absorbs the float rounding of traced
+// box and the quad are further apart than Tmax*tan(theta) + margin, or the quad lies wholly behind the
+// box along a, no ray reaches it. Candidate w: a x (each edge of both shapes) and the direction
+// between their centres orthogonal to a. `margin` (metres) absorbs the float rounding of traced
 // origins, directions and the intersection test.
 struct CD3 {
     double x, y, z;
@@ -194,35 +194,56 @@ QMC_HD CD3 cmul(CD3 a, double s) { return CD3{a.x * s, a.y * s, a.z * s}; }
 QMC_HD double cdot(CD3 a, CD3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 QMC_HD CD3 ccross(CD3 a, CD3 b) { return CD3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 
-QMC_HD bool cone_may_hit(CD3 c, CD3 hu, CD3 hv, CD3 a, double tan_theta, CD3 A, CD3 E1, CD3 E2, double margin) {
-    CD3 P[4] = {cadd(cadd(c, hu), hv), csub(cadd(c, hu), hv), cadd(csub(c, hu), hv), csub(csub(c, hu), hv)};
+QMC_HD bool cone_may_hit(CD3 c, CD3 hu, CD3 hn, CD3 hv, CD3 a, double tan_theta, CD3 A, CD3 E1, CD3 E2,
+                         double margin) {
+    const bool thick = cdot(hn, hn) > 0.0;
+    const int np = thick ? 8 : 4;
+    CD3 P[8];
+    for (int i = 0; i < 8; ++i)
+        P[i] = cadd(cadd(cadd(c, cmul(hu, (i & 1) ? -1.0 : 1.0)), cmul(hv, (i & 2) ? -1.0 : 1.0)),
+                    cmul(hn, (i & 4) ? -1.0 : 1.0));
     CD3 Q[4] = {A, cadd(A, E1), cadd(A, E2), cadd(cadd(A, E1), E2)};
     double pa_min = 1e300, qa_max = -1e300;
-    for (int i = 0; i < 4; ++i) {
-        pa_min = fmin(pa_min, cdot(P[i], a));
-        qa_max = fmax(qa_max, cdot(Q[i], a));
-    }
+    for (int i = 0; i < np; ++i) pa_min = fmin(pa_min, cdot(P[i], a));
+    for (int i = 0; i < 4; ++i) qa_max = fmax(qa_max, cdot(Q[i], a));
     const double tmax = qa_max - pa_min;
     if (tmax < -margin) return false;  // the quad lies wholly behind every origin
     const double reach = fmax(tmax, 0.0) * tan_theta + margin;
     const CD3 qc = cadd(A, cmul(cadd(E1, E2), 0.5));
     const CD3 dc = csub(qc, c);
-    CD3 axes[5] = {ccross(a, hu), ccross(a, hv), ccross(a, E1), ccross(a, E2), csub(dc, cmul(a, cdot(dc, a)))};
-    for (int k = 0; k < 5; ++k) {
+    CD3 axes[6] = {ccross(a, hu), ccross(a, hv), ccross(a, E1), ccross(a, E2), csub(dc, cmul(a, cdot(dc, a))),
+                   ccross(a, hn)};
+    for (int k = 0; k < 6; ++k) {
         const double len = sqrt(cdot(axes[k], axes[k]));
         if (!(len > 1e-12)) continue;
         const CD3 w = cmul(axes[k], 1.0 / len);
         double p_lo = 1e300, p_hi = -1e300, q_lo = 1e300, q_hi = -1e300;
-        for (int i = 0; i < 4; ++i) {
-            const double pw = cdot(P[i], w), qw = cdot(Q[i], w);
+        for (int i = 0; i < np; ++i) {
+            const double pw = cdot(P[i], w);
             p_lo = fmin(p_lo, pw);
             p_hi = fmax(p_hi, pw);
+        }
+        for (int i = 0; i < 4; ++i) {
+            const double qw = cdot(Q[i], w);
             q_lo = fmin(q_lo, qw);
             q_hi = fmax(q_hi, qw);
         }
         if (q_lo > p_hi + reach || q_hi < p_lo - reach) return false;
     }
     return true;
+}
+
+// Cheaper, weaker form of the same bound for a pre-test: origins within `r` of a centre, the quad
+// within that distance plus its own radius (together R) of its centre, D = quad centre - source centre.
+// A hit point y = p + t*d needs (y-p).a >= 0, and (y-p) differs from D by at most R, so
+// D.a < -R - margin rules a hit out; and |(y-p) orthogonal to a| <= ((y-p).a)*tan(theta) <=
+// (D.a + R)*tan(theta), so |D orthogonal to a| - R > (D.a + R)*tan(theta) + margin rules it out.
+QMC_HD bool sphere_may_hit(CD3 D, double R, CD3 a, double tan_theta, double margin) {
+    const double da = cdot(D, a);
+    if (da < -R - margin) return false;
+    const CD3 perp = csub(D, cmul(a, da));
+    const double s = (da + R) * tan_theta + R + margin;  // > 0 here
+    return cdot(perp, perp) <= s * s;
 }
 
 QMC_HD float abs_divide(float den, float num) {

@@ -155,31 +155,61 @@ __device__ __forceinline__ FastDiv tile_divisor(const DevHelio &H, const FacetTi
     return tl.ltw == 3 ? H.fd_ntx8 : H.fd_ntx4;
 }
 
-// One thread's share of the traversal-culling scan of flat heliostat H (frame fr) against the quads
-// q = first, first + step, ... of a grid: sets may_shadow / may_block when some quad other than H's
-// own can meet its shadow cone (tan_sun around the sun axis) or blocking cone (tan_block around the
-// ideal reflection), per cone_may_hit. The caller ORs the flags over its threads; a traversal whose
-// flag stays clear can only miss.
-__device__ __forceinline__ void cull_scan(const DevHelio &H, const DevFrame &f, const DevFrame &sunf,
-                                          const float4 *quads, int n_sub, double tan_sun, double tan_block,
-                                          double margin, int first, int step, bool &may_shadow, bool &may_block) {
-    if (may_shadow && may_block) return;
+// Culling shape of a flat heliostat with normal frame f: its micro-facet origins lie in the rectangle
+// of half-extents size.x/2, size.z/2 at height size.y/2 above the position, and every facet normal is
+// the heliostat normal.
+__host__ __device__ inline DevCullShape flat_cull_shape(const DevHelio &H, const DevFrame &f) {
     const CD3 u = f.identity ? CD3{1, 0, 0} : CD3{f.ux, f.uy, f.uz};
     const CD3 n = f.identity ? CD3{0, 1, 0} : CD3{f.nx, f.ny, f.nz};
     const CD3 v = f.identity ? CD3{0, 0, 1} : CD3{f.vx, f.vy, f.vz};
     const CD3 c = cadd(CD3{H.pos[0], H.pos[1], H.pos[2]}, cmul(n, 0.5 * H.size[1]));
     const CD3 hu = cmul(u, 0.5 * H.size[0]), hv = cmul(v, 0.5 * H.size[2]);
+    DevCullShape s;
+    s.c[0] = c.x; s.c[1] = c.y; s.c[2] = c.z;
+    s.hu[0] = hu.x; s.hu[1] = hu.y; s.hu[2] = hu.z;
+    s.hn[0] = 0.0; s.hn[1] = 0.0; s.hn[2] = 0.0;
+    s.hv[0] = hv.x; s.hv[1] = hv.y; s.hv[2] = hv.z;
+    s.n[0] = n.x; s.n[1] = n.y; s.n[2] = n.z;
+    s.cant = 0.0;
+    return s;
+}
+
+// One thread's share of the traversal-culling scan of a heliostat (culling shape S, own quads
+// [own_lo, own_hi)) against the quads q = first, first + step, ... of a grid: sets may_shadow /
+// may_block when some other quad can meet its shadow cone (tan_sun around the sun axis, the n axis of
+// sunf) or its blocking cone (tan_block around the sun axis reflected in the heliostat normal), per
+// cone_may_hit. The caller ORs the flags over its threads; a traversal whose flag stays clear can
+// only miss.
+__device__ __forceinline__ void cull_scan(const DevCullShape &S, const DevFrame &sunf, const float4 *quads,
+                                          int n_sub, int own_lo, int own_hi, double tan_sun, double tan_block,
+                                          double margin, int first, int step, bool &may_shadow, bool &may_block) {
+    if (may_shadow && may_block) return;
+    const CD3 c{S.c[0], S.c[1], S.c[2]}, hu{S.hu[0], S.hu[1], S.hu[2]}, hn{S.hn[0], S.hn[1], S.hn[2]},
+        hv{S.hv[0], S.hv[1], S.hv[2]}, n{S.n[0], S.n[1], S.n[2]};
     const CD3 ash = sunf.identity ? CD3{0, 1, 0} : CD3{sunf.nx, sunf.ny, sunf.nz};
     CD3 r0 = cadd(cmul(ash, -1.0), cmul(n, 2.0 * cdot(ash, n)));  // reflect(-sun axis, n)
     r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
+    const double rs = sqrt(cdot(hu, hu) + cdot(hn, hn) + cdot(hv, hv));  // source box radius
     for (int q = first; q < n_sub; q += step) {
-        if (q >= H.sub_begin && q < H.sub_begin + H.sub_count) continue;
+        if (q >= own_lo && q < own_hi) continue;
         const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
         const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
-        if (!may_shadow && cone_may_hit(c, hu, hv, ash, tan_sun, A, E1, E2, margin)) may_shadow = true;
-        if (!may_block && cone_may_hit(c, hu, hv, r0, tan_block, A, E1, E2, margin)) may_block = true;
+        // bounding spheres first (most quads of a field are far from both cones), then the full test
+        const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
+        const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
+        const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
+        if (!may_shadow && sphere_may_hit(D, R, ash, tan_sun, margin) &&
+            cone_may_hit(c, hu, hn, hv, ash, tan_sun, A, E1, E2, margin))
+            may_shadow = true;
+        if (!may_block && sphere_may_hit(D, R, r0, tan_block, margin) &&
+            cone_may_hit(c, hu, hn, hv, r0, tan_block, A, E1, E2, margin))
+            may_block = true;
+        if (may_shadow && may_block) break;
     }
 }
+
+// Half-angles (radians) at or above which a cone is not culled: the bound needs theta < 90 deg.
+constexpr double kCullMaxAngle = 80.0 * 3.14159265358979323846 / 180.0;
 
 // Margin (metres) of the culling bound: far above the float rounding of traced origins, directions
 // and the intersection test.
