@@ -176,7 +176,17 @@ typedef struct qmc_trace_stats {
 typedef struct qmc_pool_params {
     uint64_t seed_sunshape;      /* scrambles the Sobol (theta, phi) pairs */
     uint64_t seed_perturbation;  /* counter-based Gaussian slope errors     */
+    int32_t sunshape_model;      /* QMC_SUNSHAPE_*; 0 = the reference's sampler */
+    int32_t reserved_;           /* 0 */
 } qmc_pool_params;
+
+/* Sun-shape samplers. REFERENCE restates samplelights::linear_interpolation
+ * (Scene/SceneProcessor.cu:61-81): core angles land in [0, 4.65] mrad, circumsolar ones in
+ * [9.3, 10.7] mrad, with nothing between. BUIE is the physical variant: theta drawn from Buie's
+ * profile (core cos(0.326t)/cos(0.308t) up to 4.65 mrad, circumsolar e^k t^g up to 43.6 mrad,
+ * k and g from the CSR as the reference computes them) weighted by sin(theta) over the sky. */
+#define QMC_SUNSHAPE_REFERENCE 0
+#define QMC_SUNSHAPE_BUIE      1
 
 /* ------------------------------------------------------------------------------------------
  * Errors
@@ -263,6 +273,12 @@ qmc_status qmc_pool_get(qmc_ctx *ctx, float *dirs, float *perts, int32_t P);
  * samplelights::parameters_generate, Scene/SceneProcessor.cu:23-44. */
 qmc_status qmc_sunshape_table(const qmc_scene *scene, float *cdf_xy /* 2*(n+1) */,
                               float *slopes /* n */, float *abc_i /* A,B,C,I,gamma */);
+/* Table of the physical Buie sampler (QMC_SUNSHAPE_BUIE) of the current configuration: bin edges
+ * theta[n+1] (mrad), cumulative probability cdf[n+1] (cdf[0] = 0, cdf[n] = 1) and the unnormalised
+ * sky density phi(t)*sin(t) at each bin's two ends (dens_lo[n], dens_hi[n]). Null arrays are not
+ * written; *n receives the bin count. No counterpart in the reference. */
+qmc_status qmc_sunshape_table_buie(const qmc_scene *scene, double *theta, double *cdf, double *dens_lo,
+                                   double *dens_hi, int32_t *n);
 /* Transform (theta-uniforms, phi-uniforms) and (gx, gz) Gaussians into pools on the device:
  * samplelights::linear_interpolation / map_angle2xyz / map_permuation,
  * Scene/SceneProcessor.cu:61-103. Inputs are P floats each. */

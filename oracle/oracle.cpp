@@ -40,6 +40,9 @@
 //                                                   -> energy conservation closed form
 //   trimmed_mean + heaps              Tool/Smoother/ImageSmoother.cu:13-64; heap.cuh:16-225
 //                                                   -> sort-based textbook trimmed mean (k <= 1)
+//   physical Buie sampler (QMC_SUNSHAPE_BUIE, no reference counterpart) -> KS test of sampled angles
+//                                     against quadrature of Buie's profile; the table itself against
+//                                     independent quadrature
 //   Sobol / counter hash (this build's generators, no reference counterpart; the reference used
 //   cuRAND XORWOW seeded by time(NULL)) -> Sobol net property, Gaussian moments
 //
@@ -797,12 +800,42 @@ extern "C" void orc_sunshape_table(const qmc_config *cfg, float *cdf_xy, float *
     }
 }
 
-extern "C" void orc_pool_from_uniforms(const qmc_config *cfg, const float *ut, const float *up, const float *gx,
-                                       const float *gz, int P, float *dirs, float *perts) {
+// Physical Buie sampler (no reference counterpart): theta [mrad] with cumulative probability u, from
+// a table of bins [theta_i, theta_i+1] with masses cdf_i+1 - cdf_i and the density linear between the
+// bin's end values lo_i, hi_i. The bin is the last one whose cumulative start does not exceed u (a
+// scan here); inside it the quadratic cumulative lo*s + (hi-lo)*s^2/(2h) = w*(lo+hi)*h/2 is solved as
+// s = 2m / (lo + sqrt(lo^2 + 2(hi-lo)m/h)). Pinned by the KS test of sampled angles against the
+// quadrature of Buie's profile (tests).
+double table_theta(const double *theta, const double *cdf, const double *lo, const double *hi, int n, double u) {
+    int a = 0;
+    while (a + 1 < n && cdf[a + 1] <= u) ++a;
+    double h = theta[a + 1] - theta[a], mass = cdf[a + 1] - cdf[a];
+    double w = 0.0;
+    if (mass > 0.0) w = std::min(std::max((u - cdf[a]) / mass, 0.0), 1.0);
+    double m = w * (lo[a] + hi[a]) * 0.5 * h;
+    double den = lo[a] + std::sqrt(std::max(lo[a] * lo[a] + 2.0 * (hi[a] - lo[a]) / h * m, 0.0));
+    double s = den > 0.0 ? 2.0 * m / den : 0.0;
+    return theta[a] + std::min(std::max(s, 0.0), h);
+}
+
+extern "C" void orc_table_theta(const double *theta, const double *cdf, const double *lo, const double *hi, int n,
+                                const double *u, int count, double *out) {
+    for (int i = 0; i < count; ++i) out[i] = table_theta(theta, cdf, lo, hi, n, u[i]);
+}
+
+struct PhysTable {
+    const double *theta, *cdf, *lo, *hi;
+    int n;
+};
+
+void pools_from_uniforms(const qmc_config *cfg, const PhysTable *phys, const float *ut, const float *up,
+                         const float *gx, const float *gz, int P, float *dirs, float *perts) {
     Buie b = buie_table(*cfg);
     for (int i = 0; i < P; ++i) {
-        // samplelights::linear_interpolation + map_angle2xyz
-        float theta = buie_theta(b, ut[i]);
+        // samplelights::linear_interpolation (or the physical table) + map_angle2xyz
+        float theta = phys ? (float)(table_theta(phys->theta, phys->cdf, phys->lo, phys->hi, phys->n, (double)ut[i]) *
+                                     1e-3)
+                           : buie_theta(b, ut[i]);
         float phi = up[i] * 2 * 3.141592654f;
         float st = sinf_d(theta);
         store3(dirs + 3 * i, mk(st * cosf_d(phi), cosf_d(theta), st * sinf_d(phi)));
@@ -811,10 +844,43 @@ extern "C" void orc_pool_from_uniforms(const qmc_config *cfg, const float *ut, c
     }
 }
 
+extern "C" void orc_pool_from_uniforms(const qmc_config *cfg, const float *ut, const float *up, const float *gx,
+                                       const float *gz, int P, float *dirs, float *perts) {
+    pools_from_uniforms(cfg, nullptr, ut, up, gx, gz, P, dirs, perts);
+}
+
+void seeded_uniforms(const qmc_config *cfg, uint64_t seed_sun, uint64_t seed_pert, std::vector<float> &ut,
+                     std::vector<float> &up, std::vector<float> &gx, std::vector<float> &gz);
+
+extern "C" void orc_pool_generate_buie(const qmc_config *cfg, uint64_t seed_sun, uint64_t seed_pert,
+                                       const double *theta, const double *cdf, const double *lo, const double *hi,
+                                       int n, float *dirs, float *perts) {
+    std::vector<float> ut, up, gx, gz;
+    seeded_uniforms(cfg, seed_sun, seed_pert, ut, up, gx, gz);
+    PhysTable t{theta, cdf, lo, hi, n};
+    pools_from_uniforms(cfg, &t, ut.data(), up.data(), gx.data(), gz.data(), (int)ut.size(), dirs, perts);
+}
+
 extern "C" void orc_pool_generate(const qmc_config *cfg, uint64_t seed_sun, uint64_t seed_pert, float *dirs,
                                   float *perts, float *ut_out, float *up_out, float *gx_out, float *gz_out) {
+    std::vector<float> ut, up, gx, gz;
+    seeded_uniforms(cfg, seed_sun, seed_pert, ut, up, gx, gz);
+    const int P = (int)ut.size();
+    orc_pool_from_uniforms(cfg, ut.data(), up.data(), gx.data(), gz.data(), P, dirs, perts);
+    if (ut_out) std::copy(ut.begin(), ut.end(), ut_out);
+    if (up_out) std::copy(up.begin(), up.end(), up_out);
+    if (gx_out) std::copy(gx.begin(), gx.end(), gx_out);
+    if (gz_out) std::copy(gz.begin(), gz.end(), gz_out);
+}
+
+// Scrambled Sobol (theta, phi) uniforms and counter-hash Box-Muller slope errors of a seeded pool.
+void seeded_uniforms(const qmc_config *cfg, uint64_t seed_sun, uint64_t seed_pert, std::vector<float> &ut,
+                     std::vector<float> &up, std::vector<float> &gx, std::vector<float> &gz) {
     const int P = cfg->num_sunshape_groups * cfg->num_sunshape_lights_per_group;
-    std::vector<float> ut(P), up(P), gx(P), gz(P);
+    ut.assign(P, 0.0f);
+    up.assign(P, 0.0f);
+    gx.assign(P, 0.0f);
+    gz.assign(P, 0.0f);
     uint32_t v0[32], v1[32];
     sobol_dirs(v0, v1);
     uint32_t m0 = (uint32_t)(splitmix(seed_sun) >> 32);
@@ -831,11 +897,6 @@ extern "C" void orc_pool_generate(const qmc_config *cfg, uint64_t seed_sun, uint
         gx[i] = (float)((double)cfg->helio_disturb_std * rad * std::cos(ang));
         gz[i] = (float)((double)cfg->helio_disturb_std * rad * std::sin(ang));
     }
-    orc_pool_from_uniforms(cfg, ut.data(), up.data(), gx.data(), gz.data(), P, dirs, perts);
-    if (ut_out) std::copy(ut.begin(), ut.end(), ut_out);
-    if (up_out) std::copy(up.begin(), up.end(), up_out);
-    if (gx_out) std::copy(gx.begin(), gx.end(), gx_out);
-    if (gz_out) std::copy(gz.begin(), gz.end(), gz_out);
 }
 
 extern "C" void orc_offsets_generate(const orc_scene *s, uint64_t seed, int P, int h, int32_t *out) {

@@ -44,6 +44,15 @@ void orc_pool_from_uniforms(const qmc_config *cfg, const float *ut, const float 
 /* Seeded pool generation (scrambled Sobol + counter-hash Box-Muller). */
 void orc_pool_generate(const qmc_config *cfg, uint64_t seed_sunshape, uint64_t seed_perturbation, float *dirs,
                        float *perts, float *ut, float *up, float *gx, float *gz);
+/* Physical Buie pools (QMC_SUNSHAPE_BUIE): the same seeded uniforms as orc_pool_generate, theta drawn
+ * from a tabulated sky density (bin edges theta[n+1] in mrad, cdf[n+1], end densities lo[n], hi[n]);
+ * the table is the product's host table, checked separately against independent quadrature. */
+void orc_pool_generate_buie(const qmc_config *cfg, uint64_t seed_sunshape, uint64_t seed_perturbation,
+                            const double *theta, const double *cdf, const double *lo, const double *hi, int n,
+                            float *dirs, float *perts);
+/* The physical sampler's inverse CDF alone: out[i] = theta [mrad] at cumulative probability u[i]. */
+void orc_table_theta(const double *theta, const double *cdf, const double *lo, const double *hi, int n,
+                     const double *u, int count, double *out);
 /* Seeded group offsets of heliostat h: F+1 entries. */
 void orc_offsets_generate(const orc_scene *s, uint64_t seed, int P, int h, int32_t *out);
 

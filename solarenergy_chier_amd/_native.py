@@ -88,7 +88,12 @@ class TraceStats(C.Structure):
 
 
 class PoolParams(C.Structure):
-    _fields_ = [("seed_sunshape", C.c_uint64), ("seed_perturbation", C.c_uint64)]
+    _fields_ = [("seed_sunshape", C.c_uint64), ("seed_perturbation", C.c_uint64), ("sunshape_model", C.c_int32),
+                ("reserved_", C.c_int32)]
+
+
+SUNSHAPE_REFERENCE = 0
+SUNSHAPE_BUIE = 1
 
 
 class Instance(C.Structure):
@@ -113,6 +118,7 @@ _P = C.c_void_p
 _I = C.c_int32
 _F = C.c_float
 _FP = C.POINTER(C.c_float)
+_DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int32)
 
 # name -> (argtypes); every function returns qmc_status (int) unless listed in _RESTYPES
@@ -152,6 +158,7 @@ SIGNATURES = {
     "qmc_pool_set": [_P, _FP, _FP, _I, _I],
     "qmc_pool_get": [_P, _FP, _FP, _I],
     "qmc_sunshape_table": [_P, _FP, _FP, _FP],
+    "qmc_sunshape_table_buie": [_P, _DP, _DP, _DP, _DP, _IP],
     "qmc_pool_from_uniforms": [_P, _FP, _FP, _FP, _FP],
     "qmc_offsets_generate": [_P, C.c_uint64],
     "qmc_offsets_set": [_P, _I, _IP, _I],

@@ -57,6 +57,7 @@ struct qmc_ctx {
     qmc::DevFrame sun{};
     float2 *d_cdf = nullptr;
     float *d_slope = nullptr;
+    double *d_buie_phys = nullptr;  // physical sun-shape table: theta, cdf (n+1 each), lo, hi (n each)
     qmc::BuieArgs buie{};
     // batched traces (sun x aim instances)
     float *d_aims = nullptr;       // aim points, n_aim_sets * n_heliostats * 3

@@ -82,6 +82,9 @@ struct BuieArgs {
     const float *slope;  // n
     int n;
     float A, B, C, I, gamma;
+    // physical sampler (model 1): bin edges [mrad], cumulative probability, end densities
+    int model, n_phys;
+    const double *ph_theta, *ph_cdf, *ph_lo, *ph_hi;
 };
 
 struct StatsAcc {

@@ -203,7 +203,9 @@ __device__ __forceinline__ float buie_theta(const BuieArgs &b, float U) {
 }
 
 __device__ __forceinline__ float4 sunshape_dir(const BuieArgs &b, float u_theta, float u_phi) {
-    float theta = buie_theta(b, u_theta);
+    float theta = b.model == 0 ? buie_theta(b, u_theta)
+                               : (float)(table_inverse_cdf(b.ph_theta, b.ph_cdf, b.ph_lo, b.ph_hi, b.n_phys,
+                                                           (double)u_theta) * 1e-3);
     float phi = u_phi * 2 * kMathPi;
     // global_func::angle2xyz (Util/global_function.cuh:211-216)
     float st = sin_r(theta);
@@ -444,6 +446,7 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_batch_work);
     dev_free(c->d_cull);
     dev_free(c->d_cull_shapes);
+    dev_free(c->d_buie_phys);
     if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
     c->d_batch_inst = nullptr;
     qmc_adjoint_release(c);
@@ -833,13 +836,37 @@ extern "C" qmc_status qmc_pool_generate(qmc_ctx *c, const qmc_pool_params *p) {
     const int L = c->cfg.num_sunshape_lights_per_group;
     const int P = c->cfg.num_sunshape_groups * L;
     if (!c->d_cdf) return fail(QMC_E_INVALID_ARG, "qmc_pool_generate: sun-shape table missing (csr, groups)");
+    if (p->sunshape_model != QMC_SUNSHAPE_REFERENCE && p->sunshape_model != QMC_SUNSHAPE_BUIE)
+        return fail(QMC_E_INVALID_ARG, "qmc_pool_generate: unknown sunshape_model " + std::to_string(p->sunshape_model));
     if ((st = ensure_pool(c, P, L))) return st;
+    BuieArgs buie = c->buie;
+    buie.model = p->sunshape_model;
+    if (buie.model == QMC_SUNSHAPE_BUIE) {
+        BuieTable t;
+        buie_physical_table(c->cfg, t);
+        const int nb = (int)t.dens_lo.size();
+        std::vector<double> packed;
+        packed.reserve(4 * (size_t)nb + 2);
+        packed.insert(packed.end(), t.theta.begin(), t.theta.end());
+        packed.insert(packed.end(), t.cdf.begin(), t.cdf.end());
+        packed.insert(packed.end(), t.dens_lo.begin(), t.dens_lo.end());
+        packed.insert(packed.end(), t.dens_hi.begin(), t.dens_hi.end());
+        if ((st = dev_alloc(c->d_buie_phys, packed.size()))) return st;
+        HIP_TRY(hipMemcpyAsync(c->d_buie_phys, packed.data(), packed.size() * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));  // pageable source
+        buie.n_phys = nb;
+        buie.ph_theta = c->d_buie_phys;
+        buie.ph_cdf = c->d_buie_phys + nb + 1;
+        buie.ph_lo = c->d_buie_phys + 2 * (nb + 1);
+        buie.ph_hi = c->d_buie_phys + 2 * (nb + 1) + nb;
+    }
     SobolArgs sa;
     sobol_directions(sa.v0, sa.v1);
     sa.mask0 = (uint32_t)(mix64(p->seed_sunshape) >> 32);
     sa.mask1 = (uint32_t)(mix64(p->seed_sunshape ^ 0xA5A5A5A5A5A5A5A5ull) >> 32);
     const int blocks = (P + 255) / 256;
-    hipLaunchKernelGGL(k_pool_sunshape, dim3(blocks), dim3(256), 0, c->stream, c->d_pool, P, sa, c->buie);
+    hipLaunchKernelGGL(k_pool_sunshape, dim3(blocks), dim3(256), 0, c->stream, c->d_pool, P, sa, buie);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_pool_perturbation, dim3(blocks), dim3(256), 0, c->stream, c->d_pert, P,
                        mix64(p->seed_perturbation), c->cfg.helio_disturb_std);

@@ -246,6 +246,28 @@ QMC_HD bool sphere_may_hit(CD3 D, double R, CD3 a, double tan_theta, double marg
     return cdot(perp, perp) <= s * s;
 }
 
+// Inverse CDF of a tabulated density, u in [0, 1] -> angle (table units). Bin i spans
+// [theta[i], theta[i+1]] with cumulative probability cdf[i] .. cdf[i+1] and end densities lo[i],
+// hi[i]; inside a bin the density is taken linear between them and scaled to the bin's mass, so the
+// cumulative is a quadratic in the offset s: s = 2m / (lo + sqrt(lo^2 + 2 (hi - lo) m / h)), the
+// cancellation-free root, with m the target mass in the bin's own density units.
+QMC_HD double table_inverse_cdf(const double *theta, const double *cdf, const double *lo, const double *hi, int n,
+                                double u) {
+    int a = 0, b = n;  // cdf[a] <= u; b == n or cdf[b] > u
+    while (b - a > 1) {
+        const int mid = (a + b) >> 1;
+        if (cdf[mid] <= u) a = mid;
+        else b = mid;
+    }
+    const double h = theta[a + 1] - theta[a], mass = cdf[a + 1] - cdf[a];
+    const double w = mass > 0.0 ? fmin(fmax((u - cdf[a]) / mass, 0.0), 1.0) : 0.0;
+    const double p0 = lo[a], p1 = hi[a];
+    const double m = w * (p0 + p1) * 0.5 * h;
+    const double den = p0 + sqrt(fmax(p0 * p0 + 2.0 * (p1 - p0) / h * m, 0.0));
+    const double s = den > 0.0 ? 2.0 * m / den : 0.0;
+    return theta[a] + fmin(fmax(s, 0.0), h);
+}
+
 QMC_HD float abs_divide(float den, float num) {
     if (num <= kEps && num >= -kEps) return kIntMaxF;
     return fabsf(den / num);

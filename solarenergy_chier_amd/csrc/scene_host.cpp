@@ -1236,7 +1236,87 @@ void sunshape_params(const qmc_config &c, std::vector<float> &cdf_xy, std::vecto
     abcig[3] = I;
     abcig[4] = gamma;
 }
+
+// Nodes and weights of n-point Gauss-Legendre quadrature on [-1, 1]: Newton's method on the Legendre
+// polynomial P_n from the usual starting guesses cos(pi (i + 3/4) / (n + 1/2)).
+static void gauss_legendre(int n, double *x, double *w) {
+    for (int i = 0; i < n; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = 0.0;  // P_j(z), P_{j-1}(z)
+            for (int j = 1; j <= n; ++j) {
+                const double p2 = p1;
+                p1 = p0;
+                p0 = ((2.0 * j - 1.0) * z * p1 - (j - 1.0) * p2) / j;
+            }
+            dp = n * (z * p0 - p1) / (z * z - 1.0);
+            const double dz = p0 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-15) break;
+        }
+        x[i] = z;
+        w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+// Physical Buie sun shape (D. Buie, A. G. Monger, C. J. Dey, "Sunshape distributions for terrestrial
+// solar simulations", Solar Energy 74 (2003) 113-122), theta in mrad: phi = cos(0.326 t)/cos(0.308 t)
+// up to 4.65, e^k t^g from 4.65 to 43.6, k = 0.9 ln(13.5 x) x^-0.3, g = 2.2 ln(0.52 x) x^0.43 - 0.1
+// (x = CSR, the reference's own formulas, Scene/SceneProcessor.cu:203-210, here in double). Angles are
+// drawn over the sky with density phi(t) sin(t): 1024 equal bins over the core, 1024 geometric bins
+// over the circumsolar region, each bin's mass by 8-point Gauss-Legendre (the profile is smooth inside
+// each region), and the density at both bin ends for the in-bin inversion (table_inverse_cdf).
+void buie_physical_table(const qmc_config &c, BuieTable &t) {
+    constexpr int kCore = 1024, kTail = 1024, kGauss = 8;
+    constexpr double kCoreEnd = 4.65, kTailEnd = 43.6;
+    const double x = c.csr;
+    const double kap = 0.9 * std::log(13.5 * x) * std::pow(x, -0.3);
+    const double gam = 2.2 * std::log(0.52 * x) * std::pow(x, 0.43) - 0.1;
+    auto core = [](double th) { return std::cos(0.326 * th) / std::cos(0.308 * th) * std::sin(th * 1e-3); };
+    auto tail = [&](double th) { return std::exp(kap) * std::pow(th, gam) * std::sin(th * 1e-3); };
+    double gx[kGauss], gw[kGauss];
+    gauss_legendre(kGauss, gx, gw);
+    const int n = kCore + kTail;
+    t.theta.assign(n + 1, 0.0);
+    t.cdf.assign(n + 1, 0.0);
+    t.dens_lo.assign(n, 0.0);
+    t.dens_hi.assign(n, 0.0);
+    for (int i = 0; i <= kCore; ++i) t.theta[i] = kCoreEnd * i / kCore;
+    for (int i = 1; i <= kTail; ++i) t.theta[kCore + i] = kCoreEnd * std::pow(kTailEnd / kCoreEnd, (double)i / kTail);
+    t.theta[n] = kTailEnd;
+    double total = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double a = t.theta[i], b = t.theta[i + 1], half = 0.5 * (b - a), mid = 0.5 * (a + b);
+        const bool in_core = i < kCore;
+        double mass = 0.0;
+        for (int k = 0; k < kGauss; ++k) {
+            const double th = mid + half * gx[k];
+            mass += gw[k] * (in_core ? core(th) : tail(th));
+        }
+        total += mass * half;
+        t.cdf[i + 1] = total;
+        t.dens_lo[i] = in_core ? core(a) : tail(a);
+        t.dens_hi[i] = in_core ? core(b) : tail(b);
+    }
+    for (int i = 1; i <= n; ++i) t.cdf[i] /= total;
+    t.cdf[n] = 1.0;
+}
 }  // namespace qmc
+
+extern "C" qmc_status qmc_sunshape_table_buie(const qmc_scene *s, double *theta, double *cdf, double *dens_lo,
+                                              double *dens_hi, int32_t *n) {
+    if (!s) return fail(QMC_E_INVALID_ARG, "qmc_sunshape_table_buie: null scene");
+    if (!(s->cfg.csr > 0.0f)) return fail(QMC_E_INVALID_ARG, "qmc_sunshape_table_buie: need csr > 0");
+    BuieTable t;
+    buie_physical_table(s->cfg, t);
+    const size_t nb = t.dens_lo.size();
+    if (theta) std::memcpy(theta, t.theta.data(), (nb + 1) * sizeof(double));
+    if (cdf) std::memcpy(cdf, t.cdf.data(), (nb + 1) * sizeof(double));
+    if (dens_lo) std::memcpy(dens_lo, t.dens_lo.data(), nb * sizeof(double));
+    if (dens_hi) std::memcpy(dens_hi, t.dens_hi.data(), nb * sizeof(double));
+    if (n) *n = (int32_t)nb;
+    return QMC_OK;
+}
 
 extern "C" qmc_status qmc_sunshape_table(const qmc_scene *s, float *cdf_xy, float *slopes, float *abc_i) {
     if (!s) return fail(QMC_E_INVALID_ARG, "qmc_sunshape_table: null scene");

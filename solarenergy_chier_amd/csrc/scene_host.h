@@ -81,4 +81,9 @@ V3 heliostat_aim(const qmc_scene &s, int h, const float off[2]);
 // model's parameterisation); for cylinder receivers the float aim point.
 void aim_points_f64(const qmc_scene &s, std::vector<double> &out);
 void sunshape_params(const qmc_config &c, std::vector<float> &cdf_xy, std::vector<float> &slopes, float abcig[5]);
+// Table of the physical Buie sampler (bin edges in mrad, cumulative probability, end densities).
+struct BuieTable {
+    std::vector<double> theta, cdf, dens_lo, dens_hi;
+};
+void buie_physical_table(const qmc_config &c, BuieTable &t);
 }  // namespace qmc

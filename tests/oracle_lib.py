@@ -43,6 +43,9 @@ def lib():
             "orc_sunshape_table": (None, [C.POINTER(N.Config), _FP, _FP, _FP]),
             "orc_pool_from_uniforms": (None, [C.POINTER(N.Config), _FP, _FP, _FP, _FP, C.c_int, _FP, _FP]),
             "orc_pool_generate": (None, [C.POINTER(N.Config), C.c_uint64, C.c_uint64, _FP, _FP, _FP, _FP, _FP, _FP]),
+            "orc_table_theta": (None, [_DP, _DP, _DP, _DP, C.c_int, _DP, C.c_int, _DP]),
+            "orc_pool_generate_buie": (None, [C.POINTER(N.Config), C.c_uint64, C.c_uint64, _DP, _DP, _DP, _DP,
+                                              C.c_int, _FP, _FP]),
             "orc_offsets_generate": (None, [_P, C.c_uint64, C.c_int, C.c_int, _IP]),
             "orc_trace_heliostat": (C.c_int, [_P, C.c_int, _FP, _FP, C.c_int, C.c_int, _IP, _DP,
                                               C.POINTER(N.RayRecord), C.POINTER(N.TraceStats)]),
@@ -215,6 +218,27 @@ def pool_generate(config, seed_sun, seed_pert):
     lib().orc_pool_generate(C.byref(config), C.c_uint64(seed_sun), C.c_uint64(seed_pert), fp(d), fp(p), fp(ut),
                             fp(up), fp(gx), fp(gz))
     return d, p, (ut, up, gx, gz)
+
+
+def pool_generate_buie(config, seed_sun, seed_pert, table):
+    """Physical Buie pools from the table (theta, cdf, lo, hi) of qmc_sunshape_table_buie."""
+    P = config.num_sunshape_groups * config.num_sunshape_lights_per_group
+    d = np.zeros((P, 3), np.float32)
+    p = np.zeros((P, 3), np.float32)
+    th, cdf, lo, hi = (np.ascontiguousarray(a, np.float64) for a in table)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    lib().orc_pool_generate_buie(C.byref(config), C.c_uint64(seed_sun), C.c_uint64(seed_pert), dp(th), dp(cdf),
+                                 dp(lo), dp(hi), len(lo), fp(d), fp(p))
+    return d, p
+
+
+def table_theta(table, u):
+    th, cdf, lo, hi = (np.ascontiguousarray(a, np.float64) for a in table)
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.zeros(len(u))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    lib().orc_table_theta(dp(th), dp(cdf), dp(lo), dp(hi), len(lo), dp(u), len(u), dp(out))
+    return out
 
 
 def pool_from_uniforms(config, ut, up, gx, gz):

@@ -3,6 +3,8 @@
 Bar: per-ray status and hit index (sub-heliostat id or pixel) bit-exact; deposited energies
 bit-exact; flux maps within 1e-5 relative L1 (float atomics reorder the sums; the oracle sums in
 double)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -233,6 +235,26 @@ def test_pool_and_offsets_generation_match_oracle(device):
     for h in (0, 24):
         F = scene.heliostat_info(h).n_micro
         assert np.array_equal(device.offsets_get(h, F), orc.offsets(3, P, h))
+
+
+def test_physical_buie_pool_matches_oracle(device):
+    """QMC_SUNSHAPE_BUIE pools on the device equal the oracle's inversion of the same table (the table
+    itself is pinned by quadrature on the CPU); an unknown model is rejected."""
+    wl = workloads.small_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    device.upload(scene)
+    device.pool_generate(2, 4, sunshape_model=N.SUNSHAPE_BUIE)
+    cfg = scene.config
+    P = cfg.num_sunshape_groups * cfg.num_sunshape_lights_per_group
+    d, p = device.pool_get(P)
+    cd, cp = O.pool_generate_buie(cfg, 2, 4, scene.sunshape_table_buie())
+    ulp = np.abs(d.view(np.int32).astype(np.int64) - cd.view(np.int32).astype(np.int64))
+    print("buie pool max ulp", int(ulp.max()), "differing", int((ulp > 0).sum()))
+    assert ulp.max() <= 1
+    assert np.array_equal(p.view(np.uint32), cp.view(np.uint32))
+    bad = N.PoolParams(2, 4, 5, 0)
+    assert N.status_of("qmc_pool_generate", device.handle, C.byref(bad))[0] == N.QMC_E_INVALID_ARG
+    device.pool_generate(2, 4)  # back to the reference sampler for later users of this device
 
 
 @pytest.mark.parametrize("shape,r,ratio", [((128, 128), 1, 0.15), ((256, 512), 5, 0.05), ((37, 53), 2, 0.1),
