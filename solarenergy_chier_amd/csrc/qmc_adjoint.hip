@@ -76,6 +76,7 @@ struct AdjArgs {
     float *flux;          // bilinear map (forward)
     double *flux64;       // bilinear map (masked forward)
     uint32_t *mask;       // visibility bits
+    const uint8_t *cull;  // per heliostat: bit 0 shadow, bit 1 blocking traversal needed (null: both)
     const float *G;       // dL/dflux (backward)
     double *g_n;          // per heliostat dL/dn (3 doubles), indexed by heliostat id
     double *g_m;          // dL/dm (3 doubles), m = -s
@@ -170,11 +171,13 @@ __global__ __launch_bounds__(256) void k_adj_forward(AdjArgs a) {
         RayStats st{0u, 0u, 0u};
         int hit;
         const int lo = H.sub_begin, hi = H.sub_begin + H.sub_count;
-        if (!dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st)) {
+        // traversals the cull kernel proved hit-free are skipped (same visibility)
+        const unsigned need = a.cull ? (unsigned)a.cull[h] : 3u;
+        if (!((need & 1u) && dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
             const float4 q = a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)];
             const V3 normal = normalize(fr_apply(H.fr, V3{q.x, q.y, q.z}));
             dir = normalize(reflect(-dir, normal));
-            if (!dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st)) {
+            if (!((need & 2u) && dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
                 float t, u, v;
                 if (receiver_hit<0>(a.recv, origin, dir, t, u, v)) {
                     visible = true;
@@ -526,7 +529,9 @@ extern "C" qmc_status qmc_adjoint_forward(qmc_ctx *c, int32_t grid, float *d_flu
     const RecvBuf &rb = c->recvs[c->grids[grid].receiver];
     HIP_TRY(hipMemsetAsync(d_flux, 0, sizeof(float) * (size_t)rb.w * rb.h, c->stream));
     HIP_TRY(hipMemsetAsync(s->d_mask, 0, sizeof(uint32_t) * s->mask_words, c->stream));
-    hipLaunchKernelGGL(k_adj_forward, adj_grid(c, s), dim3(256), 0, c->stream, adj_args(c, s, d_flux));
+    AdjArgs a = adj_args(c, s, d_flux);
+    if ((st = launch_cull(c, grid, s->d_list, (int)s->list.size(), &a.cull))) return st;
+    hipLaunchKernelGGL(k_adj_forward, adj_grid(c, s), dim3(256), 0, c->stream, a);
     HIP_TRY(hipGetLastError());
     s->mask_ready = true;
     return QMC_OK;

@@ -125,6 +125,9 @@ inline DevFrame to_dev(const Frame &f) {
 }
 
 void qmc_adjoint_release(qmc_ctx *c);  // qmc_adjoint.hip
+// Per-heliostat traversal-culling flags (bit 0 shadow, bit 1 blocking traversal needed) of the listed
+// heliostats of a grid, computed on the ctx stream (qmc_device.hip).
+qmc_status launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const uint8_t **flags);
 
 inline void free_scene(qmc_ctx *c) {
     for (auto &g : c->grids) {

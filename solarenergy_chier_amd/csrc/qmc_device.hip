@@ -1031,6 +1031,32 @@ extern "C" qmc_status qmc_facets_get(qmc_ctx *c, int32_t h, float *origins, floa
 // ---------------------------------------------------------------------------------------------
 // Trace
 // ---------------------------------------------------------------------------------------------
+// Traversal-culling flags (k_cull) of the listed heliostats of a grid, on the ctx stream, into the
+// ctx-owned per-heliostat array.
+qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const uint8_t **flags) {
+    const GridBuf &gb = c->grids[grid];
+    qmc_status st;
+    if (c->cull_cap < c->h_helios.size()) {
+        if ((st = dev_alloc(c->d_cull, c->h_helios.size()))) return st;
+        c->cull_cap = c->h_helios.size();
+    }
+    CullArgs ca;
+    ca.helios = c->d_helios;
+    ca.shapes = c->d_cull_shapes;
+    ca.list = d_list;
+    ca.quads = gb.quads;
+    ca.n_sub = gb.n_sub;
+    ca.sun = c->sun;
+    ca.tan_sun = c->cull_tan_sun;
+    ca.ang_block = c->cull_ang_block;
+    ca.margin = kCullMargin;
+    ca.flags = c->d_cull;
+    hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
+    HIP_TRY(hipGetLastError());
+    *flags = c->d_cull;
+    return QMC_OK;
+}
+
 namespace {
 
 qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
@@ -1062,25 +1088,8 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     a.blocks_per_helio = (int)bx;
     if (bx == 0 || n == 0) return QMC_OK;
     if (!stats) {
-        qmc_status st;
-        if (c->cull_cap < c->h_helios.size()) {
-            if ((st = dev_alloc(c->d_cull, c->h_helios.size()))) return st;
-            c->cull_cap = c->h_helios.size();
-        }
-        CullArgs ca;
-        ca.helios = c->d_helios;
-        ca.list = d_list;
-        ca.quads = gb.quads;
-        ca.n_sub = gb.n_sub;
-        ca.sun = c->sun;
-        ca.shapes = c->d_cull_shapes;
-        ca.tan_sun = c->cull_tan_sun;
-        ca.ang_block = c->cull_ang_block;
-        ca.margin = kCullMargin;
-        ca.flags = c->d_cull;
-        hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
-        HIP_TRY(hipGetLastError());
-        a.cull = c->d_cull;
+        const qmc_status st = launch_cull(c, grid, d_list, n, &a.cull);
+        if (st) return st;
     }
     dim3 g((unsigned)bx, (unsigned)n), b(256);
     const int rt = rb.r.type;
