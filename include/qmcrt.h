@@ -304,6 +304,11 @@ qmc_status qmc_facets_get(qmc_ctx *ctx, int32_t heliostat, float *origins, float
  * RectangleReceiverRectangleGridRayTracing.cu:45-57, CylinderReceiverRectangleGridRayTracing.cu:44-57.
  * Heliostats of several grids are traced grid by grid. Asynchronous. */
 qmc_status qmc_trace(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n);
+/* Traversal-culling flags the trace uses for the listed heliostats under the current pools and
+ * scene: bit 0 set = the shadow traversal can meet another quad, bit 1 set = the blocking traversal
+ * can; a clear bit means every ray's traversal is proven to miss and the trace skips it (results are
+ * unchanged). No counterpart in the reference (its DDA always runs). Synchronous. */
+qmc_status qmc_cull_flags(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n, uint8_t *flags);
 /* Same trace with one record per ray of one heliostat copied to host (F*L records). Synchronous. */
 qmc_status qmc_trace_records(qmc_ctx *ctx, int32_t heliostat, qmc_ray_record *records, int64_t n);
 /* Trace and count the algorithmic work (instrumented kernel; flux is accumulated too). Synchronous. */

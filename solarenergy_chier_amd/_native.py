@@ -165,6 +165,7 @@ SIGNATURES = {
     "qmc_offsets_get": [_P, _I, _IP, _I],
     "qmc_facets_get": [_P, _I, _FP, _FP, _I],
     "qmc_trace": [_P, _IP, _I],
+    "qmc_cull_flags": [_P, _IP, _I, C.POINTER(C.c_uint8)],
     "qmc_trace_records": [_P, _I, C.POINTER(RayRecord), C.c_int64],
     "qmc_trace_stats_run": [_P, _IP, _I, C.POINTER(TraceStats)],
     "qmc_flux_clear": [_P, _I],

@@ -76,7 +76,7 @@ struct AdjArgs {
     float *flux;          // bilinear map (forward)
     double *flux64;       // bilinear map (masked forward)
     uint32_t *mask;       // visibility bits
-    const uint8_t *cull;  // per heliostat: bit 0 shadow, bit 1 blocking traversal needed (null: both)
+    const CullList *cull; // per heliostat id: candidate quads of each traversal (null: no culling)
     const float *G;       // dL/dflux (backward)
     double *g_n;          // per heliostat dL/dn (3 doubles), indexed by heliostat id
     double *g_m;          // dL/dm (3 doubles), m = -s
@@ -171,13 +171,14 @@ __global__ __launch_bounds__(256) void k_adj_forward(AdjArgs a) {
         RayStats st{0u, 0u, 0u};
         int hit;
         const int lo = H.sub_begin, hi = H.sub_begin + H.sub_count;
-        // traversals the cull kernel proved hit-free are skipped (same visibility)
-        const unsigned need = a.cull ? (unsigned)a.cull[h] : 3u;
-        if (!((need & 1u) && dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
+        // traversals proved hit-free for this ray are skipped (same visibility)
+        if (!((!a.cull || traversal_needed(a.cull[h], 0, origin, dir, a.grid.quads)) &&
+              dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
             const float4 q = a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)];
             const V3 normal = normalize(fr_apply(H.fr, V3{q.x, q.y, q.z}));
             dir = normalize(reflect(-dir, normal));
-            if (!((need & 2u) && dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
+            if (!((!a.cull || traversal_needed(a.cull[h], 1, origin, dir, a.grid.quads)) &&
+                  dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
                 float t, u, v;
                 if (receiver_hit<0>(a.recv, origin, dir, t, u, v)) {
                     visible = true;

@@ -75,9 +75,9 @@ struct qmc_ctx {
     int num_cus = 0;
     size_t max_lds = 0;
     // traversal culling: tangents of the sample pools' cone half-angles (sun shape; sun shape plus
-    // twice the slope error), negative when culling does not apply, and a flag byte per heliostat
+    // twice the slope error), negative when culling does not apply, and the candidate lists per heliostat
     double cull_tan_sun = -1.0, cull_tan_block = -1.0, cull_ang_block = -1.0;
-    uint8_t *d_cull = nullptr;
+    qmc::CullList *d_cull = nullptr;
     qmc::DevCullShape *d_cull_shapes = nullptr;
     size_t cull_cap = 0;
     // adjoint
@@ -125,9 +125,9 @@ inline DevFrame to_dev(const Frame &f) {
 }
 
 void qmc_adjoint_release(qmc_ctx *c);  // qmc_adjoint.hip
-// Per-heliostat traversal-culling flags (bit 0 shadow, bit 1 blocking traversal needed) of the listed
-// heliostats of a grid, computed on the ctx stream (qmc_device.hip).
-qmc_status launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const uint8_t **flags);
+// Traversal-culling candidate lists (CullList, indexed by heliostat id) of the listed heliostats of a
+// grid, computed on the ctx stream (qmc_device.hip).
+qmc_status launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists);
 
 inline void free_scene(qmc_ctx *c) {
     for (auto &g : c->grids) {

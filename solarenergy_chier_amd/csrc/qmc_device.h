@@ -99,6 +99,16 @@ struct DevCullShape {
     double cant;
 };
 
+// Candidate quads of a heliostat's two traversals (index 0 shadow, 1 blocking), from the culling scan:
+// n[k] quads of the grid may meet the cone; 0 means the traversal can only miss. With n[k] <=
+// kCullList the candidates are listed (in no particular order) and a ray that hits none of them
+// cannot hit anything in its traversal; above that only the count is kept.
+constexpr int kCullList = 6;
+struct CullList {
+    int32_t n[2];
+    int32_t ids[2][kCullList];
+};
+
 struct CullArgs {
     const DevHelio *helios;
     const DevCullShape *shapes;
@@ -109,7 +119,7 @@ struct CullArgs {
     double tan_sun;       // cone of shadow rays (negative: no culling)
     double ang_block;     // half-angle of the blocking cone of a flat heliostat (negative: no culling)
     double margin;
-    uint8_t *flags;
+    CullList *lists;      // per heliostat id
 };
 
 struct TraceArgs {
@@ -125,7 +135,7 @@ struct TraceArgs {
     float *flux;
     qmc_ray_record *records;   // nullable (one heliostat per launch when set)
     StatsAcc *stats;           // nullable
-    const uint8_t *cull;       // per heliostat: bit 0 shadow traversal needed, bit 1 blocking traversal needed
+    const CullList *cull;      // per heliostat id: candidate quads of each traversal (null: no culling)
     int32_t P, L;
     FastDiv fdP, fdL;
     int32_t n_list;

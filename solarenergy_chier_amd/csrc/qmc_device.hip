@@ -39,18 +39,28 @@ namespace {
 // When none can, every test of that traversal is a miss, so skipping it returns the same result for
 // every ray. Flat heliostats only; others always traverse.
 __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
+    __shared__ int s_n[2];
+    __shared__ int s_ids[2][kCullList];
     const int h = a.list[blockIdx.x];
     const DevHelio &H = a.helios[h];
     const DevCullShape S = a.shapes[h];
     // canted facets turn reflections further from the ideal one: twice the canting angle
     const double ang_block = a.ang_block < 0.0 ? -1.0 : a.ang_block + 2.0 * S.cant;
     const double tan_block = (ang_block >= 0.0 && ang_block < kCullMaxAngle) ? tan(ang_block) : -1.0;
-    bool may_shadow = a.tan_sun < 0.0, may_block = tan_block < 0.0;
-    cull_scan(S, a.sun, a.quads, a.n_sub, H.sub_begin, H.sub_begin + H.sub_count, a.tan_sun, tan_block, a.margin,
-              threadIdx.x, blockDim.x, may_shadow, may_block);
-    const int any_shadow = __syncthreads_or(may_shadow ? 1 : 0);
-    const int any_block = __syncthreads_or(may_block ? 1 : 0);
-    if (threadIdx.x == 0) a.flags[h] = (uint8_t)((any_shadow ? 1 : 0) | (any_block ? 2 : 0));
+    if (threadIdx.x < 2) s_n[threadIdx.x] = 0;
+    __syncthreads();
+    cull_collect(S, a.sun, a.quads, a.n_sub, H.sub_begin, H.sub_begin + H.sub_count, a.tan_sun, tan_block, a.margin,
+                 threadIdx.x, blockDim.x, s_n, s_ids);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        CullList L;
+        // a cone that cannot be bounded keeps its traversal for every ray
+        L.n[0] = a.tan_sun < 0.0 ? kCullList + 1 : s_n[0];
+        L.n[1] = tan_block < 0.0 ? kCullList + 1 : s_n[1];
+        for (int k = 0; k < 2; ++k)
+            for (int i = 0; i < kCullList; ++i) L.ids[k][i] = i < L.n[k] ? s_ids[k][i] : -1;
+        a.lists[h] = L;
+    }
 }
 
 template <int RT, bool REC, bool STATS>
@@ -99,9 +109,10 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         int index = -1;
         float energy = 0.0f;
         int flags = 0;
-        // traversals the cull kernel proved hit-free are skipped (never while counting work)
-        const unsigned need = (STATS || !a.cull) ? 3u : (unsigned)a.cull[h];
-        if ((need & 1u) && dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
+        // traversals proved hit-free for this ray are skipped (never while counting work)
+        const bool culling = !STATS && a.cull;
+        if ((!culling || traversal_needed(a.cull[h], 0, origin, dir, a.grid.quads)) &&
+            dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
             status = QMC_RAY_SHADOWED;
             index = hit;
         } else {
@@ -111,7 +122,8 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             const float4 q = a.pert[a2];
             V3 normal = normalize(fr_apply(*pf, V3{q.x, q.y, q.z}));
             dir = normalize(reflect(-dir, normal));
-            if ((need & 2u) && dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
+            if ((!culling || traversal_needed(a.cull[h], 1, origin, dir, a.grid.quads)) &&
+                dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
                 status = QMC_RAY_BLOCKED;
                 index = hit;
             } else {
@@ -1031,9 +1043,9 @@ extern "C" qmc_status qmc_facets_get(qmc_ctx *c, int32_t h, float *origins, floa
 // ---------------------------------------------------------------------------------------------
 // Trace
 // ---------------------------------------------------------------------------------------------
-// Traversal-culling flags (k_cull) of the listed heliostats of a grid, on the ctx stream, into the
-// ctx-owned per-heliostat array.
-qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const uint8_t **flags) {
+// Traversal-culling candidate lists (k_cull) of the listed heliostats of a grid, on the ctx stream,
+// into the ctx-owned per-heliostat array.
+qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists) {
     const GridBuf &gb = c->grids[grid];
     qmc_status st;
     if (c->cull_cap < c->h_helios.size()) {
@@ -1050,10 +1062,10 @@ qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, 
     ca.tan_sun = c->cull_tan_sun;
     ca.ang_block = c->cull_ang_block;
     ca.margin = kCullMargin;
-    ca.flags = c->d_cull;
+    ca.lists = c->d_cull;
     hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
     HIP_TRY(hipGetLastError());
-    *flags = c->d_cull;
+    *lists = c->d_cull;
     return QMC_OK;
 }
 
@@ -1154,6 +1166,33 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
 extern "C" qmc_status qmc_trace(qmc_ctx *c, const int32_t *ids, int32_t n) {
     if (!c) return fail(QMC_E_INVALID_ARG, "qmc_trace: null ctx");
     return trace_common(c, ids, n, nullptr, nullptr);
+}
+
+extern "C" qmc_status qmc_cull_flags(qmc_ctx *c, const int32_t *ids, int32_t n, uint8_t *flags) {
+    if (!c || (n > 0 && (!ids || !flags))) return fail(QMC_E_INVALID_ARG, "qmc_cull_flags: null argument");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_cull_flags: upload a scene first");
+    if (!c->pool_ready) return fail(QMC_E_STATE, "qmc_cull_flags: set or generate the sample pools first");
+    const int nh = (int)c->h_helios.size();
+    for (int i = 0; i < n; ++i)
+        if (ids[i] < 0 || ids[i] >= nh) return fail(QMC_E_RANGE, "qmc_cull_flags: heliostat out of range");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    std::vector<CullList> all((size_t)nh);
+    for (size_t g = 0; g < c->grids.size(); ++g) {
+        std::vector<int32_t> list;
+        for (int i = 0; i < n; ++i)
+            if (c->h_helios[ids[i]].grid == (int)g) list.push_back(ids[i]);
+        if (list.empty()) continue;
+        HIP_TRY(hipMemcpyAsync(c->d_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, c->stream));
+        const CullList *d_lists = nullptr;
+        if ((st = launch_cull(c, (int)g, c->d_list, (int)list.size(), &d_lists))) return st;
+        HIP_TRY(hipMemcpyAsync(all.data(), d_lists, sizeof(CullList) * (size_t)nh, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));  // the list and candidate buffers are reused per grid
+        for (int i = 0; i < n; ++i)
+            if (c->h_helios[ids[i]].grid == (int)g)
+                flags[i] = (uint8_t)((all[ids[i]].n[0] > 0 ? 1 : 0) | (all[ids[i]].n[1] > 0 ? 2 : 0));
+    }
+    return QMC_OK;
 }
 
 extern "C" qmc_status qmc_trace_records(qmc_ctx *c, int32_t h, qmc_ray_record *rec, int64_t n) {

@@ -194,17 +194,22 @@ QMC_HD CD3 cmul(CD3 a, double s) { return CD3{a.x * s, a.y * s, a.z * s}; }
 QMC_HD double cdot(CD3 a, CD3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 QMC_HD CD3 ccross(CD3 a, CD3 b) { return CD3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 
-QMC_HD bool cone_may_hit(CD3 c, CD3 hu, CD3 hn, CD3 hv, CD3 a, double tan_theta, CD3 A, CD3 E1, CD3 E2,
-                         double margin) {
-    const bool thick = cdot(hn, hn) > 0.0;
-    const int np = thick ? 8 : 4;
-    CD3 P[8];
-    for (int i = 0; i < 8; ++i)
-        P[i] = cadd(cadd(cadd(c, cmul(hu, (i & 1) ? -1.0 : 1.0)), cmul(hv, (i & 2) ? -1.0 : 1.0)),
-                    cmul(hn, (i & 4) ? -1.0 : 1.0));
+// NP = 8: a box; NP = 4: a rectangle (hn is then ignored). Loops are fully unrolled so the corner
+// arrays stay in registers.
+template <int NP>
+QMC_HD bool cone_may_hit_n(CD3 c, CD3 hu, CD3 hn, CD3 hv, CD3 a, double tan_theta, CD3 A, CD3 E1, CD3 E2,
+                           double margin) {
+    CD3 P[NP];
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        P[i] = cadd(cadd(c, cmul(hu, (i & 1) ? -1.0 : 1.0)), cmul(hv, (i & 2) ? -1.0 : 1.0));
+        if (NP == 8) P[i] = cadd(P[i], cmul(hn, (i & 4) ? -1.0 : 1.0));
+    }
     CD3 Q[4] = {A, cadd(A, E1), cadd(A, E2), cadd(cadd(A, E1), E2)};
     double pa_min = 1e300, qa_max = -1e300;
-    for (int i = 0; i < np; ++i) pa_min = fmin(pa_min, cdot(P[i], a));
+#pragma unroll
+    for (int i = 0; i < NP; ++i) pa_min = fmin(pa_min, cdot(P[i], a));
+#pragma unroll
     for (int i = 0; i < 4; ++i) qa_max = fmax(qa_max, cdot(Q[i], a));
     const double tmax = qa_max - pa_min;
     if (tmax < -margin) return false;  // the quad lies wholly behind every origin
@@ -213,16 +218,19 @@ QMC_HD bool cone_may_hit(CD3 c, CD3 hu, CD3 hn, CD3 hv, CD3 a, double tan_theta,
     const CD3 dc = csub(qc, c);
     CD3 axes[6] = {ccross(a, hu), ccross(a, hv), ccross(a, E1), ccross(a, E2), csub(dc, cmul(a, cdot(dc, a))),
                    ccross(a, hn)};
-    for (int k = 0; k < 6; ++k) {
+#pragma unroll
+    for (int k = 0; k < (NP == 8 ? 6 : 5); ++k) {
         const double len = sqrt(cdot(axes[k], axes[k]));
         if (!(len > 1e-12)) continue;
         const CD3 w = cmul(axes[k], 1.0 / len);
         double p_lo = 1e300, p_hi = -1e300, q_lo = 1e300, q_hi = -1e300;
-        for (int i = 0; i < np; ++i) {
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
             const double pw = cdot(P[i], w);
             p_lo = fmin(p_lo, pw);
             p_hi = fmax(p_hi, pw);
         }
+#pragma unroll
         for (int i = 0; i < 4; ++i) {
             const double qw = cdot(Q[i], w);
             q_lo = fmin(q_lo, qw);
@@ -231,6 +239,12 @@ QMC_HD bool cone_may_hit(CD3 c, CD3 hu, CD3 hn, CD3 hv, CD3 a, double tan_theta,
         if (q_lo > p_hi + reach || q_hi < p_lo - reach) return false;
     }
     return true;
+}
+
+QMC_HD bool cone_may_hit(CD3 c, CD3 hu, CD3 hn, CD3 hv, CD3 a, double tan_theta, CD3 A, CD3 E1, CD3 E2,
+                         double margin) {
+    return cdot(hn, hn) > 0.0 ? cone_may_hit_n<8>(c, hu, hn, hv, a, tan_theta, A, E1, E2, margin)
+                              : cone_may_hit_n<4>(c, hu, hn, hv, a, tan_theta, A, E1, E2, margin);
 }
 
 // Cheaper, weaker form of the same bound for a pre-test: origins within `r` of a centre, the quad

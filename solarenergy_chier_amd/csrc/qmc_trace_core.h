@@ -174,8 +174,8 @@ __host__ __device__ inline DevCullShape flat_cull_shape(const DevHelio &H, const
     return s;
 }
 
-// One thread's share of the traversal-culling scan of a heliostat (culling shape S, own quads
-// [own_lo, own_hi)) against the quads q = first, first + step, ... of a grid: sets may_shadow /
+// One thread's share of the traversal-culling scan of a flat heliostat (culling shape S, hn = 0; own
+// quads [own_lo, own_hi)) against the quads q = first, first + step, ... of a grid: sets may_shadow /
 // may_block when some other quad can meet its shadow cone (tan_sun around the sun axis, the n axis of
 // sunf) or its blocking cone (tan_block around the sun axis reflected in the heliostat normal), per
 // cone_may_hit. The caller ORs the flags over its threads; a traversal whose flag stays clear can
@@ -199,13 +199,70 @@ __device__ __forceinline__ void cull_scan(const DevCullShape &S, const DevFrame 
         const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
         const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
         if (!may_shadow && sphere_may_hit(D, R, ash, tan_sun, margin) &&
-            cone_may_hit(c, hu, hn, hv, ash, tan_sun, A, E1, E2, margin))
+            cone_may_hit_n<4>(c, hu, hn, hv, ash, tan_sun, A, E1, E2, margin))
             may_shadow = true;
         if (!may_block && sphere_may_hit(D, R, r0, tan_block, margin) &&
-            cone_may_hit(c, hu, hn, hv, r0, tan_block, A, E1, E2, margin))
+            cone_may_hit_n<4>(c, hu, hn, hv, r0, tan_block, A, E1, E2, margin))
             may_block = true;
         if (may_shadow && may_block) break;
     }
+}
+
+// The scan of cull_scan, keeping the candidates: every quad that passes the bound for the shadow
+// (kind 0) or blocking (kind 1) cone is counted in n[kind] and, while the count is at most kCullList,
+// listed in ids[kind]; a negative tangent skips that kind. n and ids are shared by the workgroup (LDS,
+// n zeroed beforehand). A kind stops being scanned once its count exceeds kCullList (it is then
+// traversed for every ray and its list is not used), so a final count <= kCullList is complete.
+__device__ __forceinline__ void cull_collect(const DevCullShape &S, const DevFrame &sunf, const float4 *quads,
+                                             int n_sub, int own_lo, int own_hi, double tan_sun, double tan_block,
+                                             double margin, int first, int step, int *n, int (*ids)[kCullList]) {
+    const CD3 c{S.c[0], S.c[1], S.c[2]}, hu{S.hu[0], S.hu[1], S.hu[2]}, hn{S.hn[0], S.hn[1], S.hn[2]},
+        hv{S.hv[0], S.hv[1], S.hv[2]}, nrm{S.n[0], S.n[1], S.n[2]};
+    const CD3 ash = sunf.identity ? CD3{0, 1, 0} : CD3{sunf.nx, sunf.ny, sunf.nz};
+    CD3 r0 = cadd(cmul(ash, -1.0), cmul(nrm, 2.0 * cdot(ash, nrm)));  // reflect(-sun axis, n)
+    r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
+    const double rs = sqrt(cdot(hu, hu) + cdot(hn, hn) + cdot(hv, hv));
+    for (int q = first; q < n_sub; q += step) {
+        if (q >= own_lo && q < own_hi) continue;
+        const bool want_s = tan_sun >= 0.0 &&
+                            __hip_atomic_load(n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= kCullList;
+        const bool want_b = tan_block >= 0.0 &&
+                            __hip_atomic_load(n + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= kCullList;
+        if (!want_s && !want_b) break;
+        const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
+        const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
+        const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
+        const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
+        const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
+        if (want_s && sphere_may_hit(D, R, ash, tan_sun, margin) &&
+            cone_may_hit(c, hu, hn, hv, ash, tan_sun, A, E1, E2, margin)) {
+            const int k = atomicAdd(n, 1);
+            if (k < kCullList) ids[0][k] = q;
+        }
+        if (want_b && sphere_may_hit(D, R, r0, tan_block, margin) &&
+            cone_may_hit(c, hu, hn, hv, r0, tan_block, A, E1, E2, margin)) {
+            const int k = atomicAdd(n + 1, 1);
+            if (k < kCullList) ids[1][k] = q;
+        }
+    }
+}
+
+// Whether a traversal has to run for this ray, given its heliostat's candidate list for that
+// traversal: not when no quad can be met (count 0) or when the ray misses every listed candidate —
+// the traversal tests only quads, the others provably miss, and a listed one gives the same
+// ray_quad result here as there. Otherwise (a candidate is hit, or too many to list) the traversal
+// runs and decides, with the reference's first-hit order.
+__device__ __forceinline__ bool traversal_needed(const CullList &cl, int kind, V3 o, V3 d, const float4 *quads) {
+    const int n = cl.n[kind];
+    if (n == 0) return false;
+    if (n > kCullList) return true;
+    for (int i = 0; i < n; ++i) {
+        const int id = cl.ids[kind][i];
+        const float4 qa = quads[3 * id], qb = quads[3 * id + 1], qc = quads[3 * id + 2];
+        float t, u, v;
+        if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) return true;
+    }
+    return false;
 }
 
 // Half-angles (radians) at or above which a cone is not culled: the bound needs theta < 90 deg.

@@ -363,6 +363,13 @@ class Device:
         a = np.ascontiguousarray(list(ids), np.int32)
         N.call("qmc_trace", self.handle, N.iptr(a), len(a))
 
+    def cull_flags(self, ids: Iterable[int]) -> np.ndarray:
+        """Per heliostat: bit 0 = shadow traversal needed, bit 1 = blocking traversal needed."""
+        a = np.ascontiguousarray(list(ids), np.int32)
+        out = np.zeros(len(a), np.uint8)
+        N.call("qmc_cull_flags", self.handle, N.iptr(a), len(a), out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return out
+
     def trace_records(self, h: int, n_rays: int) -> np.ndarray:
         rec = np.zeros(n_rays, dtype=N.RECORD_DTYPE)
         N.call("qmc_trace_records", self.handle, h, rec.ctypes.data_as(C.POINTER(N.RayRecord)), n_rays)

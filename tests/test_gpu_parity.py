@@ -122,6 +122,8 @@ def parity_on_scene(device, scene, heliostats, pools=None, seeds=(2, 4, 3)):
     dirs, perts = device.pool_get(P)
     orc = O.OracleScene.from_product_scene(scene)
     bad = 0
+    heliostats = list(heliostats)
+    flags = dict(zip(heliostats, device.cull_flags(heliostats)))
     for h in heliostats:
         g = scene.heliostat_info(h).grid
         r = scene.grid_desc(g).receiver
@@ -132,8 +134,14 @@ def parity_on_scene(device, scene, heliostats, pools=None, seeds=(2, 4, 3)):
         flux = device.flux(r)
         cflux, crec, _ = orc.trace_heliostat(h, dirs, perts, L, offs)
         b = compare_records(rec, crec)
-        print(f"heliostat {h}: {b} mismatches of {F * L}; deposited {(rec['status'] == 3).sum()}", flush=True)
+        print(f"heliostat {h}: {b} mismatches of {F * L}; deposited {(rec['status'] == 3).sum()}; "
+              f"cull flags {flags[h]}", flush=True)
         bad += b
+        # a traversal the cull kernel skips must be one in which the oracle's DDA finds no hit
+        if not flags[h] & 1:
+            assert not (crec["status"] == 0).any(), f"heliostat {h}: shadow traversal culled but shadowed rays"
+        if not flags[h] & 2:
+            assert not (crec["status"] == 1).any(), f"heliostat {h}: blocking traversal culled but blocked rays"
         if cflux.sum() > 0:
             assert rel_l1(flux, cflux) < FLUX_RTOL_L1
         else:
@@ -179,6 +187,21 @@ def test_ps10_field_sampled_heliostats(device):
     scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
     # innermost, a mid-field and the outermost heliostats (blocking varies with radius)
     assert parity_on_scene(device, scene, [0, 17, 300, 623]) == 0
+
+
+def test_cull_flags_on_full_fields(device):
+    """How much the culling bound removes on the two large fields (printed for the record); that no
+    culled traversal has a hit is checked ray by ray against the oracle in parity_on_scene."""
+    for wl in (workloads.ps10_field(), workloads.cylinder_field()):
+        scene = setup_device(device, wl)
+        n = scene.counts.n_heliostats
+        f = device.cull_flags(range(n))
+        print(f"{wl.name}: {n} heliostats, shadow traversal needed {int((f & 1).sum())}, "
+              f"blocking needed {int(((f >> 1) & 1).sum())}", flush=True)
+        assert f.max() <= 3
+        assert (f & 3).sum() < 2 * n  # the bound removes something on both fields
+    with pytest.raises(RuntimeError):
+        device.cull_flags([n])
 
 
 def test_cylinder_field_sampled_heliostats(device):
