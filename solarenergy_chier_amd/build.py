@@ -25,10 +25,11 @@ PIPELINE = BINDIR / "qmc_pipeline"
 
 HIP_SOURCES = ["qmc_device.hip", "qmc_batch.hip", "qmc_adjoint.hip"]
 HOST_SOURCES = ["scene_host.cpp", "image_saver.cpp"]
-# The persistent batched kernel loops over rays; machine-level loop-invariant code motion hoists
-# copies of wave-uniform operands out of that loop into vector registers (about 25 of them), which
-# costs two waves per SIMD of occupancy on a latency-bound traversal.
-EXTRA_FLAGS = {"qmc_batch.hip": ["-mllvm", "-disable-machine-licm"]}
+# The persistent kernels (batched sweep, banded plain trace) loop over rays; machine-level
+# loop-invariant code motion hoists copies of wave-uniform operands out of those loops into vector
+# registers (22 to 48 of them), which costs occupancy on a latency-bound traversal.
+EXTRA_FLAGS = {"qmc_batch.hip": ["-mllvm", "-disable-machine-licm"],
+               "qmc_device.hip": ["-mllvm", "-disable-machine-licm"]}
 
 ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")
 

@@ -247,15 +247,6 @@ std::string shape_key(qmc_ctx *c, int grid, int npix, int n_helio, int rays_per_
     return o.str();
 }
 
-qmc_status ensure_device_limits(qmc_ctx *c) {
-    if (c->num_cus > 0) return QMC_OK;
-    int v = 0;
-    HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, c->device));
-    c->num_cus = v;
-    HIP_TRY(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-    c->max_lds = (size_t)v;
-    return QMC_OK;
-}
 
 struct BatchPlan {
     BatchArgs args;
@@ -390,7 +381,7 @@ qmc_status plan_batch(qmc_ctx *c, int grid, const qmc_instance *inst, int n, Bat
         if (inst[i].aim_set < 0 || inst[i].aim_set >= c->n_aim_sets)
             return fail(QMC_E_RANGE, "batched trace: instance " + std::to_string(i) + " has aim set " +
                                          std::to_string(inst[i].aim_set) + " of " + std::to_string(c->n_aim_sets));
-    qmc_status st = ensure_device_limits(c);
+    qmc_status st = device_limits(c);
     if (st) return st;
     const GridBuf &gb = c->grids[grid];
     const RecvBuf &rb = c->recvs[gb.receiver];

@@ -73,7 +73,9 @@ struct qmc_ctx {
     int32_t batch_npix = 0;
     int32_t batch_grid = 0;
     int num_cus = 0;
-    size_t max_lds = 0;
+    size_t max_lds = 0, lds_per_cu = 0;
+    float *d_band_partials = nullptr;  // per-workgroup bands of the persistent plain trace
+    size_t band_cap = 0;
     // traversal culling: tangents of the sample pools' cone half-angles (sun shape; sun shape plus
     // twice the slope error), negative when culling does not apply, and the candidate lists per heliostat
     double cull_tan_sun = -1.0, cull_tan_block = -1.0, cull_ang_block = -1.0;
@@ -150,6 +152,19 @@ inline void free_scene(qmc_ctx *c) {
     c->h_aim64.clear();
     c->uploaded = false;
     c->offsets_ready = false;
+}
+
+// CU count and LDS limits of the ctx's device, queried once (all three together).
+inline qmc_status device_limits(qmc_ctx *c) {
+    if (c->num_cus > 0 && c->max_lds > 0 && c->lds_per_cu > 0) return QMC_OK;
+    int cus = 0, per_block = 0, per_cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    HIP_TRY(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    HIP_TRY(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device));
+    c->num_cus = cus;
+    c->max_lds = (size_t)per_block;
+    c->lds_per_cu = (size_t)per_cu;
+    return QMC_OK;
 }
 
 inline qmc_status set_device(qmc_ctx *c) {

@@ -63,22 +63,95 @@ __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
     }
 }
 
+// Ray slot -> ray id of heliostat H: flat heliostats take their rays in facet tiles (a wave's origins
+// form a compact patch); -1 for slots past the heliostat's rays.
+__device__ __forceinline__ long long slot_ray(const DevHelio &H, int L, long long slot) {
+    if (H.type == 0) {
+        const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
+        const FacetTiling tl = facet_tiling(R, C, L);
+        return slot < tl.slots ? facet_tiled_ray(tl, tile_divisor(H, tl), R, C, (int)slot) : -1;
+    }
+    return slot < (long long)H.F * L ? slot : -1;
+}
+
+// Outcome of one ray: record fields, the deposit's pixel, and what the work counters need.
+struct RayOut {
+    int status, index, flags, row, col;
+    float energy;
+    bool unshadowed, unblocked, first_of_facet;
+};
+
+// One ray of map_raytracing (RectangleReceiverRectangleGridRayTracing.cu:6-41,
+// CylinderReceiverRectangleGridRayTracing.cu:6-40) up to, not including, its deposit.
+template <int RT, bool STATS>
+__device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H, int h, long long r, RayOut &o,
+                                          RayStats &st) {
+    const int L = a.L;
+    const int m = (int)fast_div((uint32_t)r, a.fdL);
+    const int j = (int)r - m * L;
+    const int32_t *grp = a.offsets + H.offsets_base;
+    const int P = a.P;
+    const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
+    o.index = -1;
+    o.energy = 0.0f;
+    o.flags = 0;
+    o.row = o.col = -1;
+    o.unshadowed = o.unblocked = false;
+    o.first_of_facet = j == 0;
+
+    // Step 1: incident direction and shadowing (RectangleReceiverRectangleGridRayTracing.cu:14-22)
+    const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
+    const float4 s = a.pool[a1];
+    V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
+    V3 origin, fnormal;
+    const DevFrame *pf;
+    facet_geometry(H, a.facets, m, origin, pf, fnormal);
+    int hit = -1;
+    // traversals proved hit-free for this ray are skipped (never while counting work)
+    const bool culling = !STATS && a.cull;
+    if ((!culling || traversal_needed(a.cull[h], 0, origin, dir, a.grid.quads)) &&
+        dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
+        o.status = QMC_RAY_SHADOWED;
+        o.index = hit;
+        return;
+    }
+    o.unshadowed = true;
+    // Step 2: perturbed facet normal, reflection, blocking (:24-36)
+    const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
+    const float4 q = a.pert[a2];
+    V3 normal = normalize(fr_apply(*pf, V3{q.x, q.y, q.z}));
+    dir = normalize(reflect(-dir, normal));
+    if ((!culling || traversal_needed(a.cull[h], 1, origin, dir, a.grid.quads)) &&
+        dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
+        o.status = QMC_RAY_BLOCKED;
+        o.index = hit;
+        return;
+    }
+    o.unblocked = true;
+    // Step 3: receiver hit (Receiver/*ReceiverIntersection.cu)
+    float t, u, v;
+    if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v)) {
+        o.status = QMC_RAY_MISSED;
+        return;
+    }
+    o.energy = ray_energy(t, dir, normal, H.factor);
+    int row = (int)(u * (float)a.recv.res_y);
+    int col = (int)(v * (float)a.recv.res_x);
+    if (row >= a.recv.res_y) { row = a.recv.res_y - 1; o.flags |= QMC_RAYF_EDGE_CLAMPED; }
+    if (col >= a.recv.res_x) { col = a.recv.res_x - 1; o.flags |= QMC_RAYF_EDGE_CLAMPED; }
+    o.row = row;
+    o.col = col;
+    o.index = row * a.recv.res_x + col;
+    o.status = QMC_RAY_DEPOSITED;
+}
+
+// One lane per ray slot; grid.y = heliostat of the list, grid.x = blocks of 256 slots. Deposits are
+// device-scope float atomics on the receiver's map (receiver_drawing, rectangleReceiverIntersection.cu:24).
 template <int RT, bool REC, bool STATS>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const int h = a.list[blockIdx.y];
     const DevHelio &H = a.helios[h];
-    const int L = a.L;
-    const long long nrays = (long long)H.F * L;
-    // flat heliostats take their rays in facet tiles (a wave's origins form a compact patch)
-    const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long r = -1;
-    if (H.type == 0) {
-        const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
-        const FacetTiling tl = facet_tiling(R, C, L);
-        if (slot < tl.slots) r = facet_tiled_ray(tl, tile_divisor(H, tl), R, C, (int)slot);
-    } else if (slot < nrays) {
-        r = slot;
-    }
+    const long long r = slot_ray(H, a.L, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 
     __shared__ unsigned long long s_acc[9];
     if (STATS) {
@@ -86,89 +159,32 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         __syncthreads();
     }
     RayStats st{0u, 0u, 0u};
-    unsigned c_unshadowed = 0, c_unblocked = 0, c_dep = 0, c_edge = 0;
+    RayOut o;
     const bool active = r >= 0;
     if (!STATS && !active) return;
-
     if (active) {
-        const int m = (int)fast_div((uint32_t)r, a.fdL);
-        const int j = (int)r - m * L;
-        const int32_t *grp = a.offsets + H.offsets_base;
-        const int P = a.P;
-        const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
-
-        // Step 1: incident direction and shadowing (RectangleReceiverRectangleGridRayTracing.cu:14-22)
-        const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
-        const float4 s = a.pool[a1];
-        V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
-        V3 origin, fnormal;
-        const DevFrame *pf;
-        facet_geometry(H, a.facets, m, origin, pf, fnormal);
-        int hit = -1;
-        int status;
-        int index = -1;
-        float energy = 0.0f;
-        int flags = 0;
-        // traversals proved hit-free for this ray are skipped (never while counting work)
-        const bool culling = !STATS && a.cull;
-        if ((!culling || traversal_needed(a.cull[h], 0, origin, dir, a.grid.quads)) &&
-            dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
-            status = QMC_RAY_SHADOWED;
-            index = hit;
-        } else {
-            c_unshadowed = 1;
-            // Step 2: perturbed facet normal, reflection, blocking (:24-36)
-            const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
-            const float4 q = a.pert[a2];
-            V3 normal = normalize(fr_apply(*pf, V3{q.x, q.y, q.z}));
-            dir = normalize(reflect(-dir, normal));
-            if ((!culling || traversal_needed(a.cull[h], 1, origin, dir, a.grid.quads)) &&
-                dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
-                status = QMC_RAY_BLOCKED;
-                index = hit;
-            } else {
-                c_unblocked = 1;
-                // Step 3: receiver hit and deposit (Receiver/*ReceiverIntersection.cu)
-                float t, u, v;
-                if (receiver_hit<RT>(a.recv, origin, dir, t, u, v)) {
-                    energy = ray_energy(t, dir, normal, H.factor);
-                    int row = (int)(u * (float)a.recv.res_y);
-                    int col = (int)(v * (float)a.recv.res_x);
-                    if (row >= a.recv.res_y) { row = a.recv.res_y - 1; flags |= QMC_RAYF_EDGE_CLAMPED; }
-                    if (col >= a.recv.res_x) { col = a.recv.res_x - 1; flags |= QMC_RAYF_EDGE_CLAMPED; }
-                    const int addr = row * a.recv.res_x + col;
-                    unsafeAtomicAdd(a.flux + addr, energy);
-                    status = QMC_RAY_DEPOSITED;
-                    index = addr;
-                    c_dep = 1;
-                    c_edge = flags ? 1u : 0u;
-                } else {
-                    status = QMC_RAY_MISSED;
-                }
-            }
-        }
+        trace_ray<RT, STATS>(a, H, h, r, o, st);
+        if (o.status == QMC_RAY_DEPOSITED) unsafeAtomicAdd(a.flux + o.index, o.energy);
         if (REC) {
             qmc_ray_record rec;
-            rec.status = status;
-            rec.index = index;
-            rec.energy = energy;
-            rec.flags = flags;
+            rec.status = o.status;
+            rec.index = o.index;
+            rec.energy = o.energy;
+            rec.flags = o.flags;
             a.records[r] = rec;
-        }
-        if (STATS) {
-            atomicAdd(&s_acc[0], 1ull);
-            if (j == 0) atomicAdd(&s_acc[7], 1ull);
         }
     }
     if (STATS) {
         if (active) {
-            if (c_unshadowed) atomicAdd(&s_acc[1], 1ull);
-            if (c_unblocked) atomicAdd(&s_acc[2], 1ull);
-            if (c_dep) atomicAdd(&s_acc[3], 1ull);
+            atomicAdd(&s_acc[0], 1ull);
+            if (o.first_of_facet) atomicAdd(&s_acc[7], 1ull);
+            if (o.unshadowed) atomicAdd(&s_acc[1], 1ull);
+            if (o.unblocked) atomicAdd(&s_acc[2], 1ull);
+            if (o.status == QMC_RAY_DEPOSITED) atomicAdd(&s_acc[3], 1ull);
             atomicAdd(&s_acc[4], (unsigned long long)st.cells);
             atomicAdd(&s_acc[5], (unsigned long long)st.candidates);
             atomicAdd(&s_acc[6], (unsigned long long)st.tests);
-            if (c_edge) atomicAdd(&s_acc[8], 1ull);
+            if (o.status == QMC_RAY_DEPOSITED && o.flags) atomicAdd(&s_acc[8], 1ull);
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -177,6 +193,53 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
                 if (s_acc[k]) atomicAdd(g + k, s_acc[k]);
         }
     }
+}
+
+// Persistent form of k_trace for large launches. Global float atomics run at a fixed rate (about
+// 21 G/s for scattered addresses on MI355X, against 180 G/s in LDS), and a large field deposits
+// hundreds of millions of rays into one map, so each workgroup keeps a band of map rows
+// [row0, row0 + rows) in LDS for the whole launch: deposits inside it are LDS atomics, the others
+// device-scope atomics on the map. Workgroups take BS-slot blocks (heliostat, block) round robin;
+// at the end each stores its band, with plain stores, to its own slice of `partials`, which
+// k_band_fold adds into the map.
+template <int RT, int BS>
+__global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int rows, int blocks_per_helio,
+                                                    float *partials) {
+    extern __shared__ float band[];
+    const int W = a.recv.res_x, nband = rows * W;
+    for (int p = threadIdx.x; p < nband; p += BS) band[p] = 0.0f;
+    __syncthreads();
+    const long long items = (long long)a.n_list * blocks_per_helio;
+    RayStats st{0u, 0u, 0u};
+#pragma nounroll
+    for (long long it = blockIdx.x; it < items; it += gridDim.x) {
+        const int li = (int)(it / blocks_per_helio), bx = (int)(it - (long long)li * blocks_per_helio);
+        const int h = a.list[li];
+        const DevHelio &H = a.helios[h];
+        const long long r = slot_ray(H, a.L, (long long)bx * BS + threadIdx.x);
+        if (r < 0) continue;
+        RayOut o;
+        trace_ray<RT, false>(a, H, h, r, o, st);
+        if (o.status != QMC_RAY_DEPOSITED) continue;
+        const int br = o.row - row0;
+        if ((unsigned)br < (unsigned)rows)
+            atomicAdd(band + br * W + o.col, o.energy);
+        else
+            unsafeAtomicAdd(a.flux + o.index, o.energy);
+    }
+    __syncthreads();
+    float *mine = partials + (size_t)blockIdx.x * nband;
+    for (int p = threadIdx.x; p < nband; p += BS) mine[p] = band[p];
+}
+
+// Adds the workgroups' bands into rows [row0, row0 + nband / W) of the map.
+__global__ __launch_bounds__(256) void k_band_fold(float *flux, const float *partials, int n_parts, int nband,
+                                                   int offset) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nband) return;
+    float s = 0.0f;
+    for (int w = 0; w < n_parts; ++w) s += partials[(size_t)w * nband + p];
+    flux[offset + p] += s;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -458,6 +521,7 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_batch_work);
     dev_free(c->d_cull);
     dev_free(c->d_cull_shapes);
+    dev_free(c->d_band_partials);
     dev_free(c->d_buie_phys);
     if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
     c->d_batch_inst = nullptr;
@@ -1071,6 +1135,56 @@ qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, 
 
 namespace {
 
+constexpr int kBandBS = 1024;
+
+template <int RT>
+void launch_band(dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph, float *parts) {
+    hipLaunchKernelGGL((k_trace_band<RT, kBandBS>), g, dim3(kBandBS), lds, st, a, row0, rows, bph, parts);
+}
+
+// The banded persistent trace when it pays. Measured on MI355X: its persistent loop runs about 15%
+// slower than one lane per ray (traversal-heavy fields lose more), so it is used only where the band
+// takes most deposits off the global atomics — a band of at least half the map's rows, one
+// workgroup per CU with the whole LDS (4 waves per SIMD measured best) — and where every workgroup
+// deposits many times its band (its band store and fold are then a small part of the launch).
+qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, long long max_slots, bool &launched) {
+    launched = false;
+    const int per_cu = 1;
+    if (qmc_status st = device_limits(c)) return st;
+    const int W = rb.w, Hh = rb.h;
+    const size_t lds_budget = std::min(c->max_lds, c->lds_per_cu / (size_t)per_cu);
+    const int rows = (int)std::min<size_t>((size_t)Hh, lds_budget / (sizeof(float) * (size_t)W));
+    if (rows < 1 || 2 * rows < Hh) return QMC_OK;
+    const size_t lds = sizeof(float) * (size_t)rows * W;
+    int resident = 0;
+    const hipError_t e = a.recv.type == 0
+                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<0, kBandBS>, kBandBS, lds)
+                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<1, kBandBS>, kBandBS, lds);
+    if (e != hipSuccess || resident < 1) return QMC_OK;
+    const int nwg = c->num_cus * std::min(per_cu, resident);
+    const long long bph = (max_slots + kBandBS - 1) / kBandBS;
+    const long long total = bph * kBandBS * a.n_list;
+    const long long nband = (long long)rows * W;
+    if (total < 8 * (long long)nwg * nband) return QMC_OK;
+    if ((size_t)nwg * nband > c->band_cap) {
+        qmc_status st = dev_alloc(c->d_band_partials, (size_t)nwg * nband);
+        if (st) {
+            c->band_cap = 0;
+            return st;
+        }
+        c->band_cap = (size_t)nwg * nband;
+    }
+    const int row0 = (Hh - rows) / 2;  // flux concentrates around the aim point, the receiver centre
+    if (a.recv.type == 0) launch_band<0>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
+    else launch_band<1>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256)), dim3(256), 0, c->stream, rb.flux,
+                       c->d_band_partials, nwg, (int)nband, row0 * W);
+    HIP_TRY(hipGetLastError());
+    launched = true;
+    return QMC_OK;
+}
+
 qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
                         qmc_ray_record *rec, StatsAcc *stats) {
     const GridBuf &gb = c->grids[grid];
@@ -1102,6 +1216,12 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     if (!stats) {
         const qmc_status st = launch_cull(c, grid, d_list, n, &a.cull);
         if (st) return st;
+    }
+    if (!rec && !stats) {
+        bool launched = false;
+        const qmc_status st = try_band_launch(c, rb, a, max_slots, launched);
+        if (st) return st;
+        if (launched) return QMC_OK;
     }
     dim3 g((unsigned)bx, (unsigned)n), b(256);
     const int rt = rb.r.type;
