@@ -24,6 +24,7 @@ struct GridBuf {
 struct RecvBuf {
     DevReceiver r{};
     float *flux = nullptr;
+    double *acc = nullptr;  // a launch's deposits in double, folded into flux after it (zero between launches)
     int w = 0, h = 0;
 };
 
@@ -139,7 +140,10 @@ inline void free_scene(qmc_ctx *c) {
         dev_free(g.quads);
     }
     c->grids.clear();
-    for (auto &r : c->recvs) dev_free(r.flux);
+    for (auto &r : c->recvs) {
+        dev_free(r.flux);
+        dev_free(r.acc);
+    }
     c->recvs.clear();
     dev_free(c->d_helios);
     dev_free(c->d_facets);

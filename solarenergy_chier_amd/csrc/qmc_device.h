@@ -133,6 +133,7 @@ struct TraceArgs {
     const int32_t *offsets;
     const int32_t *list;       // heliostat ids of this launch
     float *flux;
+    double *acc;               // the launch's deposits (double; folded into flux after the launch)
     qmc_ray_record *records;   // nullable (one heliostat per launch when set)
     StatsAcc *stats;           // nullable
     const CullList *cull;      // per heliostat id: candidate quads of each traversal (null: no culling)

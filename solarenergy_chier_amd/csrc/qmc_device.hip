@@ -145,8 +145,12 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     o.status = QMC_RAY_DEPOSITED;
 }
 
-// One lane per ray slot; grid.y = heliostat of the list, grid.x = blocks of 256 slots. Deposits are
-// device-scope float atomics on the receiver's map (receiver_drawing, rectangleReceiverIntersection.cu:24).
+// One lane per ray slot; grid.y = heliostat of the list, grid.x = blocks of 256 slots. Deposits
+// (receiver_drawing, rectangleReceiverIntersection.cu:24) are device-scope atomics into the launch's
+// double accumulator, folded into the float map after the launch (k_acc_fold): a pixel can receive
+// tens of thousands of nearly equal deposits, and one float accumulator would round each against a
+// sum thousands of times larger (a bias near 3e-4 on the reference's example inputs); 64-bit atomics
+// run at the same rate as 32-bit ones on MI355X.
 template <int RT, bool REC, bool STATS>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const int h = a.list[blockIdx.y];
@@ -164,7 +168,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     if (!STATS && !active) return;
     if (active) {
         trace_ray<RT, STATS>(a, H, h, r, o, st);
-        if (o.status == QMC_RAY_DEPOSITED) unsafeAtomicAdd(a.flux + o.index, o.energy);
+        if (o.status == QMC_RAY_DEPOSITED) unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
         if (REC) {
             qmc_ray_record rec;
             rec.status = o.status;
@@ -199,9 +203,9 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
 // 21 G/s for scattered addresses on MI355X, against 180 G/s in LDS), and a large field deposits
 // hundreds of millions of rays into one map, so each workgroup keeps a band of map rows
 // [row0, row0 + rows) in LDS for the whole launch: deposits inside it are LDS atomics, the others
-// device-scope atomics on the map. Workgroups take BS-slot blocks (heliostat, block) round robin;
-// at the end each stores its band, with plain stores, to its own slice of `partials`, which
-// k_band_fold adds into the map.
+// device-scope atomics on the launch's double accumulator. Workgroups take BS-slot blocks
+// (heliostat, block) round robin; at the end each stores its band, with plain stores, to its own
+// slice of `partials`, which k_band_fold adds into the accumulator.
 template <int RT, int BS>
 __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int rows, int blocks_per_helio,
                                                     float *partials) {
@@ -225,21 +229,32 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
         if ((unsigned)br < (unsigned)rows)
             atomicAdd(band + br * W + o.col, o.energy);
         else
-            unsafeAtomicAdd(a.flux + o.index, o.energy);
+            unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
     }
     __syncthreads();
     float *mine = partials + (size_t)blockIdx.x * nband;
     for (int p = threadIdx.x; p < nband; p += BS) mine[p] = band[p];
 }
 
-// Adds the workgroups' bands into rows [row0, row0 + nband / W) of the map.
-__global__ __launch_bounds__(256) void k_band_fold(float *flux, const float *partials, int n_parts, int nband,
+// Adds the workgroups' bands (in double) into rows [row0, row0 + nband / W) of the accumulator.
+__global__ __launch_bounds__(256) void k_band_fold(double *acc, const float *partials, int n_parts, int nband,
                                                    int offset) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nband) return;
-    float s = 0.0f;
-    for (int w = 0; w < n_parts; ++w) s += partials[(size_t)w * nband + p];
-    flux[offset + p] += s;
+    double s = 0.0;
+    for (int w = 0; w < n_parts; ++w) s += (double)partials[(size_t)w * nband + p];
+    acc[offset + p] += s;
+}
+
+// Adds a launch's accumulator into the float map (one rounding per pixel) and clears it.
+__global__ __launch_bounds__(256) void k_acc_fold(float *flux, double *acc, int npix) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const double v = acc[p];
+    if (v != 0.0) {
+        flux[p] = (float)((double)flux[p] + v);
+        acc[p] = 0.0;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -714,6 +729,8 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         }
         if ((st = dev_alloc(rb.flux, (size_t)rb.w * rb.h))) return st;
         HIP_TRY(hipMemsetAsync(rb.flux, 0, sizeof(float) * (size_t)rb.w * rb.h, c->stream));
+        if ((st = dev_alloc(rb.acc, (size_t)rb.w * rb.h))) return st;
+        HIP_TRY(hipMemsetAsync(rb.acc, 0, sizeof(double) * (size_t)rb.w * rb.h, c->stream));
     }
     // grids
     c->grids.resize(s->grids.size());
@@ -1178,7 +1195,7 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
     if (a.recv.type == 0) launch_band<0>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
     else launch_band<1>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256)), dim3(256), 0, c->stream, rb.flux,
+    hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256)), dim3(256), 0, c->stream, rb.acc,
                        c->d_band_partials, nwg, (int)nband, row0 * W);
     HIP_TRY(hipGetLastError());
     launched = true;
@@ -1201,6 +1218,7 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     a.offsets = c->d_offsets;
     a.list = d_list;
     a.flux = rb.flux;
+    a.acc = rb.acc;
     a.records = rec;
     a.stats = stats;
     a.P = c->P;
@@ -1217,11 +1235,17 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
         const qmc_status st = launch_cull(c, grid, d_list, n, &a.cull);
         if (st) return st;
     }
+    const int npix = rb.w * rb.h;
     if (!rec && !stats) {
         bool launched = false;
         const qmc_status st = try_band_launch(c, rb, a, max_slots, launched);
         if (st) return st;
-        if (launched) return QMC_OK;
+        if (launched) {
+            hipLaunchKernelGGL(k_acc_fold, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, rb.flux,
+                               rb.acc, npix);
+            HIP_TRY(hipGetLastError());
+            return QMC_OK;
+        }
     }
     dim3 g((unsigned)bx, (unsigned)n), b(256);
     const int rt = rb.r.type;
@@ -1235,6 +1259,9 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
         if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, false>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL((k_trace<1, false, false>), g, b, 0, c->stream, a);
     }
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_acc_fold, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, rb.flux, rb.acc,
+                       npix);
     HIP_TRY(hipGetLastError());
     return QMC_OK;
 }
