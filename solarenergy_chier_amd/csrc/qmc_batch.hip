@@ -61,7 +61,7 @@ struct BatchArgs {
     DevFrame *fr;      // [instance][heliostat]
     float *fac;        // [instance][heliostat]
     float4 *quads;     // [instance][sub-heliostat][3]
-    uint32_t *cull;    // [instance][heliostat]: bit 0 shadow, bit 1 blocking traversal needed (null: both)
+    CullList *cull;    // [instance][heliostat]: candidate quads of each traversal (null: no culling)
     int32_t *queue;    // next instance to take (reset by k_batch_setup)
 };
 
@@ -104,19 +104,52 @@ __global__ __launch_bounds__(256) void k_batch_setup(BatchArgs a) {
 }
 
 // Traversal culling of every (instance, heliostat), one wavefront each, from the frames and quads
-// k_batch_setup wrote: the per-instance counterpart of the plain trace's k_cull. Bits left clear mark
-// shadow / blocking traversals that can only miss for that instance's sun and aim points.
+// k_batch_setup wrote: the per-instance counterpart of the plain trace's k_cull. The wave scans the
+// instance's quads 64 at a time and lists the candidates of each traversal (ballot and lane ranks
+// give each passing lane its slot); a kind stops being scanned once it has more than kCullList.
 __global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
     const int ii = blockIdx.y;
     const int k = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
     if (k >= a.n_helio) return;
     const DevHelio &H = a.helios[a.helio_begin + k];
     const DevCullShape S = flat_cull_shape(H, a.fr[(size_t)ii * a.n_helio + k]);
-    bool may_shadow = false, may_block = false;
-    cull_scan(S, a.sun_fr[ii], a.quads + (size_t)ii * a.n_sub * 3, a.n_sub, H.sub_begin, H.sub_begin + H.sub_count,
-              a.tan_sun, a.tan_block, kCullMargin, lane, 64, may_shadow, may_block);
-    const bool any_shadow = __ballot(may_shadow) != 0, any_block = __ballot(may_block) != 0;
-    if (lane == 0) a.cull[(size_t)ii * a.n_helio + k] = (any_shadow ? 1u : 0u) | (any_block ? 2u : 0u);
+    const DevFrame sunf = a.sun_fr[ii];
+    const float4 *quads = a.quads + (size_t)ii * a.n_sub * 3;
+    CullList *out = a.cull + (size_t)ii * a.n_helio + k;
+    const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
+    const CD3 c{S.c[0], S.c[1], S.c[2]}, hu{S.hu[0], S.hu[1], S.hu[2]}, hn{0, 0, 0}, hv{S.hv[0], S.hv[1], S.hv[2]},
+        nrm{S.n[0], S.n[1], S.n[2]};
+    const CD3 ash = sunf.identity ? CD3{0, 1, 0} : CD3{sunf.nx, sunf.ny, sunf.nz};
+    CD3 r0 = cadd(cmul(ash, -1.0), cmul(nrm, 2.0 * cdot(ash, nrm)));  // reflect(-sun axis, n)
+    r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
+    const double rs = sqrt(cdot(hu, hu) + cdot(hv, hv));
+    const uint64_t below = (1ull << lane) - 1ull;  // lanes ranked before this one
+    int ns = 0, nb = 0;                             // wave-uniform counts
+    for (int base = 0; base < a.n_sub && (ns <= kCullList || nb <= kCullList); base += 64) {
+        const int q = base + lane;
+        bool ps = false, pb = false;
+        if (q < a.n_sub && (q < own_lo || q >= own_hi)) {
+            const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
+            const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
+            const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
+            const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
+            const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
+            ps = ns <= kCullList && sphere_may_hit(D, R, ash, a.tan_sun, kCullMargin) &&
+                 cone_may_hit_n<4>(c, hu, hn, hv, ash, a.tan_sun, A, E1, E2, kCullMargin);
+            pb = nb <= kCullList && sphere_may_hit(D, R, r0, a.tan_block, kCullMargin) &&
+                 cone_may_hit_n<4>(c, hu, hn, hv, r0, a.tan_block, A, E1, E2, kCullMargin);
+        }
+        const uint64_t bs = __ballot(ps), bb = __ballot(pb);
+        const int rs_ = ns + __popcll(bs & below), rb_ = nb + __popcll(bb & below);
+        if (ps && rs_ < kCullList) out->ids[0][rs_] = q;
+        if (pb && rb_ < kCullList) out->ids[1][rb_] = q;
+        ns += __popcll(bs);
+        nb += __popcll(bb);
+    }
+    if (lane == 0) {
+        out->n[0] = ns;
+        out->n[1] = nb;
+    }
 }
 
 // Load of data that no kernel of the launch writes (per-heliostat and per-instance records) through
@@ -166,8 +199,7 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
             const float factor = ld_uniform(a.fac + (size_t)ii * a.n_helio + k);
             const int32_t *grp = a.offsets + H.offsets_base;
             const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
-            const uint32_t need = a.cull ? ld_uniform(a.cull + (size_t)ii * a.n_helio + k) : 3u;
-            const bool shadow_pass = (need & 1u) != 0, block_pass = (need & 2u) != 0;
+            const CullList *cl = a.cull ? a.cull + (size_t)ii * a.n_helio + k : nullptr;
             const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
             const FacetTiling tl = facet_tiling(R, C, L);
             const int nslots = TILES ? tl.slots : H.F * L;
@@ -181,12 +213,16 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
                 V3 dir = fr_apply(sunf, V3{s.x, s.y, s.z});
                 const V3 origin = flat_facet_origin(H, fr, m);
                 int hit;
-                if (shadow_pass && dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
+                if ((!cl || traversal_needed(*cl, 0, origin, dir, quads)) &&
+                    dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
+                    continue;
                 const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
                 const float4 q = a.pert[a2];
                 const V3 normal = normalize(fr_apply(fr, V3{q.x, q.y, q.z}));
                 dir = normalize(reflect(-dir, normal));
-                if (block_pass && dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st)) continue;
+                if ((!cl || traversal_needed(*cl, 1, origin, dir, quads)) &&
+                    dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
+                    continue;
                 float t, u, v;
                 if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v)) continue;
                 const float energy = ray_energy(t, dir, normal, factor);
@@ -257,7 +293,7 @@ size_t lds_bytes(const BatchArgs &a, bool lds_map) { return lds_map ? sizeof(flo
 
 // Per-instance data of one chunk, carved out of one ctx-owned allocation.
 size_t work_bytes_per_instance(const BatchArgs &a) {
-    return sizeof(DevFrame) * (size_t)(a.n_helio + 1) + (sizeof(float) + sizeof(uint32_t)) * a.n_helio +
+    return sizeof(DevFrame) * (size_t)(a.n_helio + 1) + (sizeof(float) + sizeof(CullList)) * a.n_helio +
            sizeof(float4) * 3 * a.n_sub;
 }
 
@@ -266,7 +302,7 @@ qmc_status ensure_work(qmc_ctx *c, BatchArgs &a, int chunk) {
     const size_t fac = sizeof(float) * (size_t)chunk * a.n_helio;
     const size_t quads = sizeof(float4) * 3 * (size_t)chunk * a.n_sub;
     const size_t sun = sizeof(DevFrame) * (size_t)chunk;
-    const size_t cull = sizeof(uint32_t) * (size_t)chunk * a.n_helio;
+    const size_t cull = sizeof(CullList) * (size_t)chunk * a.n_helio;
     auto up = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t total = up(quads) + up(fr) + up(fac) + up(sun) + up(cull) + 256;
     if (total > c->batch_work_cap) {
@@ -287,7 +323,7 @@ qmc_status ensure_work(qmc_ctx *c, BatchArgs &a, int chunk) {
     a.sun_fr = reinterpret_cast<DevFrame *>(p);
     p += up(sun);
     // culling needs both cone tangents; without them every traversal runs
-    a.cull = (a.tan_sun >= 0.0 && a.tan_block >= 0.0) ? reinterpret_cast<uint32_t *>(p) : nullptr;
+    a.cull = (a.tan_sun >= 0.0 && a.tan_block >= 0.0) ? reinterpret_cast<CullList *>(p) : nullptr;
     p += up(cull);
     a.queue = reinterpret_cast<int32_t *>(p);
     return QMC_OK;

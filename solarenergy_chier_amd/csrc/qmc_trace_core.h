@@ -174,45 +174,14 @@ __host__ __device__ inline DevCullShape flat_cull_shape(const DevHelio &H, const
     return s;
 }
 
-// One thread's share of the traversal-culling scan of a flat heliostat (culling shape S, hn = 0; own
-// quads [own_lo, own_hi)) against the quads q = first, first + step, ... of a grid: sets may_shadow /
-// may_block when some other quad can meet its shadow cone (tan_sun around the sun axis, the n axis of
-// sunf) or its blocking cone (tan_block around the sun axis reflected in the heliostat normal), per
-// cone_may_hit. The caller ORs the flags over its threads; a traversal whose flag stays clear can
-// only miss.
-__device__ __forceinline__ void cull_scan(const DevCullShape &S, const DevFrame &sunf, const float4 *quads,
-                                          int n_sub, int own_lo, int own_hi, double tan_sun, double tan_block,
-                                          double margin, int first, int step, bool &may_shadow, bool &may_block) {
-    if (may_shadow && may_block) return;
-    const CD3 c{S.c[0], S.c[1], S.c[2]}, hu{S.hu[0], S.hu[1], S.hu[2]}, hn{S.hn[0], S.hn[1], S.hn[2]},
-        hv{S.hv[0], S.hv[1], S.hv[2]}, n{S.n[0], S.n[1], S.n[2]};
-    const CD3 ash = sunf.identity ? CD3{0, 1, 0} : CD3{sunf.nx, sunf.ny, sunf.nz};
-    CD3 r0 = cadd(cmul(ash, -1.0), cmul(n, 2.0 * cdot(ash, n)));  // reflect(-sun axis, n)
-    r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
-    const double rs = sqrt(cdot(hu, hu) + cdot(hn, hn) + cdot(hv, hv));  // source box radius
-    for (int q = first; q < n_sub; q += step) {
-        if (q >= own_lo && q < own_hi) continue;
-        const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
-        const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
-        // bounding spheres first (most quads of a field are far from both cones), then the full test
-        const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
-        const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
-        const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
-        if (!may_shadow && sphere_may_hit(D, R, ash, tan_sun, margin) &&
-            cone_may_hit_n<4>(c, hu, hn, hv, ash, tan_sun, A, E1, E2, margin))
-            may_shadow = true;
-        if (!may_block && sphere_may_hit(D, R, r0, tan_block, margin) &&
-            cone_may_hit_n<4>(c, hu, hn, hv, r0, tan_block, A, E1, E2, margin))
-            may_block = true;
-        if (may_shadow && may_block) break;
-    }
-}
-
-// The scan of cull_scan, keeping the candidates: every quad that passes the bound for the shadow
-// (kind 0) or blocking (kind 1) cone is counted in n[kind] and, while the count is at most kCullList,
-// listed in ids[kind]; a negative tangent skips that kind. n and ids are shared by the workgroup (LDS,
-// n zeroed beforehand). A kind stops being scanned once its count exceeds kCullList (it is then
-// traversed for every ray and its list is not used), so a final count <= kCullList is complete.
+// One thread's share of the traversal-culling scan of a heliostat (culling shape S, own quads
+// [own_lo, own_hi)) over the quads q = first, first + step, ... of a grid: every quad that passes the
+// bound (sphere_may_hit, then cone_may_hit) for the shadow cone (kind 0: tan_sun around the sun axis,
+// the n axis of sunf) or the blocking cone (kind 1: tan_block around the sun axis reflected in the
+// heliostat normal) is counted in n[kind] and, while the count is at most kCullList, listed in
+// ids[kind]; a negative tangent skips that kind. n and ids are shared by the workgroup (LDS, n zeroed
+// beforehand). A kind stops being scanned once its count exceeds kCullList (it is then traversed for
+// every ray and its list is not used), so a final count <= kCullList is complete.
 __device__ __forceinline__ void cull_collect(const DevCullShape &S, const DevFrame &sunf, const float4 *quads,
                                              int n_sub, int own_lo, int own_hi, double tan_sun, double tan_block,
                                              double margin, int first, int step, int *n, int (*ids)[kCullList]) {
