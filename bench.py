@@ -4,8 +4,9 @@ on a bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ps10|small] [--no-cpu-baseline]
 
-A step = clear the flux map + trace every heliostat of the field (one kernel launch) with the
-sample pools and micro-facet offsets resident in HBM. Prints ONE JSON line on stdout.
+A step = clear the flux map + trace every heliostat of the field (one trace call: the culling
+kernel, the trace kernel and its folds) with the sample pools and micro-facet offsets resident in
+HBM. Prints ONE JSON line on stdout.
 """
 from __future__ import annotations
 
@@ -117,13 +118,27 @@ def main() -> int:
     achieved = alg_bytes / (kernel_avg_ms / 1e3) / 1e9
     # HBM bytes per launch and VALU busy fraction of this kernel, from the committed rocprofv3 PMC
     # summaries of the same workload (profiles/trace_<workload>_{hbm_traffic,valu}.md)
-    traffic = valu_busy = None
+    traffic = valu_busy = atom = None
     tfile = ROOT / "profiles" / f"trace_{args.workload}_hbm_traffic.json"
     vfile = ROOT / "profiles" / f"trace_{args.workload}_valu.json"
+    afile = ROOT / "profiles" / f"trace_{args.workload}_atomics.json"
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
     if vfile.exists():
         valu_busy = json.loads(vfile.read_text()).get("valu_busy")
+    if afile.exists():
+        atom = json.loads(afile.read_text())
+    # deposit rates of the trace against the device's measured atomic ceilings (capability probe)
+    ceil = dev.probe_atomic_rates()
+    log(f"atomic ceilings (G ops/s): {ceil}")
+    atomics = {"ceiling_gops": ceil}
+    if atom:
+        lds_rate = atom["lds_atomic_lanes_per_launch"] / (kernel_avg_ms / 1e3) / 1e9
+        glob_rate = atom["global_atomics_per_launch"] / (kernel_avg_ms / 1e3) / 1e9
+        atomics.update({"lds_atomic_lanes_per_launch": atom["lds_atomic_lanes_per_launch"],
+                        "global_atomics_per_launch": atom["global_atomics_per_launch"],
+                        "lds_gops": lds_rate, "lds_frac": lds_rate / ceil["lds_f32"],
+                        "global_gops": glob_rate, "global_frac": glob_rate / ceil["global_f64"]})
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -174,8 +189,9 @@ def main() -> int:
             "kernel_avg_ms": kernel_avg_ms,
             "algorithmic_bytes_per_launch": alg_bytes,
             "hbm_gbs_measured": traffic / (kernel_avg_ms / 1e3) / 1e9 if traffic else None,
-            "binding_resource": "VALU issue",
+            "binding_resource": "latency of the persistent loop; device-scope atomics outside the LDS band",
             "valu_busy": valu_busy,
+            "atomics": atomics,
         },
         "cpu_baseline": cpu,
         "batched": batched,

@@ -304,6 +304,15 @@ qmc_status qmc_facets_get(qmc_ctx *ctx, int32_t heliostat, float *origins, float
  * RectangleReceiverRectangleGridRayTracing.cu:45-57, CylinderReceiverRectangleGridRayTracing.cu:44-57.
  * Heliostats of several grids are traced grid by grid. Asynchronous. */
 qmc_status qmc_trace(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n);
+/* Rates (G operations/s) of the atomic adds the tracer deposits with, measured on the ctx's device:
+ * float and 64-bit device-scope adds scattered over a 65,536-entry map, float LDS adds scattered over
+ * a 64 KB tile. A capability probe (about 0.1 s) to state the trace's deposit rates against; no
+ * counterpart in the reference. Synchronous. */
+typedef struct qmc_atomic_rates {
+    double global_f32, global_f64, lds_f32;
+} qmc_atomic_rates;
+qmc_status qmc_probe_atomic_rates(qmc_ctx *ctx, qmc_atomic_rates *rates);
+
 /* Traversal-culling flags the trace uses for the listed heliostats under the current pools and
  * scene: bit 0 set = the shadow traversal can meet another quad, bit 1 set = the blocking traversal
  * can; a clear bit means every ray's traversal is proven to miss and the trace skips it (results are

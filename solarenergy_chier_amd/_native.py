@@ -96,6 +96,10 @@ SUNSHAPE_REFERENCE = 0
 SUNSHAPE_BUIE = 1
 
 
+class AtomicRates(C.Structure):
+    _fields_ = [("global_f32", C.c_double), ("global_f64", C.c_double), ("lds_f32", C.c_double)]
+
+
 class Instance(C.Structure):
     _fields_ = [("sun_dir", C.c_float * 3), ("dni", C.c_float), ("aim_set", C.c_int32), ("reserved", C.c_int32)]
 
@@ -166,6 +170,7 @@ SIGNATURES = {
     "qmc_facets_get": [_P, _I, _FP, _FP, _I],
     "qmc_trace": [_P, _IP, _I],
     "qmc_cull_flags": [_P, _IP, _I, C.POINTER(C.c_uint8)],
+    "qmc_probe_atomic_rates": [_P, C.POINTER(AtomicRates)],
     "qmc_trace_records": [_P, _I, C.POINTER(RayRecord), C.c_int64],
     "qmc_trace_stats_run": [_P, _IP, _I, C.POINTER(TraceStats)],
     "qmc_flux_clear": [_P, _I],

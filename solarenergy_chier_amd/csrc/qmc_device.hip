@@ -235,14 +235,17 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
     for (int p = threadIdx.x; p < nband; p += BS) mine[p] = band[p];
 }
 
-// Adds the workgroups' bands (in double) into rows [row0, row0 + nband / W) of the accumulator.
+// Adds the workgroups' bands (in double) into rows [row0, row0 + nband / W) of the accumulator;
+// grid.y splits the workgroups into slices, each slice adding its sum with one 64-bit atomic.
 __global__ __launch_bounds__(256) void k_band_fold(double *acc, const float *partials, int n_parts, int nband,
                                                    int offset) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nband) return;
+    const int per = (n_parts + gridDim.y - 1) / gridDim.y;
+    const int w0 = blockIdx.y * per, w1 = min(n_parts, w0 + per);
     double s = 0.0;
-    for (int w = 0; w < n_parts; ++w) s += (double)partials[(size_t)w * nband + p];
-    acc[offset + p] += s;
+    for (int w = w0; w < w1; ++w) s += (double)partials[(size_t)w * nband + p];
+    if (s != 0.0) unsafeAtomicAdd(acc + offset + p, s);
 }
 
 // Adds a launch's accumulator into the float map (one rounding per pixel) and clears it.
@@ -1108,7 +1111,7 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
     if (a.recv.type == 0) launch_band<0>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
     else launch_band<1>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256)), dim3(256), 0, c->stream, rb.acc,
+    hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256), 8), dim3(256), 0, c->stream, rb.acc,
                        c->d_band_partials, nwg, (int)nband, row0 * W);
     HIP_TRY(hipGetLastError());
     launched = true;

@@ -363,6 +363,12 @@ class Device:
         a = np.ascontiguousarray(list(ids), np.int32)
         N.call("qmc_trace", self.handle, N.iptr(a), len(a))
 
+    def probe_atomic_rates(self) -> dict:
+        """Measured rates (G ops/s) of scattered float / 64-bit global and float LDS atomic adds."""
+        r = N.AtomicRates()
+        N.call("qmc_probe_atomic_rates", self.handle, C.byref(r))
+        return {"global_f32": r.global_f32, "global_f64": r.global_f64, "lds_f32": r.lds_f32}
+
     def cull_flags(self, ids: Iterable[int]) -> np.ndarray:
         """Per heliostat: bit 0 = shadow traversal needed, bit 1 = blocking traversal needed."""
         a = np.ascontiguousarray(list(ids), np.int32)

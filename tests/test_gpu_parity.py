@@ -378,6 +378,15 @@ def test_cylinder_field_full_trace_equals_counting_pass(device):
     assert err < FLUX_RTOL_L1 and got.sum() > 0
 
 
+def test_atomic_rate_probe(device):
+    """The capability probe returns positive rates, and LDS atomics outrun device-scope ones (the
+    reason the trace accumulates its deposits in LDS bands)."""
+    r = device.probe_atomic_rates()
+    print("atomic rates (G ops/s)", r, flush=True)
+    assert all(v > 0 for v in r.values())
+    assert r["lds_f32"] > r["global_f32"]
+
+
 def test_offsets_set_validates_range(device):
     """Group offsets index the sample pool, so values outside [0, P] are refused before any launch."""
     wl = workloads.small_field(L=8, groups=16)
