@@ -172,12 +172,12 @@ __global__ __launch_bounds__(256) void k_adj_forward(AdjArgs a) {
         int hit;
         const int lo = H.sub_begin, hi = H.sub_begin + H.sub_count;
         // traversals proved hit-free for this ray are skipped (same visibility)
-        if (!((!a.cull || traversal_needed(a.cull[h], 0, origin, dir, a.grid.quads)) &&
+        if (!((!a.cull || traversal_needed(a.cull + h, 0, origin, dir, a.grid.quads)) &&
               dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
             const float4 q = a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)];
             const V3 normal = normalize(fr_apply(H.fr, V3{q.x, q.y, q.z}));
             dir = normalize(reflect(-dir, normal));
-            if (!((!a.cull || traversal_needed(a.cull[h], 1, origin, dir, a.grid.quads)) &&
+            if (!((!a.cull || traversal_needed(a.cull + h, 1, origin, dir, a.grid.quads)) &&
                   dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
                 float t, u, v;
                 if (receiver_hit<0>(a.recv, origin, dir, t, u, v)) {

@@ -152,20 +152,6 @@ __global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
     }
 }
 
-// Load of data that no kernel of the launch writes (per-heliostat and per-instance records) through
-// the constant address space: with a wave-uniform address it becomes a scalar load into SGPRs.
-template <class T>
-__device__ __forceinline__ T ld_uniform(const T *p) {
-    static_assert(sizeof(T) % 4 == 0, "records are whole dwords");
-    typedef const __attribute__((address_space(4))) uint32_t cdword;
-    cdword *src = reinterpret_cast<cdword *>(reinterpret_cast<uintptr_t>(p));
-    T out;
-    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
-#pragma unroll
-    for (size_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
-    return out;
-}
-
 // 1024-thread workgroups are built for two per CU (8 waves per SIMD, at most 64 VGPRs): with a 64 KB LDS
 // map each, that is the only way an LDS-accumulated instance runs at full occupancy.
 template <int RT, bool LDSMAP, bool TILES, int BS>
@@ -213,14 +199,14 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
                 V3 dir = fr_apply(sunf, V3{s.x, s.y, s.z});
                 const V3 origin = flat_facet_origin(H, fr, m);
                 int hit;
-                if ((!cl || traversal_needed(*cl, 0, origin, dir, quads)) &&
+                if ((!cl || traversal_needed(cl, 0, origin, dir, quads)) &&
                     dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
                     continue;
                 const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
                 const float4 q = a.pert[a2];
                 const V3 normal = normalize(fr_apply(fr, V3{q.x, q.y, q.z}));
                 dir = normalize(reflect(-dir, normal));
-                if ((!cl || traversal_needed(*cl, 1, origin, dir, quads)) &&
+                if ((!cl || traversal_needed(cl, 1, origin, dir, quads)) &&
                     dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
                     continue;
                 float t, u, v;

@@ -83,8 +83,8 @@ struct RayOut {
 // One ray of map_raytracing (RectangleReceiverRectangleGridRayTracing.cu:6-41,
 // CylinderReceiverRectangleGridRayTracing.cu:6-40) up to, not including, its deposit.
 template <int RT, bool STATS>
-__device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H, int h, long long r, RayOut &o,
-                                          RayStats &st) {
+__device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H, const CullList *cl, long long r,
+                                          RayOut &o, RayStats &st) {
     const int L = a.L;
     const int m = (int)fast_div((uint32_t)r, a.fdL);
     const int j = (int)r - m * L;
@@ -103,12 +103,12 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     const float4 s = a.pool[a1];
     V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
     V3 origin, fnormal;
-    const DevFrame *pf;
+    DevFrame pf;
     facet_geometry(H, a.facets, m, origin, pf, fnormal);
     int hit = -1;
     // traversals proved hit-free for this ray are skipped (never while counting work)
-    const bool culling = !STATS && a.cull;
-    if ((!culling || traversal_needed(a.cull[h], 0, origin, dir, a.grid.quads)) &&
+    const bool culling = !STATS && cl;
+    if ((!culling || traversal_needed(cl, 0, origin, dir, a.grid.quads)) &&
         dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
         o.status = QMC_RAY_SHADOWED;
         o.index = hit;
@@ -118,9 +118,9 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     // Step 2: perturbed facet normal, reflection, blocking (:24-36)
     const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
     const float4 q = a.pert[a2];
-    V3 normal = normalize(fr_apply(*pf, V3{q.x, q.y, q.z}));
+    V3 normal = normalize(fr_apply(pf, V3{q.x, q.y, q.z}));
     dir = normalize(reflect(-dir, normal));
-    if ((!culling || traversal_needed(a.cull[h], 1, origin, dir, a.grid.quads)) &&
+    if ((!culling || traversal_needed(cl, 1, origin, dir, a.grid.quads)) &&
         dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
         o.status = QMC_RAY_BLOCKED;
         o.index = hit;
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const bool active = r >= 0;
     if (!STATS && !active) return;
     if (active) {
-        trace_ray<RT, STATS>(a, H, h, r, o, st);
+        trace_ray<RT, STATS>(a, H, a.cull ? a.cull + h : nullptr, r, o, st);
         if (o.status == QMC_RAY_DEPOSITED) unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
         if (REC) {
             qmc_ray_record rec;
@@ -212,23 +212,33 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
     const int W = a.recv.res_x, nband = rows * W;
     for (int p = threadIdx.x; p < nband; p += BS) band[p] = 0.0f;
     __syncthreads();
-    const long long items = (long long)a.n_list * blocks_per_helio;
     RayStats st{0u, 0u, 0u};
+    // block (li, bx) of the launch, advanced by gridDim.x blocks per step; the heliostat id is made
+    // explicitly wave-uniform so that its record is read with scalar loads, as in k_trace
+    int li = (int)(blockIdx.x / (unsigned)blocks_per_helio);
+    int bx = (int)blockIdx.x - li * blocks_per_helio;
 #pragma nounroll
-    for (long long it = blockIdx.x; it < items; it += gridDim.x) {
-        const int li = (int)(it / blocks_per_helio), bx = (int)(it - (long long)li * blocks_per_helio);
-        const int h = a.list[li];
-        const DevHelio &H = a.helios[h];
-        const long long r = slot_ray(H, a.L, (long long)bx * BS + threadIdx.x);
-        if (r < 0) continue;
-        RayOut o;
-        trace_ray<RT, false>(a, H, h, r, o, st);
-        if (o.status != QMC_RAY_DEPOSITED) continue;
-        const int br = o.row - row0;
-        if ((unsigned)br < (unsigned)rows)
-            atomicAdd(band + br * W + o.col, o.energy);
-        else
-            unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
+    while (li < a.n_list) {
+        // the launch writes none of these records, so they are read with scalar loads
+        const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
+        const DevHelio H = ld_uniform(a.helios + h);
+        const long long r = slot_ray(H, a.L, (long long)__builtin_amdgcn_readfirstlane(bx) * BS + threadIdx.x);
+        if (r >= 0) {
+            RayOut o;
+            trace_ray<RT, false>(a, H, a.cull ? a.cull + h : nullptr, r, o, st);
+            if (o.status == QMC_RAY_DEPOSITED) {
+                const int br = o.row - row0;
+                if ((unsigned)br < (unsigned)rows)
+                    atomicAdd(band + br * W + o.col, o.energy);
+                else
+                    unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
+            }
+        }
+        bx += (int)gridDim.x;
+        while (bx >= blocks_per_helio) {
+            bx -= blocks_per_helio;
+            ++li;
+        }
     }
     __syncthreads();
     float *mine = partials + (size_t)blockIdx.x * nband;
@@ -382,7 +392,7 @@ __global__ __launch_bounds__(256) void k_facets(const DevHelio *helios, const De
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= H.F) return;
     V3 o, n;
-    const DevFrame *pf;
+    DevFrame pf;
     facet_geometry(H, facets, m, o, pf, n);
     origins[3 * m] = o.x;
     origins[3 * m + 1] = o.y;

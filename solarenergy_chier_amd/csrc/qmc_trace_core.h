@@ -216,18 +216,35 @@ __device__ __forceinline__ void cull_collect(const DevCullShape &S, const DevFra
     }
 }
 
+// Load of data that no kernel of the launch writes (per-heliostat and per-instance records) through
+// the constant address space: with a wave-uniform address it becomes a scalar load into SGPRs.
+template <class T>
+__device__ __forceinline__ T ld_uniform(const T *p) {
+    static_assert(sizeof(T) % 4 == 0, "records are whole dwords");
+    typedef const __attribute__((address_space(4))) uint32_t cdword;
+    cdword *src = reinterpret_cast<cdword *>(reinterpret_cast<uintptr_t>(p));
+    T out;
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&out);
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 4; ++i) dst[i] = src[i];
+    return out;
+}
+
 // Whether a traversal has to run for this ray, given its heliostat's candidate list for that
 // traversal: not when no quad can be met (count 0) or when the ray misses every listed candidate —
 // the traversal tests only quads, the others provably miss, and a listed one gives the same
 // ray_quad result here as there. Otherwise (a candidate is hit, or too many to list) the traversal
 // runs and decides, with the reference's first-hit order.
-__device__ __forceinline__ bool traversal_needed(const CullList &cl, int kind, V3 o, V3 d, const float4 *quads) {
-    const int n = cl.n[kind];
+// The list is wave-uniform and written by an earlier launch: its fields are read with scalar loads.
+__device__ __forceinline__ bool traversal_needed(const CullList *cl, int kind, V3 o, V3 d, const float4 *quads) {
+    const int n = ld_uniform(&cl->n[kind]);
     if (n == 0) return false;
     if (n > kCullList) return true;
     for (int i = 0; i < n; ++i) {
-        const int id = cl.ids[kind][i];
-        const float4 qa = quads[3 * id], qb = quads[3 * id + 1], qc = quads[3 * id + 2];
+        const int id = ld_uniform(&cl->ids[kind][i]);
+        // the candidates are wave-uniform and no kernel writes the quads: scalar loads
+        const float4 qa = ld_uniform(quads + 3 * id), qb = ld_uniform(quads + 3 * id + 1),
+                     qc = ld_uniform(quads + 3 * id + 2);
         float t, u, v;
         if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) return true;
     }
@@ -242,16 +259,16 @@ constexpr double kCullMaxAngle = 80.0 * 3.14159265358979323846 / 180.0;
 constexpr double kCullMargin = 0.01;
 
 // Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
-// onto the facet normal.
+// onto the facet normal (returned by value: H may be a copy in registers).
 __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
-                                               const DevFrame *&pert_frame, V3 &normal) {
+                                               DevFrame &pert_frame, V3 &normal) {
     const int cpr = H.cols * H.sub_cols;
     int row = m / cpr;
     int col = m % cpr;
     const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
     if (H.type == 0) {
         origin = flat_facet_origin(H, H.fr, m);
-        pert_frame = &H.fr;
+        pert_frame = H.fr;
         normal = V3{H.fr.nx, H.fr.ny, H.fr.nz};
     } else {
         // FocusFlatRectangleHelio.cu:21-39,53-64
@@ -263,7 +280,7 @@ __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet
         V3 local{(col + 0.5f) * pl - H.len_col / 2, 0.0f, (row + 0.5f) * pl - H.len_row / 2};
         local = fr_apply(Fc.fr_local, local) + V3{Fc.center[0], Fc.center[1], Fc.center[2]};
         origin = fr_apply(H.fr_focus, local) + pos;
-        pert_frame = &Fc.fr_pert;
+        pert_frame = Fc.fr_pert;
         normal = V3{Fc.normal[0], Fc.normal[1], Fc.normal[2]};
     }
 }
