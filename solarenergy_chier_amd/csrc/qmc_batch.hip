@@ -194,16 +194,17 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
                 const int r = TILES ? facet_tiled_ray(tl, tile_divisor(H, tl), R, C, slot) : slot;
                 if (r < 0) continue;
                 const int m = (int)fast_div((uint32_t)r, a.fdL), j = r - m * L;
+                // both samples are read before the tests: two dependent load pairs, not a chain of four
                 const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
+                const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
                 const float4 s = a.pool[a1];
+                const float4 q = a.pert[a2];
                 V3 dir = fr_apply(sunf, V3{s.x, s.y, s.z});
                 const V3 origin = flat_facet_origin(H, fr, m);
                 int hit;
                 if ((!cl || traversal_needed(cl, 0, origin, dir, quads)) &&
                     dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
                     continue;
-                const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
-                const float4 q = a.pert[a2];
                 const V3 normal = normalize(fr_apply(fr, V3{q.x, q.y, q.z}));
                 dir = normalize(reflect(-dir, normal));
                 if ((!cl || traversal_needed(cl, 1, origin, dir, quads)) &&

@@ -99,8 +99,12 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     o.first_of_facet = j == 0;
 
     // Step 1: incident direction and shadowing (RectangleReceiverRectangleGridRayTracing.cu:14-22)
+    // both pool indices and samples are read before any test, so the four loads form two
+    // dependent pairs instead of a chain of four
     const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
+    const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
     const float4 s = a.pool[a1];
+    const float4 q = a.pert[a2];
     V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
     V3 origin, fnormal;
     DevFrame pf;
@@ -116,8 +120,6 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     }
     o.unshadowed = true;
     // Step 2: perturbed facet normal, reflection, blocking (:24-36)
-    const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
-    const float4 q = a.pert[a2];
     V3 normal = normalize(fr_apply(pf, V3{q.x, q.y, q.z}));
     dir = normalize(reflect(-dir, normal));
     if ((!culling || traversal_needed(cl, 1, origin, dir, a.grid.quads)) &&
