@@ -531,7 +531,7 @@ extern "C" qmc_status qmc_adjoint_forward(qmc_ctx *c, int32_t grid, float *d_flu
     HIP_TRY(hipMemsetAsync(d_flux, 0, sizeof(float) * (size_t)rb.w * rb.h, c->stream));
     HIP_TRY(hipMemsetAsync(s->d_mask, 0, sizeof(uint32_t) * s->mask_words, c->stream));
     AdjArgs a = adj_args(c, s, d_flux);
-    if ((st = launch_cull(c, grid, s->d_list, (int)s->list.size(), &a.cull))) return st;
+    if ((st = launch_cull(c, grid, s->d_list, (int)s->list.size(), &a.cull, nullptr))) return st;
     hipLaunchKernelGGL(k_adj_forward, adj_grid(c, s), dim3(256), 0, c->stream, a);
     HIP_TRY(hipGetLastError());
     s->mask_ready = true;

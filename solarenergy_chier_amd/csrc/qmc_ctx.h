@@ -83,6 +83,10 @@ struct qmc_ctx {
     qmc::CullList *d_cull = nullptr;
     qmc::DevCullShape *d_cull_shapes = nullptr;
     size_t cull_cap = 0;
+    qmc::WideList *d_wide = nullptr;  // wide candidate lists per heliostat (plain trace)
+    size_t wide_cap = 0;
+    uint64_t *d_wmask = nullptr;      // per-wave candidate masks of the last plain trace launch
+    size_t wmask_cap = 0;
     // adjoint
     void *adj = nullptr;
 };
@@ -130,7 +134,8 @@ inline DevFrame to_dev(const Frame &f) {
 void qmc_adjoint_release(qmc_ctx *c);  // qmc_adjoint.hip
 // Traversal-culling candidate lists (CullList, indexed by heliostat id) of the listed heliostats of a
 // grid, computed on the ctx stream (qmc_device.hip).
-qmc_status launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists);
+qmc_status launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists,
+                       const WideList **wide);
 
 inline void free_scene(qmc_ctx *c) {
     for (auto &g : c->grids) {

@@ -109,6 +109,17 @@ struct CullList {
     int32_t ids[2][kCullList];
 };
 
+// The same candidates with room for kCullWide ids, for the plain trace: each wave of slots then keeps,
+// in a bit mask over these ids, the candidates that its own rays may meet (k_cull_waves).
+constexpr int kCullWide = 64;
+struct WideList {
+    int32_t n[2];
+    int32_t ids[2][kCullWide];
+};
+
+// Largest number of candidates a ray tests itself; a wave with more runs the traversal instead.
+constexpr int kRayTests = 8;
+
 struct CullArgs {
     const DevHelio *helios;
     const DevCullShape *shapes;
@@ -120,6 +131,23 @@ struct CullArgs {
     double ang_block;     // half-angle of the blocking cone of a flat heliostat (negative: no culling)
     double margin;
     CullList *lists;      // per heliostat id
+    WideList *wide;       // per heliostat id (nullable: collect for `lists` only)
+};
+
+// Per-wave candidate masks of the plain trace: wave w of list entry li (slots [64w, 64w + 64) of
+// heliostat list[li]) gets masks[2 * (li * wstride + w) + k], bit i set when ray origins of that
+// wave and directions of traversal k may meet wide[h].ids[k][i].
+struct WaveArgs {
+    const DevHelio *helios;
+    const DevFacet *facets;
+    const int32_t *list;
+    const WideList *wide;
+    const float4 *quads;
+    DevFrame sun;
+    double tan_sun, tan_block;  // cone tangents (negative: that traversal is not culled)
+    double margin;
+    int32_t n_list, L, wstride;
+    uint64_t *masks;
 };
 
 struct TraceArgs {
@@ -136,7 +164,9 @@ struct TraceArgs {
     double *acc;               // the launch's deposits (double; folded into flux after the launch)
     qmc_ray_record *records;   // nullable (one heliostat per launch when set)
     StatsAcc *stats;           // nullable
-    const CullList *cull;      // per heliostat id: candidate quads of each traversal (null: no culling)
+    const WideList *wide;      // per heliostat id: candidate quads of each traversal (null: no culling)
+    const uint64_t *wmask;     // per-wave candidate masks (WaveArgs), with wide
+    int32_t wstride;           // waves per list entry in wmask
     int32_t P, L;
     FastDiv fdP, fdL;
     int32_t n_list;

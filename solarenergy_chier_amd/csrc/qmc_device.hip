@@ -36,10 +36,11 @@ namespace {
 // pool's half-angle of the sun axis) or of its blocking cone (within sun shape + twice the slope
 // error of the ideal reflection) can meet another quad of the grid (cone_may_hit, conservative).
 // When none can, every test of that traversal is a miss, so skipping it returns the same result for
-// every ray. Flat heliostats only; others always traverse.
+// every ray. Candidates are listed up to kCullList (CullList) or, when a.wide is set, up to kCullWide
+// (WideList, for the per-wave masks of the plain trace).
 __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
     __shared__ int s_n[2];
-    __shared__ int s_ids[2][kCullList];
+    __shared__ int s_ids[2][kCullWide];
     const int h = a.list[blockIdx.x];
     const DevHelio &H = a.helios[h];
     const DevCullShape S = a.shapes[h];
@@ -49,8 +50,17 @@ __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
     if (threadIdx.x < 2) s_n[threadIdx.x] = 0;
     __syncthreads();
     cull_collect(S, a.sun, a.quads, a.n_sub, H.sub_begin, H.sub_begin + H.sub_count, a.tan_sun, tan_block, a.margin,
-                 threadIdx.x, blockDim.x, s_n, s_ids);
+                 threadIdx.x, blockDim.x, a.wide ? kCullWide : kCullList, s_n, s_ids);
     __syncthreads();
+    if (a.wide) {
+        WideList *W = a.wide + h;
+        const int nw0 = a.tan_sun < 0.0 ? kCullWide + 1 : s_n[0], nw1 = tan_block < 0.0 ? kCullWide + 1 : s_n[1];
+        if (threadIdx.x < 2) W->n[threadIdx.x] = threadIdx.x ? nw1 : nw0;
+        for (int t = threadIdx.x; t < 2 * kCullWide; t += blockDim.x) {
+            const int k = t / kCullWide, i = t - k * kCullWide;
+            W->ids[k][i] = i < (k ? nw1 : nw0) ? s_ids[k][i] : -1;
+        }
+    }
     if (threadIdx.x == 0) {
         CullList L;
         // a cone that cannot be bounded keeps its traversal for every ray
@@ -60,6 +70,166 @@ __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
             for (int i = 0; i < kCullList; ++i) L.ids[k][i] = i < L.n[k] ? s_ids[k][i] : -1;
         a.lists[h] = L;
     }
+}
+
+// Axes of a frame in double (u, n, v: where local x, y, z go); the identity frame gives the world axes.
+__device__ __forceinline__ void frame_axes(const DevFrame &f, CD3 &u, CD3 &n, CD3 &v) {
+    u = f.identity ? CD3{1, 0, 0} : CD3{f.ux, f.uy, f.uz};
+    n = f.identity ? CD3{0, 1, 0} : CD3{f.nx, f.ny, f.nz};
+    v = f.identity ? CD3{0, 0, 1} : CD3{f.vx, f.vy, f.vz};
+}
+
+// Sets, in m[k], the bit of each wide-list candidate of traversal k (n[k] of them, 0 < n[k] <=
+// kCullWide) that rays starting in the parallelogram c +- hu +- hv may meet: shadow rays within the
+// sun-shape cone of the sun axis, blocking rays within the sun shape plus twice the slope error of the
+// sun axis reflected about nrm. The same two tests as k_cull (sphere_may_hit, then cone_may_hit), with
+// the wave's origins in place of the heliostat's.
+__device__ void wave_shape_bits(const WaveArgs &a, const WideList *wl, const int n[2], CD3 c, CD3 hu, CD3 hv,
+                                CD3 nrm, uint64_t m[2]) {
+    const CD3 zero{0, 0, 0};
+    const CD3 ash = a.sun.identity ? CD3{0, 1, 0} : CD3{a.sun.nx, a.sun.ny, a.sun.nz};
+    nrm = cmul(nrm, 1.0 / sqrt(cdot(nrm, nrm)));
+    CD3 r0 = cadd(cmul(ash, -1.0), cmul(nrm, 2.0 * cdot(ash, nrm)));  // reflect(-sun axis, nrm)
+    r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
+    const double rs = sqrt(cdot(hu, hu) + cdot(hv, hv));
+    for (int k = 0; k < 2; ++k) {
+        if (n[k] <= 0 || n[k] > kCullWide) continue;
+        const CD3 ax = k == 0 ? ash : r0;
+        const double tn = k == 0 ? a.tan_sun : a.tan_block;
+        for (int i = 0; i < n[k]; ++i) {
+            if ((m[k] >> i) & 1ull) continue;
+            const int q = ld_uniform(&wl->ids[k][i]);
+            const float4 qa = ld_uniform(a.quads + 3 * q), qb = ld_uniform(a.quads + 3 * q + 1),
+                         qc = ld_uniform(a.quads + 3 * q + 2);
+            const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
+            const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
+            const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
+            const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
+            if (sphere_may_hit(D, R, ax, tn, a.margin) && cone_may_hit_n<4>(c, hu, zero, hv, ax, tn, A, E1, E2, a.margin))
+                m[k] |= 1ull << i;
+        }
+    }
+}
+
+// One lane per wave of slots (grid.y = list entry): the candidate masks of that wave (WaveArgs). The
+// wave's rays start at the centres of a run of micro-facets — a tile of the facet grid (flat
+// heliostats with tiled ray order, as slot_ray) or up to 64 consecutive facets in row-major order
+// (one row, or the end of one row and the start of the next) — and every such centre lies in the parallelogram spanned by the extreme rows
+// and columns of the run: on a flat heliostat the centre is affine in (column offset, row offset)
+// over the whole heliostat (RectangleHelio.cu:54-80), on a focus-flat one within each facet
+// (FocusFlatRectangleHelio.cu:21-39), whose own perturbation frame then gives the reflection axis
+// (no canting term). Origins are float-rounded; the 1 cm margin of the bound covers that. A wave whose
+// run needs more than 8 parallelograms keeps every candidate of its heliostat.
+__global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
+    static_assert(kCullWide == 64, "one 64-bit mask per wave and traversal");
+    const int li = blockIdx.y;
+    const int w = (int)(blockIdx.x * 256u + threadIdx.x);
+    if (w >= a.wstride) return;
+    const int h = ld_uniform(a.list + li);
+    const WideList *wl = a.wide + h;
+    const int n[2] = {ld_uniform(&wl->n[0]), ld_uniform(&wl->n[1])};
+    uint64_t m[2] = {0ull, 0ull};
+    const bool any = (n[0] > 0 && n[0] <= kCullWide) || (n[1] > 0 && n[1] <= kCullWide);
+    const DevHelio &H = a.helios[h];
+    if (any) {
+        const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
+        // slot order as slot_ray: flat heliostats in facet tiles, focus-flat ones in ray order
+        const FacetTiling tl = facet_tiling(R, C, a.L);
+        int nr = 0, r0[2], r1[2], c0[2], c1[2];
+        bool full = false;
+        const long long s0 = 64LL * w;
+        if (H.type == 0 && tl.tiled) {
+            if (s0 < tl.slots) {
+                const int ty = w / tl.ntx, tx = w - ty * tl.ntx;
+                const int ra = ty << tl.lth, ca = tx << tl.ltw;
+                if (ra < R && ca < C) {
+                    r0[0] = ra;
+                    r1[0] = min(R, ra + (1 << tl.lth)) - 1;
+                    c0[0] = ca;
+                    c1[0] = min(C, ca + (1 << tl.ltw)) - 1;
+                    nr = 1;
+                }
+            }
+        } else {
+            const long long nrays = (long long)H.F * a.L;
+            if (s0 < nrays) {
+                const long long e = min(s0 + 63, nrays - 1);
+                const int m0 = (int)(s0 / a.L), m1 = (int)(e / a.L);
+                const int ra = m0 / C, rb = m1 / C;
+                if (ra == rb) {
+                    r0[0] = r1[0] = ra;
+                    c0[0] = m0 - ra * C;
+                    c1[0] = m1 - rb * C;
+                    nr = 1;
+                } else if (rb == ra + 1) {
+                    r0[0] = r1[0] = ra;
+                    c0[0] = m0 - ra * C;
+                    c1[0] = C - 1;
+                    r0[1] = r1[1] = rb;
+                    c0[1] = 0;
+                    c1[1] = m1 - rb * C;
+                    nr = 2;
+                } else {
+                    full = true;
+                }
+            }
+        }
+        const CD3 pos{H.pos[0], H.pos[1], H.pos[2]};
+        const double pl = H.pl;
+        int shapes = 0;
+        for (int g = 0; g < nr && !full; ++g) {
+            if (H.type == 0) {
+                CD3 u, nn, v;
+                frame_axes(H.fr, u, nn, v);
+                auto xcol = [&](int col) {
+                    return col * pl + (col / H.sub_cols) * (double)H.gap[0] + pl / 2 - (double)H.size[0] / 2;
+                };
+                auto zrow = [&](int row) {
+                    return row * pl + (row / H.sub_rows) * (double)H.gap[1] + pl / 2 - (double)H.size[2] / 2;
+                };
+                const double xa = xcol(c0[g]), xb = xcol(c1[g]), za = zrow(r0[g]), zb = zrow(r1[g]);
+                const CD3 c = cadd(pos, cadd(cadd(cmul(u, 0.5 * (xa + xb)), cmul(nn, 0.5 * (double)H.size[1])),
+                                             cmul(v, 0.5 * (za + zb))));
+                wave_shape_bits(a, wl, n, c, cmul(u, 0.5 * (xb - xa)), cmul(v, 0.5 * (zb - za)), nn, m);
+                ++shapes;
+            } else {
+                CD3 U, N, V;
+                frame_axes(H.fr_focus, U, N, V);
+                for (int br = r0[g] / H.sub_rows; br <= r1[g] / H.sub_rows && !full; ++br) {
+                    for (int bc = c0[g] / H.sub_cols; bc <= c1[g] / H.sub_cols && !full; ++bc) {
+                        if (++shapes > 8) {
+                            full = true;
+                            break;
+                        }
+                        const DevFacet &Fc = a.facets[H.facet_base + br * H.cols + bc];
+                        const int ri0 = max(r0[g], br * H.sub_rows) - br * H.sub_rows;
+                        const int ri1 = min(r1[g], br * H.sub_rows + H.sub_rows - 1) - br * H.sub_rows;
+                        const int ci0 = max(c0[g], bc * H.sub_cols) - bc * H.sub_cols;
+                        const int ci1 = min(c1[g], bc * H.sub_cols + H.sub_cols - 1) - bc * H.sub_cols;
+                        const double xa = (ci0 + 0.5) * pl - (double)H.len_col / 2,
+                                     xb = (ci1 + 0.5) * pl - (double)H.len_col / 2;
+                        const double za = (ri0 + 0.5) * pl - (double)H.len_row / 2,
+                                     zb = (ri1 + 0.5) * pl - (double)H.len_row / 2;
+                        CD3 uf, nf, vf, up, np, vp;
+                        frame_axes(Fc.fr_local, uf, nf, vf);
+                        frame_axes(Fc.fr_pert, up, np, vp);
+                        auto rot = [&](CD3 p) { return cadd(cadd(cmul(U, p.x), cmul(N, p.y)), cmul(V, p.z)); };
+                        const CD3 ch = cadd(CD3{Fc.center[0], Fc.center[1], Fc.center[2]},
+                                            cadd(cmul(uf, 0.5 * (xa + xb)), cmul(vf, 0.5 * (za + zb))));
+                        wave_shape_bits(a, wl, n, cadd(pos, rot(ch)), cmul(rot(uf), 0.5 * (xb - xa)),
+                                        cmul(rot(vf), 0.5 * (zb - za)), np, m);
+                    }
+                }
+            }
+        }
+        if (full) {
+            for (int k = 0; k < 2; ++k)
+                m[k] = (n[k] > 0 && n[k] <= kCullWide) ? (n[k] >= 64 ? ~0ull : (1ull << n[k]) - 1ull) : 0ull;
+        }
+    }
+    uint64_t *out = a.masks + 2 * ((size_t)li * a.wstride + w);
+    out[0] = m[0];
+    out[1] = m[1];
 }
 
 // Ray slot -> ray id of heliostat H: flat heliostats take their rays in facet tiles (a wave's origins
@@ -83,8 +253,8 @@ struct RayOut {
 // One ray of map_raytracing (RectangleReceiverRectangleGridRayTracing.cu:6-41,
 // CylinderReceiverRectangleGridRayTracing.cu:6-40) up to, not including, its deposit.
 template <int RT, bool STATS>
-__device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H, const CullList *cl, long long r,
-                                          RayOut &o, RayStats &st) {
+__device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H, const WideList *wl,
+                                          const uint64_t *wm, long long r, RayOut &o, RayStats &st) {
     const int L = a.L;
     const int m = (int)fast_div((uint32_t)r, a.fdL);
     const int j = (int)r - m * L;
@@ -111,8 +281,8 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     facet_geometry(H, a.facets, m, origin, pf, fnormal);
     int hit = -1;
     // traversals proved hit-free for this ray are skipped (never while counting work)
-    const bool culling = !STATS && cl;
-    if ((!culling || traversal_needed(cl, 0, origin, dir, a.grid.quads)) &&
+    const bool culling = !STATS && wl;
+    if ((!culling || traversal_needed_wave(wl, wm, 0, origin, dir, a.grid.quads)) &&
         dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
         o.status = QMC_RAY_SHADOWED;
         o.index = hit;
@@ -122,7 +292,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     // Step 2: perturbed facet normal, reflection, blocking (:24-36)
     V3 normal = normalize(fr_apply(pf, V3{q.x, q.y, q.z}));
     dir = normalize(reflect(-dir, normal));
-    if ((!culling || traversal_needed(cl, 1, origin, dir, a.grid.quads)) &&
+    if ((!culling || traversal_needed_wave(wl, wm, 1, origin, dir, a.grid.quads)) &&
         dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
         o.status = QMC_RAY_BLOCKED;
         o.index = hit;
@@ -168,7 +338,10 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const bool active = r >= 0;
     if (!STATS && !active) return;
     if (active) {
-        trace_ray<RT, STATS>(a, H, a.cull ? a.cull + h : nullptr, r, o, st);
+        // the wave's candidate masks (slots of a wave are consecutive and 64-aligned)
+        const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+        const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)blockIdx.y * a.wstride + wave) : nullptr;
+        trace_ray<RT, STATS>(a, H, a.wide ? a.wide + h : nullptr, wm, r, o, st);
         if (o.status == QMC_RAY_DEPOSITED) unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
         if (REC) {
             qmc_ray_record rec;
@@ -227,7 +400,9 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
         const long long r = slot_ray(H, a.L, (long long)__builtin_amdgcn_readfirstlane(bx) * BS + threadIdx.x);
         if (r >= 0) {
             RayOut o;
-            trace_ray<RT, false>(a, H, a.cull ? a.cull + h : nullptr, r, o, st);
+            const int wave = __builtin_amdgcn_readfirstlane(bx * (BS / 64) + (int)(threadIdx.x >> 6));
+            const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
+            trace_ray<RT, false>(a, H, a.wide ? a.wide + h : nullptr, wm, r, o, st);
             if (o.status == QMC_RAY_DEPOSITED) {
                 const int br = o.row - row0;
                 if ((unsigned)br < (unsigned)rows)
@@ -464,6 +639,8 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_batch_work);
     dev_free(c->d_cull);
     dev_free(c->d_cull_shapes);
+    dev_free(c->d_wide);
+    dev_free(c->d_wmask);
     dev_free(c->d_band_partials);
     dev_free(c->d_buie_phys);
     if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
@@ -1054,12 +1231,20 @@ extern "C" qmc_status qmc_facets_get(qmc_ctx *c, int32_t h, float *origins, floa
 // ---------------------------------------------------------------------------------------------
 // Traversal-culling candidate lists (k_cull) of the listed heliostats of a grid, on the ctx stream,
 // into the ctx-owned per-heliostat array.
-qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists) {
+qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists,
+                            const WideList **wide) {
     const GridBuf &gb = c->grids[grid];
     qmc_status st;
     if (c->cull_cap < c->h_helios.size()) {
         if ((st = dev_alloc(c->d_cull, c->h_helios.size()))) return st;
         c->cull_cap = c->h_helios.size();
+    }
+    if (wide && c->wide_cap < c->h_helios.size()) {
+        if ((st = dev_alloc(c->d_wide, c->h_helios.size()))) {
+            c->wide_cap = 0;
+            return st;
+        }
+        c->wide_cap = c->h_helios.size();
     }
     CullArgs ca;
     ca.helios = c->d_helios;
@@ -1072,11 +1257,53 @@ qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, 
     ca.ang_block = c->cull_ang_block;
     ca.margin = kCullMargin;
     ca.lists = c->d_cull;
+    ca.wide = wide ? c->d_wide : nullptr;
     hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
     HIP_TRY(hipGetLastError());
     *lists = c->d_cull;
+    if (wide) *wide = c->d_wide;
     return QMC_OK;
 }
+
+namespace {
+
+// Per-wave candidate masks (k_cull_waves) of the listed heliostats of a grid, after their wide lists,
+// into the ctx-owned mask buffer (waves of max_slots slots per list entry).
+qmc_status launch_cull_waves(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
+                             TraceArgs &a) {
+    const long long wstride = (max_slots + 63) / 64;
+    if (wstride > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
+    const size_t need = 2 * (size_t)n * (size_t)wstride;
+    if (need > c->wmask_cap) {
+        if (qmc_status st = dev_alloc(c->d_wmask, need)) {
+            c->wmask_cap = 0;
+            return st;
+        }
+        c->wmask_cap = need;
+    }
+    WaveArgs w;
+    std::memset(&w, 0, sizeof(w));
+    w.helios = c->d_helios;
+    w.facets = c->d_facets;
+    w.list = d_list;
+    w.wide = a.wide;
+    w.quads = c->grids[grid].quads;
+    w.sun = c->sun;
+    w.tan_sun = c->cull_tan_sun;
+    w.tan_block = c->cull_tan_block;
+    w.margin = kCullMargin;
+    w.n_list = n;
+    w.L = c->L;
+    w.wstride = (int)wstride;
+    w.masks = c->d_wmask;
+    hipLaunchKernelGGL(k_cull_waves, dim3((unsigned)((wstride + 255) / 256), (unsigned)n), dim3(256), 0, c->stream, w);
+    HIP_TRY(hipGetLastError());
+    a.wmask = c->d_wmask;
+    a.wstride = (int)wstride;
+    return QMC_OK;
+}
+
+}  // namespace
 
 namespace {
 
@@ -1160,7 +1387,9 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     a.blocks_per_helio = (int)bx;
     if (bx == 0 || n == 0) return QMC_OK;
     if (!stats) {
-        const qmc_status st = launch_cull(c, grid, d_list, n, &a.cull);
+        const CullList *lists = nullptr;
+        qmc_status st = launch_cull(c, grid, d_list, n, &lists, &a.wide);
+        if (!st) st = launch_cull_waves(c, grid, d_list, n, max_slots, a);
         if (st) return st;
     }
     const int npix = rb.w * rb.h;
@@ -1260,7 +1489,7 @@ extern "C" qmc_status qmc_cull_flags(qmc_ctx *c, const int32_t *ids, int32_t n, 
         if (list.empty()) continue;
         HIP_TRY(hipMemcpyAsync(c->d_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, c->stream));
         const CullList *d_lists = nullptr;
-        if ((st = launch_cull(c, (int)g, c->d_list, (int)list.size(), &d_lists))) return st;
+        if ((st = launch_cull(c, (int)g, c->d_list, (int)list.size(), &d_lists, nullptr))) return st;
         HIP_TRY(hipMemcpyAsync(all.data(), d_lists, sizeof(CullList) * (size_t)nh, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));  // the list and candidate buffers are reused per grid
         for (int i = 0; i < n; ++i)

@@ -178,13 +178,14 @@ __host__ __device__ inline DevCullShape flat_cull_shape(const DevHelio &H, const
 // [own_lo, own_hi)) over the quads q = first, first + step, ... of a grid: every quad that passes the
 // bound (sphere_may_hit, then cone_may_hit) for the shadow cone (kind 0: tan_sun around the sun axis,
 // the n axis of sunf) or the blocking cone (kind 1: tan_block around the sun axis reflected in the
-// heliostat normal) is counted in n[kind] and, while the count is at most kCullList, listed in
-// ids[kind]; a negative tangent skips that kind. n and ids are shared by the workgroup (LDS, n zeroed
-// beforehand). A kind stops being scanned once its count exceeds kCullList (it is then traversed for
-// every ray and its list is not used), so a final count <= kCullList is complete.
+// heliostat normal) is counted in n[kind] and, while the count is at most cap (<= kCullWide), listed
+// in ids[kind]; a negative tangent skips that kind. n and ids are shared by the workgroup (LDS, n
+// zeroed beforehand). A kind stops being scanned once its count exceeds cap (it is then traversed for
+// every ray and its list is not used), so a final count <= cap is complete.
 __device__ __forceinline__ void cull_collect(const DevCullShape &S, const DevFrame &sunf, const float4 *quads,
                                              int n_sub, int own_lo, int own_hi, double tan_sun, double tan_block,
-                                             double margin, int first, int step, int *n, int (*ids)[kCullList]) {
+                                             double margin, int first, int step, int cap, int *n,
+                                             int (*ids)[kCullWide]) {
     const CD3 c{S.c[0], S.c[1], S.c[2]}, hu{S.hu[0], S.hu[1], S.hu[2]}, hn{S.hn[0], S.hn[1], S.hn[2]},
         hv{S.hv[0], S.hv[1], S.hv[2]}, nrm{S.n[0], S.n[1], S.n[2]};
     const CD3 ash = sunf.identity ? CD3{0, 1, 0} : CD3{sunf.nx, sunf.ny, sunf.nz};
@@ -194,9 +195,9 @@ __device__ __forceinline__ void cull_collect(const DevCullShape &S, const DevFra
     for (int q = first; q < n_sub; q += step) {
         if (q >= own_lo && q < own_hi) continue;
         const bool want_s = tan_sun >= 0.0 &&
-                            __hip_atomic_load(n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= kCullList;
+                            __hip_atomic_load(n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= cap;
         const bool want_b = tan_block >= 0.0 &&
-                            __hip_atomic_load(n + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= kCullList;
+                            __hip_atomic_load(n + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= cap;
         if (!want_s && !want_b) break;
         const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
         const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
@@ -206,12 +207,12 @@ __device__ __forceinline__ void cull_collect(const DevCullShape &S, const DevFra
         if (want_s && sphere_may_hit(D, R, ash, tan_sun, margin) &&
             cone_may_hit(c, hu, hn, hv, ash, tan_sun, A, E1, E2, margin)) {
             const int k = atomicAdd(n, 1);
-            if (k < kCullList) ids[0][k] = q;
+            if (k < cap) ids[0][k] = q;
         }
         if (want_b && sphere_may_hit(D, R, r0, tan_block, margin) &&
             cone_may_hit(c, hu, hn, hv, r0, tan_block, A, E1, E2, margin)) {
             const int k = atomicAdd(n + 1, 1);
-            if (k < kCullList) ids[1][k] = q;
+            if (k < cap) ids[1][k] = q;
         }
     }
 }
@@ -243,6 +244,30 @@ __device__ __forceinline__ bool traversal_needed(const CullList *cl, int kind, V
     for (int i = 0; i < n; ++i) {
         const int id = ld_uniform(&cl->ids[kind][i]);
         // the candidates are wave-uniform and no kernel writes the quads: scalar loads
+        const float4 qa = ld_uniform(quads + 3 * id), qb = ld_uniform(quads + 3 * id + 1),
+                     qc = ld_uniform(quads + 3 * id + 2);
+        float t, u, v;
+        if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) return true;
+    }
+    return false;
+}
+
+// As traversal_needed, for the plain trace: the heliostat's candidates come from its wide list and
+// the ray tests only those that its wave's origins and directions may meet (mask over wl->ids[kind],
+// from k_cull_waves); every other quad is out of reach of every ray of the wave, so a ray that misses
+// the masked ones cannot hit anything in its traversal. More than kRayTests masked candidates run
+// the traversal. wm points at the wave's two masks; both records are read with scalar loads.
+__device__ __forceinline__ bool traversal_needed_wave(const WideList *wl, const uint64_t *wm, int kind, V3 o, V3 d,
+                                                      const float4 *quads) {
+    const int n = ld_uniform(&wl->n[kind]);
+    if (n == 0) return false;
+    if (n > kCullWide) return true;
+    uint64_t m = ld_uniform(wm + kind);
+    if (__builtin_popcountll(m) > kRayTests) return true;
+    while (m) {
+        const int i = __builtin_ctzll(m);
+        m &= m - 1;
+        const int id = ld_uniform(&wl->ids[kind][i]);
         const float4 qa = ld_uniform(quads + 3 * id), qb = ld_uniform(quads + 3 * id + 1),
                      qc = ld_uniform(quads + 3 * id + 2);
         float t, u, v;
