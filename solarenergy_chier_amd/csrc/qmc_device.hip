@@ -280,10 +280,14 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     DevFrame pf;
     facet_geometry(H, a.facets, m, origin, pf, fnormal);
     int hit = -1;
-    // traversals proved hit-free for this ray are skipped (never while counting work)
+    // traversals proved hit-free for this ray are skipped, and the others look up the hits that the
+    // ray's candidate tests found (never while counting work)
     const bool culling = !STATS && wl;
-    if ((!culling || traversal_needed_wave(wl, wm, 0, origin, dir, a.grid.quads)) &&
-        dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
+    RayHits rh;
+    int plan = culling ? traversal_plan(wl, wm, 0, origin, dir, a.grid.quads, rh) : 2;
+    if (plan != 0 &&
+        (plan == 2 ? dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)
+                   : dda_known<STATS>(rh, origin, dir, a.grid, own_lo, own_hi, hit, st))) {
         o.status = QMC_RAY_SHADOWED;
         o.index = hit;
         return;
@@ -292,8 +296,10 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     // Step 2: perturbed facet normal, reflection, blocking (:24-36)
     V3 normal = normalize(fr_apply(pf, V3{q.x, q.y, q.z}));
     dir = normalize(reflect(-dir, normal));
-    if ((!culling || traversal_needed_wave(wl, wm, 1, origin, dir, a.grid.quads)) &&
-        dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)) {
+    plan = culling ? traversal_plan(wl, wm, 1, origin, dir, a.grid.quads, rh) : 2;
+    if (plan != 0 &&
+        (plan == 2 ? dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)
+                   : dda_known<STATS>(rh, origin, dir, a.grid, own_lo, own_hi, hit, st))) {
         o.status = QMC_RAY_BLOCKED;
         o.index = hit;
         return;

@@ -24,11 +24,12 @@ struct RayStats {
     unsigned cells, candidates, tests;
 };
 
-// 3D-DDA over the uniform grid; true when a non-own sub-heliostat quad is hit (first hit in
-// traversal and cell-list order; its id goes to hit_id).
-template <bool STATS>
-__device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, const float4 *quads, int own_lo,
-                                              int own_hi, int &hit_id, RayStats &st) {
+// Walk of the 3D-DDA over the uniform grid: cells in the reference's order, each cell's candidate
+// ids in list order, own sub-heliostats skipped; test(id) says whether the ray hits quad id. True at
+// the first hit, with its id in hit_id.
+template <bool STATS, class Test>
+__device__ __forceinline__ bool dda_walk(V3 o, V3 d, const DevGrid &g, int own_lo, int own_hi, int &hit_id,
+                                         RayStats &st, Test test) {
     const V3 gpos{g.pos[0], g.pos[1], g.pos[2]};
     const V3 gint{g.interval[0], g.interval[1], g.interval[2]};
     const V3 cell = (o - gpos) / gint;
@@ -58,11 +59,7 @@ __device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, cons
             const int id = i == 0 ? rec.y : i == 1 ? rec.z : i == 2 ? rec.w : g.match[g.cell_start[addr] + i];
             if (id < own_lo || id >= own_hi) {
                 if (STATS) st.tests += 1;
-                const float4 qa = quads[3 * id];
-                const float4 qb = quads[3 * id + 1];
-                const float4 qc = quads[3 * id + 2];
-                float t, u, v;
-                if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) {
+                if (test(id)) {
                     hit_id = id;
                     return true;
                 }
@@ -92,6 +89,20 @@ __device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, cons
         tz += fz ? dz : 0.0f;
     }
     return false;
+}
+
+// 3D-DDA over the uniform grid; true when a non-own sub-heliostat quad is hit (first hit in
+// traversal and cell-list order; its id goes to hit_id).
+template <bool STATS>
+__device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, const float4 *quads, int own_lo,
+                                              int own_hi, int &hit_id, RayStats &st) {
+    return dda_walk<STATS>(o, d, g, own_lo, own_hi, hit_id, st, [&](int id) {
+        const float4 qa = quads[3 * id];
+        const float4 qb = quads[3 * id + 1];
+        const float4 qc = quads[3 * id + 2];
+        float t, u, v;
+        return ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v);
+    });
 }
 
 // Origin of micro-facet m of a flat heliostat whose normal frame is fr (RectangleHelio.cu:64-79):
@@ -252,28 +263,62 @@ __device__ __forceinline__ bool traversal_needed(const CullList *cl, int kind, V
     return false;
 }
 
-// As traversal_needed, for the plain trace: the heliostat's candidates come from its wide list and
-// the ray tests only those that its wave's origins and directions may meet (mask over wl->ids[kind],
-// from k_cull_waves); every other quad is out of reach of every ray of the wave, so a ray that misses
-// the masked ones cannot hit anything in its traversal. More than kRayTests masked candidates run
-// the traversal. wm points at the wave's two masks; both records are read with scalar loads.
-__device__ __forceinline__ bool traversal_needed_wave(const WideList *wl, const uint64_t *wm, int kind, V3 o, V3 d,
-                                                      const float4 *quads) {
+// A ray's tests of its wave's masked candidates (traversal_plan): their ids, wave-uniform, and which
+// of them the ray hits.
+struct RayHits {
+    int n;
+    int ids[kRayTests];
+    uint32_t bits;  // bit j: ray_quad hits ids[j]
+};
+
+// How the plain trace runs a ray's traversal of kind `kind`, given its heliostat's wide list and the
+// mask of its wave (wm: the wave's two masks, from k_cull_waves): every quad outside the mask is out
+// of reach of every ray of the wave, so ray_quad misses it. Returns
+//   0 — no traversal: no candidate, or the ray misses every masked one, so no quad can be hit;
+//   1 — the ray hits a masked candidate: the traversal's cell walk runs, and since every quad's
+//       ray_quad result is known (rh: the masked candidates' results, a miss for all others), it
+//       looks ids up instead of testing them; its first hit is the reference traversal's;
+//   2 — the traversal runs with its own tests (too many candidates to list or mask).
+// Records are read with scalar loads.
+__device__ __forceinline__ int traversal_plan(const WideList *wl, const uint64_t *wm, int kind, V3 o, V3 d,
+                                              const float4 *quads, RayHits &rh) {
     const int n = ld_uniform(&wl->n[kind]);
-    if (n == 0) return false;
-    if (n > kCullWide) return true;
+    if (n == 0) return 0;
+    if (n > kCullWide) return 2;
     uint64_t m = ld_uniform(wm + kind);
-    if (__builtin_popcountll(m) > kRayTests) return true;
-    while (m) {
+    if (__builtin_popcountll(m) > kRayTests) return 2;
+    rh.n = 0;
+    rh.bits = 0u;
+#pragma unroll
+    for (int j = 0; j < kRayTests; ++j) {
+        if (!m) break;
         const int i = __builtin_ctzll(m);
         m &= m - 1;
         const int id = ld_uniform(&wl->ids[kind][i]);
         const float4 qa = ld_uniform(quads + 3 * id), qb = ld_uniform(quads + 3 * id + 1),
                      qc = ld_uniform(quads + 3 * id + 2);
         float t, u, v;
-        if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) return true;
+        if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v))
+            rh.bits |= 1u << j;
+        rh.ids[j] = id;
+        rh.n = j + 1;
     }
-    return false;
+    return rh.bits ? 1 : 0;
+}
+
+// The traversal of a ray with plan 1 (traversal_plan): the reference's cell walk, taking each quad's
+// hit from rh (a masked candidate's ray_quad result, a miss for every other quad) instead of testing
+// it. Kept apart from dda_collision's walk, which plan 2 runs: one loop with both tests holds more
+// registers through the whole trace.
+template <bool STATS>
+__device__ __forceinline__ bool dda_known(const RayHits &rh, V3 o, V3 d, const DevGrid &g, int own_lo, int own_hi,
+                                          int &hit_id, RayStats &st) {
+    return dda_walk<STATS>(o, d, g, own_lo, own_hi, hit_id, st, [&](int id) {
+        bool hit = false;
+#pragma unroll
+        for (int j = 0; j < kRayTests; ++j) hit |= (j < rh.n) & (id == rh.ids[j]) & (((rh.bits >> j) & 1u) != 0u);
+        return hit;
+    });
 }
 
 // Half-angles (radians) at or above which a cone is not culled: the bound needs theta < 90 deg.
