@@ -17,7 +17,9 @@ struct GridBuf {
     int32_t *cell_start = nullptr;
     int32_t *match = nullptr;
     float4 *quads = nullptr;
+    QuadGroup *groups = nullptr;  // culling scan groups of the quads
     int n_sub = 0;
+    int n_groups = 0;
     int receiver = 0;
 };
 
@@ -143,6 +145,7 @@ inline void free_scene(qmc_ctx *c) {
         dev_free(g.cell_start);
         dev_free(g.match);
         dev_free(g.quads);
+        dev_free(g.groups);
     }
     c->grids.clear();
     for (auto &r : c->recvs) {

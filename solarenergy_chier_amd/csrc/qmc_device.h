@@ -99,6 +99,15 @@ struct DevCullShape {
     double cant;
 };
 
+// Quads q in [begin, begin + count) of a grid (one heliostat's sub-heliostats, or one quad alone),
+// with a ball (centre c, radius r, metres) holding each quad's own bounding ball: the culling scan
+// tests the group's ball before its quads.
+struct QuadGroup {
+    double c[3];
+    double r;
+    int32_t begin, count;
+};
+
 // Candidate quads of a heliostat's two traversals (index 0 shadow, 1 blocking), from the culling scan:
 // n[k] quads of the grid may meet the cone; 0 means the traversal can only miss. With n[k] <=
 // kCullList the candidates are listed (in no particular order) and a ray that hits none of them
@@ -125,7 +134,8 @@ struct CullArgs {
     const DevCullShape *shapes;
     const int32_t *list;
     const float4 *quads;  // the grid's blocking quads (A, E1, E2)
-    int32_t n_sub;
+    const QuadGroup *groups;
+    int32_t n_groups;
     DevFrame sun;         // frame of -sun_dir: shadow rays leave along its n axis
     double tan_sun;       // cone of shadow rays (negative: no culling)
     double ang_block;     // half-angle of the blocking cone of a flat heliostat (negative: no culling)

@@ -49,8 +49,8 @@ __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
     const double tan_block = (ang_block >= 0.0 && ang_block < kCullMaxAngle) ? tan(ang_block) : -1.0;
     if (threadIdx.x < 2) s_n[threadIdx.x] = 0;
     __syncthreads();
-    cull_collect(S, a.sun, a.quads, a.n_sub, H.sub_begin, H.sub_begin + H.sub_count, a.tan_sun, tan_block, a.margin,
-                 threadIdx.x, blockDim.x, a.wide ? kCullWide : kCullList, s_n, s_ids);
+    cull_collect(S, a.sun, a.quads, a.groups, a.n_groups, H.sub_begin, H.sub_begin + H.sub_count, a.tan_sun,
+                 tan_block, a.margin, threadIdx.x, blockDim.x, a.wide ? kCullWide : kCullList, s_n, s_ids);
     __syncthreads();
     if (a.wide) {
         WideList *W = a.wide + h;
@@ -798,6 +798,68 @@ std::vector<float4> pack_quads(const HostGrid &hg) {
     return quads;
 }
 
+// Culling scan groups of grid g's quads: each heliostat's sub-heliostats (runs of at most 64), every
+// quad of no heliostat alone. The group ball holds each quad's bounding ball (centre A + (E1+E2)/2,
+// radius half the longer diagonal, as the scan computes it from the same floats), widened by 1e-9
+// relative for rounding.
+std::vector<QuadGroup> pack_groups(const std::vector<float4> &quads, int n_sub, const std::vector<DevHelio> &helios,
+                                   int g) {
+    std::vector<int> owner((size_t)n_sub, -1);
+    for (size_t h = 0; h < helios.size(); ++h)
+        if (helios[h].grid == g)
+            for (int q = helios[h].sub_begin; q < helios[h].sub_begin + helios[h].sub_count; ++q)
+                if (q >= 0 && q < n_sub) owner[q] = (int)h;
+    std::vector<QuadGroup> out;
+    for (int q = 0; q < n_sub;) {
+        int e = q + 1;
+        if (owner[q] >= 0)
+            while (e < n_sub && owner[e] == owner[q] && e - q < 64) ++e;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int k = q; k < e; ++k) {
+            const float4 A = quads[3 * k], E1 = quads[3 * k + 1], E2 = quads[3 * k + 2];
+            const double a[3] = {A.x, A.y, A.z}, e1[3] = {E1.x, E1.y, E1.z}, e2[3] = {E2.x, E2.y, E2.z};
+            for (int i = 0; i < 4; ++i)
+                for (int d = 0; d < 3; ++d) {
+                    const double p = a[d] + ((i & 1) ? e1[d] : 0.0) + ((i & 2) ? e2[d] : 0.0);
+                    lo[d] = std::min(lo[d], p);
+                    hi[d] = std::max(hi[d], p);
+                }
+        }
+        QuadGroup G;
+        for (int d = 0; d < 3; ++d) G.c[d] = 0.5 * (lo[d] + hi[d]);
+        double r = 0.0;
+        for (int k = q; k < e; ++k) {
+            const float4 A = quads[3 * k], E1 = quads[3 * k + 1], E2 = quads[3 * k + 2];
+            const double s0 = (double)E1.x + E2.x, s1 = (double)E1.y + E2.y, s2 = (double)E1.z + E2.z;
+            const double d0 = (double)E1.x - E2.x, d1 = (double)E1.y - E2.y, d2 = (double)E1.z - E2.z;
+            const double cq[3] = {A.x + 0.5 * s0, A.y + 0.5 * s1, A.z + 0.5 * s2};
+            const double rq = 0.5 * std::sqrt(std::max(s0 * s0 + s1 * s1 + s2 * s2, d0 * d0 + d1 * d1 + d2 * d2));
+            const double dc = std::sqrt((cq[0] - G.c[0]) * (cq[0] - G.c[0]) + (cq[1] - G.c[1]) * (cq[1] - G.c[1]) +
+                                        (cq[2] - G.c[2]) * (cq[2] - G.c[2]));
+            r = std::max(r, dc + rq);
+        }
+        G.r = r * (1.0 + 1e-9) + 1e-9;
+        G.begin = q;
+        G.count = e - q;
+        out.push_back(G);
+        q = e;
+    }
+    return out;
+}
+
+// Uploads grid g's culling scan groups (after its quads and the heliostat records are known).
+qmc_status upload_groups(qmc_ctx *c, int g, const std::vector<float4> &quads) {
+    GridBuf &gb = c->grids[g];
+    const std::vector<QuadGroup> groups = pack_groups(quads, gb.n_sub, c->h_helios, g);
+    if ((size_t)gb.n_groups != groups.size() || !gb.groups) {
+        if (qmc_status st = dev_alloc(gb.groups, groups.size())) return st;
+        gb.n_groups = (int)groups.size();
+    }
+    if (!groups.empty())
+        HIP_TRY(hipMemcpy(gb.groups, groups.data(), groups.size() * sizeof(QuadGroup), hipMemcpyHostToDevice));
+    return QMC_OK;
+}
+
 }  // namespace
 
 extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
@@ -897,6 +959,8 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     int64_t off_base = pack_heliostats(*s, c->h_helios, facets, shapes);
     aim_points_f64(*s, c->h_aim64);
     c->offsets_total = off_base;
+    for (size_t gi = 0; gi < s->grids.size(); ++gi)
+        if ((st = upload_groups(c, (int)gi, pack_quads(s->grids[gi])))) return st;
     if ((st = dev_alloc(c->d_helios, c->h_helios.size()))) return st;
     if ((st = dev_alloc(c->d_facets, facets.size()))) return st;
     if ((st = dev_alloc(c->d_offsets, (size_t)off_base))) return st;
@@ -983,6 +1047,8 @@ extern "C" qmc_status qmc_update(qmc_ctx *c, const qmc_scene *s) {
         if (!q.empty())
             HIP_TRY(hipMemcpyAsync(c->grids[g].quads, q.data(), q.size() * sizeof(float4), hipMemcpyHostToDevice,
                                    c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if ((st = upload_groups(c, (int)g, q))) return st;
     }
     // pageable sources: wait until the copies have consumed them
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1257,7 +1323,8 @@ qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, 
     ca.shapes = c->d_cull_shapes;
     ca.list = d_list;
     ca.quads = gb.quads;
-    ca.n_sub = gb.n_sub;
+    ca.groups = gb.groups;
+    ca.n_groups = gb.n_groups;
     ca.sun = c->sun;
     ca.tan_sun = c->cull_tan_sun;
     ca.ang_block = c->cull_ang_block;
