@@ -194,23 +194,20 @@ QMC_HD CD3 cmul(CD3 a, double s) { return CD3{a.x * s, a.y * s, a.z * s}; }
 QMC_HD double cdot(CD3 a, CD3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 QMC_HD CD3 ccross(CD3 a, CD3 b) { return CD3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 
-// NP = 8: a box; NP = 4: a rectangle (hn is then ignored). Loops are fully unrolled so the corner
-// arrays stay in registers.
+// NP = 8: a box; NP = 4: a rectangle (hn is then ignored). Projections use support functions: on a
+// direction w the box spans c.w -+ (|hu.w| + |hv.w| + |hn.w|) and the parallelogram spans A.w +
+// [min(0, E1.w) + min(0, E2.w), max(0, E1.w) + max(0, E2.w)]. Axes are not normalised: a gap g
+// between the two spans on an axis w of length |w| is a distance g / |w|, so "separated by more than
+// reach" is g > 0 and g^2 > reach^2 |w|^2.
 template <int NP>
 QMC_HD bool cone_may_hit_n(CD3 c, CD3 hu, CD3 hn, CD3 hv, CD3 a, double tan_theta, CD3 A, CD3 E1, CD3 E2,
                            double margin) {
-    CD3 P[NP];
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-        P[i] = cadd(cadd(c, cmul(hu, (i & 1) ? -1.0 : 1.0)), cmul(hv, (i & 2) ? -1.0 : 1.0));
-        if (NP == 8) P[i] = cadd(P[i], cmul(hn, (i & 4) ? -1.0 : 1.0));
-    }
-    CD3 Q[4] = {A, cadd(A, E1), cadd(A, E2), cadd(cadd(A, E1), E2)};
-    double pa_min = 1e300, qa_max = -1e300;
-#pragma unroll
-    for (int i = 0; i < NP; ++i) pa_min = fmin(pa_min, cdot(P[i], a));
-#pragma unroll
-    for (int i = 0; i < 4; ++i) qa_max = fmax(qa_max, cdot(Q[i], a));
+    auto box_r = [&](CD3 w) {
+        return fabs(cdot(hu, w)) + fabs(cdot(hv, w)) + (NP == 8 ? fabs(cdot(hn, w)) : 0.0);
+    };
+    const double e1a = cdot(E1, a), e2a = cdot(E2, a);
+    const double pa_min = cdot(c, a) - box_r(a);
+    const double qa_max = cdot(A, a) + fmax(0.0, e1a) + fmax(0.0, e2a);
     const double tmax = qa_max - pa_min;
     if (tmax < -margin) return false;  // the quad lies wholly behind every origin
     const double reach = fmax(tmax, 0.0) * tan_theta + margin;
@@ -220,23 +217,14 @@ QMC_HD bool cone_may_hit_n(CD3 c, CD3 hu, CD3 hn, CD3 hv, CD3 a, double tan_thet
                    ccross(a, hn)};
 #pragma unroll
     for (int k = 0; k < (NP == 8 ? 6 : 5); ++k) {
-        const double len = sqrt(cdot(axes[k], axes[k]));
-        if (!(len > 1e-12)) continue;
-        const CD3 w = cmul(axes[k], 1.0 / len);
-        double p_lo = 1e300, p_hi = -1e300, q_lo = 1e300, q_hi = -1e300;
-#pragma unroll
-        for (int i = 0; i < NP; ++i) {
-            const double pw = cdot(P[i], w);
-            p_lo = fmin(p_lo, pw);
-            p_hi = fmax(p_hi, pw);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const double qw = cdot(Q[i], w);
-            q_lo = fmin(q_lo, qw);
-            q_hi = fmax(q_hi, qw);
-        }
-        if (q_lo > p_hi + reach || q_hi < p_lo - reach) return false;
+        const CD3 w = axes[k];
+        const double ww = cdot(w, w);
+        if (!(ww > 1e-24)) continue;
+        const double cw = cdot(c, w), r = box_r(w);
+        const double aw = cdot(A, w), e1 = cdot(E1, w), e2 = cdot(E2, w);
+        const double q_lo = aw + fmin(0.0, e1) + fmin(0.0, e2), q_hi = aw + fmax(0.0, e1) + fmax(0.0, e2);
+        const double gap = fmax(q_lo - (cw + r), (cw - r) - q_hi);
+        if (gap > 0.0 && gap * gap > reach * reach * ww) return false;
     }
     return true;
 }
