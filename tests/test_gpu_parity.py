@@ -189,6 +189,15 @@ def test_ps10_field_sampled_heliostats(device):
     assert parity_on_scene(device, scene, [0, 17, 300, 623]) == 0
 
 
+@pytest.mark.parametrize("L", [1, 4])
+def test_ps10_field_tiled_ray_order(device, L):
+    """PS10-like field with 1 or 4 rays per micro-facet: flat heliostats then take their rays in 8x8 or
+    4x4 facet tiles, so each wave's candidate mask is bounded from a tile of facets."""
+    wl = workloads.ps10_field(L=L)
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    assert parity_on_scene(device, scene, [0, 17, 300, 623]) == 0
+
+
 def test_cull_flags_on_full_fields(device):
     """How much the culling bound removes on the two large fields (printed for the record); that no
     culled traversal has a hit is checked ray by ray against the oracle in parity_on_scene."""
