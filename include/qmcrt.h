@@ -432,6 +432,11 @@ qmc_status qmc_image_save_text(const char *path, int32_t height, int32_t width, 
 /* HIP-event timing on the ctx stream (slots 0..15). */
 qmc_status qmc_event_record(qmc_ctx *ctx, int32_t slot);
 qmc_status qmc_event_elapsed_ms(qmc_ctx *ctx, int32_t slot_begin, int32_t slot_end, float *ms);
+/* Instrumentation (no counterpart in the reference): from now on every plain trace launch records
+ * event slot `before` just before its trace kernel (k_trace or k_trace_band) and `after` just after
+ * it, so that kernel's own duration can be read with qmc_event_elapsed_ms apart from the culling
+ * kernels and folds around it. (-1, -1) turns this off. */
+qmc_status qmc_kernel_events(qmc_ctx *ctx, int32_t before, int32_t after);
 
 #ifdef __cplusplus
 }

@@ -192,6 +192,7 @@ SIGNATURES = {
     "qmc_image_save_text": [C.c_char_p, _I, _I, _FP, _I, C.POINTER(ImageStats)],
     "qmc_event_record": [_P, _I],
     "qmc_event_elapsed_ms": [_P, _I, _I, _FP],
+    "qmc_kernel_events": [_P, _I, _I],
 }
 _RESTYPES = {"qmc_last_error": C.c_char_p, "qmc_status_string": C.c_char_p}
 

@@ -36,6 +36,7 @@ struct qmc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[16] = {};
+    int kev_before = -1, kev_after = -1;  // event slots around each plain trace kernel (-1: none)
     bool uploaded = false;
     qmc_config cfg{};
     std::vector<qmc::GridBuf> grids;

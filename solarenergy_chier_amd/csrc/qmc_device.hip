@@ -1420,9 +1420,11 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
         c->band_cap = (size_t)nwg * nband;
     }
     const int row0 = (Hh - rows) / 2;  // flux concentrates around the aim point, the receiver centre
+    if (c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
     if (a.recv.type == 0) launch_band<0>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
     else launch_band<1>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
     HIP_TRY(hipGetLastError());
+    if (c->kev_after >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_after], c->stream));
     hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256), 8), dim3(256), 0, c->stream, rb.acc,
                        c->d_band_partials, nwg, (int)nband, row0 * W);
     HIP_TRY(hipGetLastError());
@@ -1479,6 +1481,8 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     }
     dim3 g((unsigned)bx, (unsigned)n), b(256);
     const int rt = rb.r.type;
+    const bool mark = !rec && !stats;
+    if (mark && c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
     if (rec) {
         if (rt == 0) hipLaunchKernelGGL((k_trace<0, true, false>), g, b, 0, c->stream, a);
         else hipLaunchKernelGGL((k_trace<1, true, false>), g, b, 0, c->stream, a);
@@ -1490,6 +1494,7 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
         else hipLaunchKernelGGL((k_trace<1, false, false>), g, b, 0, c->stream, a);
     }
     HIP_TRY(hipGetLastError());
+    if (mark && c->kev_after >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_after], c->stream));
     hipLaunchKernelGGL(k_acc_fold, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, rb.flux, rb.acc,
                        npix);
     HIP_TRY(hipGetLastError());
@@ -1664,6 +1669,16 @@ extern "C" qmc_status qmc_flux_device_ptr(qmc_ctx *c, int32_t r, float **p, int3
 extern "C" qmc_status qmc_event_record(qmc_ctx *c, int32_t slot) {
     if (!c || slot < 0 || slot >= 16) return fail(QMC_E_INVALID_ARG, "qmc_event_record: bad slot");
     HIP_TRY(hipEventRecord(c->ev[slot], c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_kernel_events(qmc_ctx *c, int32_t before, int32_t after) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_kernel_events: null ctx");
+    const bool off = before == -1 && after == -1;
+    if (!off && (before < 0 || before >= 16 || after < 0 || after >= 16 || before == after))
+        return fail(QMC_E_INVALID_ARG, "qmc_kernel_events: slots must be two distinct values in 0..15, or -1, -1");
+    c->kev_before = before;
+    c->kev_after = after;
     return QMC_OK;
 }
 

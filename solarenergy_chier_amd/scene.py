@@ -492,6 +492,10 @@ class Device:
     def event_record(self, slot: int) -> None:
         N.call("qmc_event_record", self.handle, slot)
 
+    def kernel_events(self, before: int, after: int) -> None:
+        """Record event slots `before` / `after` around every plain trace kernel from now on (-1, -1: off)."""
+        N.call("qmc_kernel_events", self.handle, before, after)
+
     def event_elapsed_ms(self, a: int, b: int) -> float:
         ms = C.c_float(0)
         N.call("qmc_event_elapsed_ms", self.handle, a, b, C.byref(ms))
