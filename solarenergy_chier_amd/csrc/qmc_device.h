@@ -157,6 +157,7 @@ struct WaveArgs {
     double tan_sun, tan_block;  // cone tangents (negative: that traversal is not culled)
     double margin;
     int32_t n_list, L, wstride;
+    FastDiv fdL;
     uint64_t *masks;
 };
 

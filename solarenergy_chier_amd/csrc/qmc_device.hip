@@ -84,8 +84,15 @@ __device__ __forceinline__ void frame_axes(const DevFrame &f, CD3 &u, CD3 &n, CD
 // sun-shape cone of the sun axis, blocking rays within the sun shape plus twice the slope error of the
 // sun axis reflected about nrm. The same two tests as k_cull (sphere_may_hit, then cone_may_hit), with
 // the wave's origins in place of the heliostat's.
-__device__ void wave_shape_bits(const WaveArgs &a, const WideList *wl, const int n[2], CD3 c, CD3 hu, CD3 hv,
-                                CD3 nrm, uint64_t m[2]) {
+// A heliostat's listed candidates in double, staged once per workgroup of k_cull_waves: corner A,
+// edges E1, E2, centre A + (E1 + E2)/2 and the radius of their bounding ball (half the longer
+// diagonal), as k_cull computes them.
+struct StagedQuad {
+    double A[3], E1[3], E2[3], qc[3], r;
+};
+
+__device__ void wave_shape_bits(const WaveArgs &a, const StagedQuad (*sq)[kCullWide], const int n[2], CD3 c, CD3 hu,
+                                CD3 hv, CD3 nrm, uint64_t m[2]) {
     const CD3 zero{0, 0, 0};
     const CD3 ash = a.sun.identity ? CD3{0, 1, 0} : CD3{a.sun.nx, a.sun.ny, a.sun.nz};
     nrm = cmul(nrm, 1.0 / sqrt(cdot(nrm, nrm)));
@@ -98,14 +105,11 @@ __device__ void wave_shape_bits(const WaveArgs &a, const WideList *wl, const int
         const double tn = k == 0 ? a.tan_sun : a.tan_block;
         for (int i = 0; i < n[k]; ++i) {
             if ((m[k] >> i) & 1ull) continue;
-            const int q = ld_uniform(&wl->ids[k][i]);
-            const float4 qa = ld_uniform(a.quads + 3 * q), qb = ld_uniform(a.quads + 3 * q + 1),
-                         qc = ld_uniform(a.quads + 3 * q + 2);
-            const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
-            const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
-            const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
-            const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
-            if (sphere_may_hit(D, R, ax, tn, a.margin) && cone_may_hit_n<4>(c, hu, zero, hv, ax, tn, A, E1, E2, a.margin))
+            const StagedQuad &Q = sq[k][i];
+            const CD3 D = csub(CD3{Q.qc[0], Q.qc[1], Q.qc[2]}, c);
+            if (sphere_may_hit(D, rs + Q.r, ax, tn, a.margin) &&
+                cone_may_hit_n<4>(c, hu, zero, hv, ax, tn, CD3{Q.A[0], Q.A[1], Q.A[2]}, CD3{Q.E1[0], Q.E1[1], Q.E1[2]},
+                                  CD3{Q.E2[0], Q.E2[1], Q.E2[2]}, a.margin))
                 m[k] |= 1ull << i;
         }
     }
@@ -122,12 +126,31 @@ __device__ void wave_shape_bits(const WaveArgs &a, const WideList *wl, const int
 // run needs more than 8 parallelograms keeps every candidate of its heliostat.
 __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
     static_assert(kCullWide == 64, "one 64-bit mask per wave and traversal");
+    __shared__ StagedQuad sq[2][kCullWide];
     const int li = blockIdx.y;
     const int w = (int)(blockIdx.x * 256u + threadIdx.x);
-    if (w >= a.wstride) return;
     const int h = ld_uniform(a.list + li);
     const WideList *wl = a.wide + h;
     const int n[2] = {ld_uniform(&wl->n[0]), ld_uniform(&wl->n[1])};
+    // the heliostat's candidates, one lane each
+    if (threadIdx.x < 2 * kCullWide) {
+        const int k = threadIdx.x / kCullWide, i = threadIdx.x - k * kCullWide;
+        if (n[k] <= kCullWide && i < n[k]) {
+            const int q = wl->ids[k][i];
+            const float4 qa = a.quads[3 * q], qb = a.quads[3 * q + 1], qc = a.quads[3 * q + 2];
+            const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
+            const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
+            const CD3 ctr = cadd(A, cmul(sum, 0.5));
+            StagedQuad &Q = sq[k][i];
+            Q.A[0] = A.x; Q.A[1] = A.y; Q.A[2] = A.z;
+            Q.E1[0] = E1.x; Q.E1[1] = E1.y; Q.E1[2] = E1.z;
+            Q.E2[0] = E2.x; Q.E2[1] = E2.y; Q.E2[2] = E2.z;
+            Q.qc[0] = ctr.x; Q.qc[1] = ctr.y; Q.qc[2] = ctr.z;
+            Q.r = 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
+        }
+    }
+    __syncthreads();
+    if (w >= a.wstride) return;
     uint64_t m[2] = {0ull, 0ull};
     const bool any = (n[0] > 0 && n[0] <= kCullWide) || (n[1] > 0 && n[1] <= kCullWide);
     const DevHelio &H = a.helios[h];
@@ -140,7 +163,7 @@ __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
         const long long s0 = 64LL * w;
         if (H.type == 0 && tl.tiled) {
             if (s0 < tl.slots) {
-                const int ty = w / tl.ntx, tx = w - ty * tl.ntx;
+                const int ty = (int)fast_div((uint32_t)w, tile_divisor(H, tl)), tx = w - ty * tl.ntx;
                 const int ra = ty << tl.lth, ca = tx << tl.ltw;
                 if (ra < R && ca < C) {
                     r0[0] = ra;
@@ -153,9 +176,10 @@ __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
         } else {
             const long long nrays = (long long)H.F * a.L;
             if (s0 < nrays) {
+                // slot counts stay below 2^31 (checked at launch): multiply-high quotients are exact
                 const long long e = min(s0 + 63, nrays - 1);
-                const int m0 = (int)(s0 / a.L), m1 = (int)(e / a.L);
-                const int ra = m0 / C, rb = m1 / C;
+                const int m0 = (int)fast_div((uint32_t)s0, a.fdL), m1 = (int)fast_div((uint32_t)e, a.fdL);
+                const int ra = (int)fast_div((uint32_t)m0, H.fd_cpr), rb = (int)fast_div((uint32_t)m1, H.fd_cpr);
                 if (ra == rb) {
                     r0[0] = r1[0] = ra;
                     c0[0] = m0 - ra * C;
@@ -182,21 +206,27 @@ __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
                 CD3 u, nn, v;
                 frame_axes(H.fr, u, nn, v);
                 auto xcol = [&](int col) {
-                    return col * pl + (col / H.sub_cols) * (double)H.gap[0] + pl / 2 - (double)H.size[0] / 2;
+                    return col * pl + (int)fast_div((uint32_t)col, H.fd_sub_cols) * (double)H.gap[0] + pl / 2 -
+                           (double)H.size[0] / 2;
                 };
                 auto zrow = [&](int row) {
-                    return row * pl + (row / H.sub_rows) * (double)H.gap[1] + pl / 2 - (double)H.size[2] / 2;
+                    return row * pl + (int)fast_div((uint32_t)row, H.fd_sub_rows) * (double)H.gap[1] + pl / 2 -
+                           (double)H.size[2] / 2;
                 };
                 const double xa = xcol(c0[g]), xb = xcol(c1[g]), za = zrow(r0[g]), zb = zrow(r1[g]);
                 const CD3 c = cadd(pos, cadd(cadd(cmul(u, 0.5 * (xa + xb)), cmul(nn, 0.5 * (double)H.size[1])),
                                              cmul(v, 0.5 * (za + zb))));
-                wave_shape_bits(a, wl, n, c, cmul(u, 0.5 * (xb - xa)), cmul(v, 0.5 * (zb - za)), nn, m);
+                wave_shape_bits(a, sq, n, c, cmul(u, 0.5 * (xb - xa)), cmul(v, 0.5 * (zb - za)), nn, m);
                 ++shapes;
             } else {
                 CD3 U, N, V;
                 frame_axes(H.fr_focus, U, N, V);
-                for (int br = r0[g] / H.sub_rows; br <= r1[g] / H.sub_rows && !full; ++br) {
-                    for (int bc = c0[g] / H.sub_cols; bc <= c1[g] / H.sub_cols && !full; ++bc) {
+                const int br0 = (int)fast_div((uint32_t)r0[g], H.fd_sub_rows),
+                          br1 = (int)fast_div((uint32_t)r1[g], H.fd_sub_rows);
+                const int bc0 = (int)fast_div((uint32_t)c0[g], H.fd_sub_cols),
+                          bc1 = (int)fast_div((uint32_t)c1[g], H.fd_sub_cols);
+                for (int br = br0; br <= br1 && !full; ++br) {
+                    for (int bc = bc0; bc <= bc1 && !full; ++bc) {
                         if (++shapes > 8) {
                             full = true;
                             break;
@@ -216,7 +246,7 @@ __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
                         auto rot = [&](CD3 p) { return cadd(cadd(cmul(U, p.x), cmul(N, p.y)), cmul(V, p.z)); };
                         const CD3 ch = cadd(CD3{Fc.center[0], Fc.center[1], Fc.center[2]},
                                             cadd(cmul(uf, 0.5 * (xa + xb)), cmul(vf, 0.5 * (za + zb))));
-                        wave_shape_bits(a, wl, n, cadd(pos, rot(ch)), cmul(rot(uf), 0.5 * (xb - xa)),
+                        wave_shape_bits(a, sq, n, cadd(pos, rot(ch)), cmul(rot(uf), 0.5 * (xb - xa)),
                                         cmul(rot(vf), 0.5 * (zb - za)), np, m);
                     }
                 }
@@ -1367,6 +1397,7 @@ qmc_status launch_cull_waves(qmc_ctx *c, int grid, const int32_t *d_list, int n,
     w.margin = kCullMargin;
     w.n_list = n;
     w.L = c->L;
+    w.fdL = make_fastdiv((uint32_t)c->L);
     w.wstride = (int)wstride;
     w.masks = c->d_wmask;
     hipLaunchKernelGGL(k_cull_waves, dim3((unsigned)((wstride + 255) / 256), (unsigned)n), dim3(256), 0, c->stream, w);
@@ -1457,7 +1488,8 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     a.fdL = make_fastdiv((uint32_t)c->L);
     a.n_list = n;
     const long long bx = (max_slots + 255) / 256;
-    if (bx > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
+    // ray ids and slots are divided by multiply-high constants, exact below 2^31
+    if (max_slots >= 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: 2^31 or more rays per heliostat");
     if (n > 65535) return fail(QMC_E_UNSUPPORTED, "trace: more than 65535 heliostats in one grid launch");
     a.blocks_per_helio = (int)bx;
     if (bx == 0 || n == 0) return QMC_OK;

@@ -396,6 +396,33 @@ def test_atomic_rate_probe(device):
     assert r["lds_f32"] > r["global_f32"]
 
 
+def test_kernel_events_bracket_the_trace_kernel(device):
+    """qmc_kernel_events: the library's events around the trace kernel of a plain trace measure a
+    positive time within the whole call's (which adds the culling kernels and folds); bad slots are
+    refused and (-1, -1) turns the events off."""
+    wl = workloads.ps10_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    device.upload(scene)
+    device.pool_generate(2, 4)
+    device.offsets_generate(3)
+    ids = list(range(scene.counts.n_heliostats))
+    device.kernel_events(2, 3)
+    try:
+        for _ in range(2):
+            device.flux_clear(0)
+            device.event_record(0)
+            device.trace(ids)
+            device.event_record(1)
+            k, total = device.event_elapsed_ms(2, 3), device.event_elapsed_ms(0, 1)
+            print(f"trace kernel {k:.3f} ms of a {total:.3f} ms call", flush=True)
+            assert 0.0 < k <= total
+    finally:
+        device.kernel_events(-1, -1)
+    for bad in ((2, 2), (-1, 3), (0, 16)):
+        with pytest.raises(N.QmcError):
+            device.kernel_events(*bad)
+
+
 def test_offsets_set_validates_range(device):
     """Group offsets index the sample pool, so values outside [0, P] are refused before any launch."""
     wl = workloads.small_field(L=8, groups=16)
