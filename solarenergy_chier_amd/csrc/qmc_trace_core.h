@@ -340,11 +340,9 @@ constexpr double kCullMargin = 0.01;
 
 // Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
 // onto the facet normal (returned by value: H may be a copy in registers).
+// Facet indices use the heliostat's multiply-high divisors (exact for m < 2^31).
 __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
                                                DevFrame &pert_frame, V3 &normal) {
-    const int cpr = H.cols * H.sub_cols;
-    int row = m / cpr;
-    int col = m % cpr;
     const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
     if (H.type == 0) {
         origin = flat_facet_origin(H, H.fr, m);
@@ -352,10 +350,13 @@ __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet
         normal = V3{H.fr.nx, H.fr.ny, H.fr.nz};
     } else {
         // FocusFlatRectangleHelio.cu:21-39,53-64
-        const int block = (row / H.sub_rows) * H.cols + (col / H.sub_cols);
-        const DevFacet &Fc = facets[H.facet_base + block];
-        row %= H.sub_rows;
-        col %= H.sub_cols;
+        const int cpr = H.cols * H.sub_cols;
+        int row = (int)fast_div((uint32_t)m, H.fd_cpr);
+        int col = m - row * cpr;
+        const int brow = (int)fast_div((uint32_t)row, H.fd_sub_rows), bcol = (int)fast_div((uint32_t)col, H.fd_sub_cols);
+        const DevFacet &Fc = facets[H.facet_base + brow * H.cols + bcol];
+        row -= brow * H.sub_rows;
+        col -= bcol * H.sub_cols;
         const float pl = H.pl;
         V3 local{(col + 0.5f) * pl - H.len_col / 2, 0.0f, (row + 0.5f) * pl - H.len_row / 2};
         local = fr_apply(Fc.fr_local, local) + V3{Fc.center[0], Fc.center[1], Fc.center[2]};
