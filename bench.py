@@ -191,7 +191,7 @@ def main() -> int:
             "kernel_avg_ms": kernel_avg_ms,
             "algorithmic_bytes_per_launch": alg_bytes,
             "hbm_gbs_measured": traffic / (kernel_avg_ms / 1e3) / 1e9 if traffic else None,
-            "binding_resource": "latency of the persistent loop; device-scope atomics outside the LDS band",
+            "binding_resource": "vector issue (VALU about 80% busy) at 4 waves per SIMD, the occupancy the LDS band allows",
             "valu_busy": valu_busy,
             "atomics": atomics,
         },
