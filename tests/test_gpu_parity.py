@@ -370,9 +370,9 @@ def test_reference_example_full_map_vs_oracle(device):
 
 
 def test_cylinder_field_full_trace_equals_counting_pass(device):
-    """Config-3 field in full (191M rays, cylinder receiver): the map of the uncounted trace (the
-    persistent form with an LDS band of map rows for launches this large) equals the counting pass's
-    map (one lane per ray, device-scope atomics), up to float summation order."""
+    """Config-3 field in full (191M rays, cylinder receiver): the map of the plain trace (per-heliostat,
+    per-wave and per-ray culling, lookup walks) equals the counting pass's map (never culled, every
+    traversal run with its own tests), up to float summation order."""
     wl = workloads.cylinder_field()
     scene = setup_device(device, wl)
     n = scene.counts.n_heliostats
