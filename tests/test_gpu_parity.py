@@ -198,6 +198,19 @@ def test_ps10_field_tiled_ray_order(device, L):
     assert parity_on_scene(device, scene, [0, 17, 300, 623]) == 0
 
 
+def test_wave_masks_spanning_rows_and_facets(device):
+    """Waves whose rays span several micro-facet rows (flat heliostats, 3 rays per facet: no facet tiles,
+    about 21 facets per wave) and several canted facets (focus-flat, 1 ray per facet: 64 facets per
+    wave, up to the parallelogram limit and its keep-every-candidate fallback) still trace every ray
+    as the oracle does."""
+    wl = workloads.ps10_field(L=3)
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    assert parity_on_scene(device, scene, [0, 300, 623]) == 0
+    wl = workloads.cylinder_field(L=1)
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    assert parity_on_scene(device, scene, [0, 1000, 2649]) == 0
+
+
 def test_cull_flags_on_full_fields(device):
     """How much the culling bound removes on the two large fields (printed for the record); that no
     culled traversal has a hit is checked ray by ray against the oracle in parity_on_scene."""
