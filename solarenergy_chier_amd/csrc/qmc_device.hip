@@ -198,6 +198,8 @@ __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
                 }
             }
         }
+        // centres grow with column and row only for non-negative facet gaps; otherwise no bound
+        if (H.type == 0 && !(H.gap[0] >= 0.0f && H.gap[1] >= 0.0f)) full = true;
         const CD3 pos{H.pos[0], H.pos[1], H.pos[2]};
         const double pl = H.pl;
         int shapes = 0;
