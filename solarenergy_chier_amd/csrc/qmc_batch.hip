@@ -104,12 +104,17 @@ __global__ __launch_bounds__(256) void k_batch_setup(BatchArgs a) {
 }
 
 // Traversal culling of every (instance, heliostat), one wavefront each, from the frames and quads
-// k_batch_setup wrote: the per-instance counterpart of the plain trace's k_cull. The wave scans the
-// instance's quads 64 at a time and lists the candidates of each traversal (ballot and lane ranks
-// give each passing lane its slot); a kind stops being scanned once it has more than kCullList.
+// k_batch_setup wrote: the per-instance counterpart of the plain trace's k_cull. The wave first scans
+// the instance's quads 64 at a time with the bounding-ball test alone, collecting the quads that pass
+// in LDS (ballot and lane ranks give each its slot), then runs the cone test once over the collected
+// quads, one per lane, and lists the candidates of each traversal. More than 64 ball-passing quads
+// for a traversal keep it for every ray (counted as more than kCullList).
+constexpr int kBallCap = 64;
 __global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
+    __shared__ int s_q[4][2][kBallCap];
     const int ii = blockIdx.y;
-    const int k = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int k = blockIdx.x * (blockDim.x / 64) + wv;
     if (k >= a.n_helio) return;
     const DevHelio &H = a.helios[a.helio_begin + k];
     const DevCullShape S = flat_cull_shape(H, a.fr[(size_t)ii * a.n_helio + k]);
@@ -124,8 +129,8 @@ __global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
     r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
     const double rs = sqrt(cdot(hu, hu) + cdot(hv, hv));
     const uint64_t below = (1ull << lane) - 1ull;  // lanes ranked before this one
-    int ns = 0, nb = 0;                             // wave-uniform counts
-    for (int base = 0; base < a.n_sub && (ns <= kCullList || nb <= kCullList); base += 64) {
+    int cs = 0, cb = 0;                             // wave-uniform counts of ball-passing quads
+    for (int base = 0; base < a.n_sub && (cs <= kBallCap || cb <= kBallCap); base += 64) {
         const int q = base + lane;
         bool ps = false, pb = false;
         if (q < a.n_sub && (q < own_lo || q >= own_hi)) {
@@ -134,21 +139,44 @@ __global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
             const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
             const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
             const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
-            ps = ns <= kCullList && sphere_may_hit(D, R, ash, a.tan_sun, kCullMargin) &&
-                 cone_may_hit_n<4>(c, hu, hn, hv, ash, a.tan_sun, A, E1, E2, kCullMargin);
-            pb = nb <= kCullList && sphere_may_hit(D, R, r0, a.tan_block, kCullMargin) &&
-                 cone_may_hit_n<4>(c, hu, hn, hv, r0, a.tan_block, A, E1, E2, kCullMargin);
+            ps = a.tan_sun >= 0.0 && sphere_may_hit(D, R, ash, a.tan_sun, kCullMargin);
+            pb = a.tan_block >= 0.0 && sphere_may_hit(D, R, r0, a.tan_block, kCullMargin);
         }
         const uint64_t bs = __ballot(ps), bb = __ballot(pb);
-        const int rs_ = ns + __popcll(bs & below), rb_ = nb + __popcll(bb & below);
-        if (ps && rs_ < kCullList) out->ids[0][rs_] = q;
-        if (pb && rb_ < kCullList) out->ids[1][rb_] = q;
-        ns += __popcll(bs);
-        nb += __popcll(bb);
+        const int rs_ = cs + __popcll(bs & below), rb_ = cb + __popcll(bb & below);
+        if (ps && rs_ < kBallCap) s_q[wv][0][rs_] = q;
+        if (pb && rb_ < kBallCap) s_q[wv][1][rb_] = q;
+        cs += __popcll(bs);
+        cb += __popcll(bb);
+    }
+    // the wave's own LDS lists: visible to its lanes once the wave's stores are done
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int nk[2] = {0, 0};
+    for (int kind = 0; kind < 2; ++kind) {
+        const int cnt = kind == 0 ? cs : cb;
+        const double tn = kind == 0 ? a.tan_sun : a.tan_block;
+        if (cnt > kBallCap) {
+            nk[kind] = kCullList + 1;
+            continue;
+        }
+        bool pass = false;
+        int q = -1;
+        if (lane < cnt) {
+            q = s_q[wv][kind][lane];
+            const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
+            pass = cone_may_hit_n<4>(c, hu, hn, hv, kind == 0 ? ash : r0, tn, CD3{qa.x, qa.y, qa.z},
+                                     CD3{qb.x, qb.y, qb.z}, CD3{qc.x, qc.y, qc.z}, kCullMargin);
+        }
+        const uint64_t bp = __ballot(pass);
+        const int rank = __popcll(bp & below);
+        if (pass && rank < kCullList) out->ids[kind][rank] = q;
+        nk[kind] = __popcll(bp);
     }
     if (lane == 0) {
-        out->n[0] = ns;
-        out->n[1] = nb;
+        out->n[0] = nk[0];
+        out->n[1] = nk[1];
     }
 }
 
