@@ -189,6 +189,22 @@ def test_ps10_field_sampled_heliostats(device):
     assert parity_on_scene(device, scene, [0, 17, 300, 623]) == 0
 
 
+def test_ps10_field_every_eighth_heliostat(device):
+    """Every ray of every eighth heliostat of the bench's PS10-like field (78 heliostats x 774,144 rays)
+    equals the oracle's, with all culling on: blocked and unblocked heliostats across the field."""
+    wl = workloads.ps10_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    assert parity_on_scene(device, scene, range(0, scene.counts.n_heliostats, 8)) == 0
+
+
+def test_cylinder_field_every_sixteenth_heliostat(device):
+    """Every ray of every sixteenth heliostat of the config-3 field (166 focus-flat heliostats x 72,128
+    rays) equals the oracle's, with all culling on."""
+    wl = workloads.cylinder_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    assert parity_on_scene(device, scene, range(0, scene.counts.n_heliostats, 16)) == 0
+
+
 @pytest.mark.parametrize("L", [1, 4])
 def test_ps10_field_tiled_ray_order(device, L):
     """PS10-like field with 1 or 4 rays per micro-facet: flat heliostats then take their rays in 8x8 or
