@@ -23,13 +23,15 @@ LIB = LIBDIR / "libqmcrt.so"
 BINDIR = PKG / "bin"
 PIPELINE = BINDIR / "qmc_pipeline"
 
-HIP_SOURCES = ["qmc_device.hip", "qmc_batch.hip", "qmc_adjoint.hip", "qmc_smooth.hip", "qmc_probe.hip"]
+HIP_SOURCES = ["qmc_device.hip", "qmc_cull.hip", "qmc_batch.hip", "qmc_adjoint.hip", "qmc_smooth.hip",
+               "qmc_probe.hip"]
 HOST_SOURCES = ["scene_host.cpp", "image_saver.cpp"]
 # The persistent kernels (batched sweep, banded plain trace) loop over rays; machine-level
 # loop-invariant code motion hoists copies of wave-uniform operands out of those loops into vector
 # registers (22 to 48 of them), which costs occupancy on a latency-bound traversal.
 EXTRA_FLAGS = {"qmc_batch.hip": ["-mllvm", "-disable-machine-licm"],
-               "qmc_device.hip": ["-mllvm", "-disable-machine-licm"]}
+               "qmc_device.hip": ["-mllvm", "-disable-machine-licm"],
+               "qmc_cull.hip": ["-mllvm", "-disable-machine-licm"]}
 
 ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")
 

@@ -135,10 +135,17 @@ inline DevFrame to_dev(const Frame &f) {
 }
 
 void qmc_adjoint_release(qmc_ctx *c);  // qmc_adjoint.hip
-// Traversal-culling candidate lists (CullList, indexed by heliostat id) of the listed heliostats of a
-// grid, computed on the ctx stream (qmc_device.hip).
+// Traversal culling (qmc_cull.hip), on the ctx stream:
+// candidate lists (CullList, and WideList when `wide` is set; indexed by heliostat id) of the listed
+// heliostats of a grid;
 qmc_status launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists,
                        const WideList **wide);
+// per-wave candidate masks of a plain trace launch over waves of max_slots slots per list entry, after
+// launch_cull with wide lists (a.wide); sets a.wmask and a.wstride;
+qmc_status launch_cull_waves(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
+                             TraceArgs &a);
+// the culling scan's quad groups of grid g (after its quads and the heliostat records are known).
+qmc_status upload_groups(qmc_ctx *c, int g, const std::vector<float4> &quads);
 
 inline void free_scene(qmc_ctx *c) {
     for (auto &g : c->grids) {
