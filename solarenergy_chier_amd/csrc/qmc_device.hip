@@ -8,6 +8,8 @@
 //                               Grid/rectGridDDA.cu:31-119 and the deposits of Receiver/*.cu.
 //                               One launch covers every heliostat of a task list (grid.y), micro-facets
 //                               are generated in registers instead of being materialised.
+//   k_trace_band<RECV, BS>      the same rays for large launches: persistent, deposits into an LDS band of
+//                               map rows (k_band_fold adds the workgroups' bands).
 //   k_pool_sunshape / k_pool_perturbation / k_pool_from_uniforms
 //                               samplelights::linear_interpolation, map_angle2xyz, map_permuation
 //                               (Scene/SceneProcessor.cu:61-103) fed by a scrambled Sobol sequence and a
@@ -15,6 +17,7 @@
 //   k_offsets                   map_float2int (Tool/RandomNumberGenerator/RandomGenerator.cu:20-25).
 //   k_facets                    micro-facet kernels of RectangleHelio.cu:54-80 and
 //                               FocusFlatRectangleHelio.cu:12-64.
+// The traversal culling that the plain trace launches first is in qmc_cull.hip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
