@@ -66,11 +66,6 @@ __device__ __forceinline__ void frame_axes(const DevFrame &f, CD3 &u, CD3 &n, CD
     v = f.identity ? CD3{0, 0, 1} : CD3{f.vx, f.vy, f.vz};
 }
 
-// Sets, in m[k], the bit of each wide-list candidate of traversal k (n[k] of them, 0 < n[k] <=
-// kCullWide) that rays starting in the parallelogram c +- hu +- hv may meet: shadow rays within the
-// sun-shape cone of the sun axis, blocking rays within the sun shape plus twice the slope error of the
-// sun axis reflected about nrm. The same two tests as k_cull (sphere_may_hit, then cone_may_hit), with
-// the wave's origins in place of the heliostat's.
 // A heliostat's listed candidates in double, staged once per workgroup of k_cull_waves: corner A,
 // edges E1, E2, centre A + (E1 + E2)/2 and the radius of their bounding ball (half the longer
 // diagonal), as k_cull computes them.
@@ -78,6 +73,11 @@ struct StagedQuad {
     double A[3], E1[3], E2[3], qc[3], r;
 };
 
+// Sets, in m[k], the bit of each wide-list candidate of traversal k (n[k] of them, 0 < n[k] <=
+// kCullWide) that rays starting in the parallelogram c +- hu +- hv may meet: shadow rays within the
+// sun-shape cone of the sun axis, blocking rays within the sun shape plus twice the slope error of the
+// sun axis reflected about nrm. The same two tests as k_cull (sphere_may_hit, then cone_may_hit), with
+// the wave's origins in place of the heliostat's.
 __device__ void wave_shape_bits(const WaveArgs &a, const StagedQuad (*sq)[kCullWide], const int n[2], CD3 c, CD3 hu,
                                 CD3 hv, CD3 nrm, uint64_t m[2]) {
     const CD3 zero{0, 0, 0};
@@ -105,8 +105,8 @@ __device__ void wave_shape_bits(const WaveArgs &a, const StagedQuad (*sq)[kCullW
 // One lane per wave of slots (grid.y = list entry): the candidate masks of that wave (WaveArgs). The
 // wave's rays start at the centres of a run of micro-facets — a tile of the facet grid (flat
 // heliostats with tiled ray order, as slot_ray) or up to 64 consecutive facets in row-major order
-// (one row, or the end of one row and the start of the next) — and every such centre lies in the parallelogram spanned by the extreme rows
-// and columns of the run: on a flat heliostat the centre is affine in (column offset, row offset)
+// (one row, or the end of one row and the start of the next) — and every such centre lies in the
+// parallelogram spanned by the extreme rows and columns of the run: on a flat heliostat the centre is affine in (column offset, row offset)
 // over the whole heliostat (RectangleHelio.cu:54-80), on a focus-flat one within each facet
 // (FocusFlatRectangleHelio.cu:21-39), whose own perturbation frame then gives the reflection axis
 // (no canting term). Origins are float-rounded; the 1 cm margin of the bound covers that. A wave whose
