@@ -106,8 +106,9 @@ __device__ void wave_shape_bits(const WaveArgs &a, const StagedQuad (*sq)[kCullW
 // wave's rays start at the centres of a run of micro-facets — a tile of the facet grid (flat
 // heliostats with tiled ray order, as slot_ray) or up to 64 consecutive facets in row-major order
 // (one row, or the end of one row and the start of the next) — and every such centre lies in the
-// parallelogram spanned by the extreme rows and columns of the run: on a flat heliostat the centre is affine in (column offset, row offset)
-// over the whole heliostat (RectangleHelio.cu:54-80), on a focus-flat one within each facet
+// parallelogram spanned by the extreme rows and columns of the run: on a flat heliostat the centre
+// grows with column and row over the whole heliostat (affine within a facet, a gap between facets;
+// RectangleHelio.cu:54-80), on a focus-flat one it is affine within each facet
 // (FocusFlatRectangleHelio.cu:21-39), whose own perturbation frame then gives the reflection axis
 // (no canting term). Origins are float-rounded; the 1 cm margin of the bound covers that. A wave whose
 // run needs more than 8 parallelograms keeps every candidate of its heliostat.
