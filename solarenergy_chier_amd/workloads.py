@@ -88,8 +88,10 @@ def ps10_layout(n: int = 624, r_min: float = 100.0, r_max: float = 750.0, span_d
     return arr
 
 
-def ps10_field(L: int = 16, groups: int = 4096, n: int = 624, seed: int = 1) -> Workload:
-    cfg = make_config(sun_dir=sun_direction(180.0, 50.0), dni=1000.0, csr=0.05, num_sunshape_groups=groups,
+def ps10_field(L: int = 16, groups: int = 4096, n: int = 624, seed: int = 1,
+               sun_deg: tuple[float, float] = (180.0, 50.0)) -> Workload:
+    """PS10-like field; `sun_deg` is the sun's (azimuth, elevation) in degrees."""
+    cfg = make_config(sun_dir=sun_direction(*sun_deg), dni=1000.0, csr=0.05, num_sunshape_groups=groups,
                       num_sunshape_lights_per_group=L, inverse_transform_sampling_groups=64,
                       receiver_pixel_length=12.0 / 256.0, helio_disturb_std=0.0015, helio_pixel_length=0.05,
                       helio_reflected_rate=0.88)

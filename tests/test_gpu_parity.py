@@ -214,6 +214,18 @@ def test_ps10_field_tiled_ray_order(device, L):
     assert parity_on_scene(device, scene, [0, 17, 300, 623]) == 0
 
 
+@pytest.mark.parametrize("sun, L", [((135.0, 15.0), 16), ((90.0, 10.0), 16), ((90.0, 10.0), 1)])
+def test_ps10_field_low_sun_shadowing(device, sun, L):
+    """PS10-like field under a low sun (azimuth, elevation in degrees), where neighbours shade flat
+    heliostats: the shadow traversal, its wave masks (per facet row and, at 1 ray per facet, per 8x8
+    facet tile) and its lookup walk run with candidates, and every ray still equals the oracle's."""
+    wl = workloads.ps10_field(L=L, sun_deg=sun)
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    ids = [0, 17, 100, 300, 450, 623]
+    assert parity_on_scene(device, scene, ids) == 0
+    assert (device.cull_flags(ids) & 1).sum() >= 2  # the shadow traversal is needed on several of them
+
+
 def test_wave_masks_spanning_rows_and_facets(device):
     """Waves whose rays span several micro-facet rows (flat heliostats, 3 rays per facet: no facet tiles,
     about 21 facets per wave) and several canted facets (focus-flat, 1 ray per facet: 64 facets per
