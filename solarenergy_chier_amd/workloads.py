@@ -238,11 +238,12 @@ def adjoint_small_field(seed: int = 6) -> tuple[Workload, np.ndarray]:
     return wl, offs
 
 
-def ps10_sampled_field() -> Workload:
+def ps10_sampled_field(sun_deg: tuple[float, float] = (180.0, 50.0)) -> Workload:
     """PS10-like field at instance sampling: 624 heliostats, 0.2 m micro-facets (3008 per heliostat)
     x 1 ray = 1.88M rays, 128x128 map, every heliostat aimed at the receiver centre. The field of
-    the batched sun x aim sweep and (with aim offsets) of the differentiable trace."""
-    wl = ps10_field(L=1, groups=65536)
+    the batched sun x aim sweep and (with aim offsets) of the differentiable trace. `sun_deg` is the
+    sun's (azimuth, elevation) in degrees."""
+    wl = ps10_field(L=1, groups=65536, sun_deg=sun_deg)
     wl.config.helio_pixel_length = 0.2
     r = wl.receivers[0]
     r.res_x = r.res_y = 128
@@ -250,10 +251,11 @@ def ps10_sampled_field() -> Workload:
     return wl
 
 
-def adjoint_ps10_field(seed: int = 6) -> tuple[Workload, np.ndarray]:
+def adjoint_ps10_field(seed: int = 6, sun_deg: tuple[float, float] = (180.0, 50.0)) -> tuple[Workload, np.ndarray]:
     """Differentiated form on the PS10-like field: 624 heliostats, 0.2 m micro-facets (3008 per
-    heliostat) x 1 ray = 1.88M rays per forward pass, 128x128 map, aim offsets in [-2, 2] m."""
-    wl = ps10_sampled_field()
+    heliostat) x 1 ray = 1.88M rays per forward pass, 128x128 map, aim offsets in [-2, 2] m; `sun_deg`
+    is the sun's (azimuth, elevation) in degrees."""
+    wl = ps10_sampled_field(sun_deg)
     offs = np.random.default_rng(seed).uniform(-2.0, 2.0, size=(len(wl.heliostats), 2)).astype(np.float32)
     for h, o in zip(wl.heliostats, offs):
         h.aim_offset = (float(o[0]), float(o[1]))

@@ -89,12 +89,17 @@ class TestGpuAdjoint:
         offs = np.concatenate([device.offsets_get(h, scene.heliostat_info(h).n_micro) for h in range(n)])
         return scene, L, dirs, perts, offs
 
-    @pytest.mark.parametrize("case", ["small", "ps10"])
+    @pytest.mark.parametrize("case", ["small", "ps10", "ps10_low_sun"])
     def test_gradient_matches_fp64_adjoint(self, device, case):
         """The GPU model runs in double like the host adjoint, so the gradients agree far inside
         1e-3 even on the PS10-like field, whose gradient moves by 1e-2 under a 1e-8 rad sun shift;
-        the forward map differs only by float atomic summation."""
-        wl, _ = workloads.adjoint_small_field() if case == "small" else workloads.adjoint_ps10_field()
+        the forward map differs only by float atomic summation. Under the low sun (azimuth 90,
+        elevation 10 degrees) neighbours shade flat heliostats, so the recorded visibility also
+        drops shadowed rays."""
+        if case == "small":
+            wl, _ = workloads.adjoint_small_field()
+        else:
+            wl, _ = workloads.adjoint_ps10_field(sun_deg=(90.0, 10.0) if case == "ps10_low_sun" else (180.0, 50.0))
         scene, L, dirs, perts, offs = self.common(device, wl)
         buf = device.flux_device_ptr(0)
         device.adjoint_forward(0, buf)
