@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1166,6 +1167,26 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     return QMC_OK;
 }
 
+// Reorders a cylinder grid's heliostats for a plain trace. A cylinder map is lit around the azimuth
+// facing each heliostat, and the trace has only a few heliostats of its list in flight at a time:
+// taken one from each of 64 azimuth sectors in turn, those in flight light different columns, so
+// their atomic adds meet on the same pixels less often (2650-heliostat 512x256 field: 6.92 ms against
+// 7.47 in layout order and 7.76 sorted by azimuth). Only the order of the double-precision
+// accumulation changes.
+void interleave_by_azimuth(const qmc_ctx *c, const DevReceiver &R, int32_t *ids, int n) {
+    constexpr int kSectors = 64;
+    std::vector<std::pair<double, int32_t>> key((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const DevHelio &H = c->h_helios[ids[i]];
+        key[(size_t)i] = {std::atan2((double)H.pos[2] - R.pos[2], (double)H.pos[0] - R.pos[0]), ids[i]};
+    }
+    std::stable_sort(key.begin(), key.end(), [](const std::pair<double, int32_t> &x,
+                                                const std::pair<double, int32_t> &y) { return x.first < y.first; });
+    const int q = n / kSectors;  // heliostats per sector; the last n % kSectors follow in azimuth order
+    for (int i = 0; i < n; ++i)
+        ids[i] = i < kSectors * q ? key[(size_t)((i % kSectors) * q + i / kSectors)].second : key[(size_t)i].second;
+}
+
 qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *rec, StatsAcc *stats) {
     if (!c->uploaded) return fail(QMC_E_STATE, "trace: upload a scene first");
     if (!c->pool_ready) return fail(QMC_E_STATE, "trace: set or generate the sample pools first");
@@ -1178,7 +1199,7 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
                                          " is invalid.The heliostat index should between 0 and " + std::to_string(nh) + ".");
     qmc_status st = set_device(c);
     if (st) return st;
-    // group by grid, keep task order inside a grid
+    // group by grid, keep task order inside a grid (a plain trace of a cylinder grid interleaves it)
     std::vector<int32_t> sorted;
     sorted.reserve(n);
     std::vector<std::pair<int, int>> spans;  // (grid, count)
@@ -1187,6 +1208,9 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
         for (int i = 0; i < n; ++i)
             if (c->h_helios[ids[i]].grid == (int)g) sorted.push_back(ids[i]);
         spans.emplace_back((int)g, (int)sorted.size() - before);
+        const DevReceiver &R = c->recvs[c->grids[g].receiver].r;
+        if (!rec && !stats && R.type == 1)
+            interleave_by_azimuth(c, R, sorted.data() + before, (int)sorted.size() - before);
     }
     if (sorted.empty()) return QMC_OK;
     HIP_TRY(hipMemcpyAsync(c->d_list, sorted.data(), sizeof(int32_t) * sorted.size(), hipMemcpyHostToDevice, c->stream));
