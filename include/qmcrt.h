@@ -290,6 +290,15 @@ qmc_status qmc_pool_from_uniforms(qmc_ctx *ctx, const float *u_theta, const floa
  * Replaces Heliostat::generateDeviceMicrohelioGroup, Heliostat.cu:57-62 + map_float2int,
  * Tool/RandomNumberGenerator/RandomGenerator.cu:20-25: offset = int(u*P), u in (0,1]. */
 qmc_status qmc_offsets_generate(qmc_ctx *ctx, uint64_t seed);
+/* Per-call resampling of the group offsets, as the reference draws fresh offsets for its heliostat on
+ * every rayTracing call (QuasiMonteCarloRayTracer.cpp:135-161 -> Heliostat::generateDeviceMicrohelioGroup,
+ * Heliostat.cu:57-62). enable = 1: from now on every trace (qmc_trace, qmc_trace_records*,
+ * qmc_trace_stats_run) first regenerates the offsets of the heliostats it traces, as
+ * qmc_offsets_generate would with seed + k for the k-th such regeneration (k = 0, 1, ... counted from
+ * this call); a heliostat listed twice in one call is traced twice, each time with fresh offsets.
+ * qmc_offsets_get returns the offsets of the last trace. enable = 0 (the default): traces use the
+ * offsets generated or set before, unchanged. */
+qmc_status qmc_offsets_resample(qmc_ctx *ctx, int32_t enable, uint64_t seed);
 qmc_status qmc_offsets_set(qmc_ctx *ctx, int32_t heliostat, const int32_t *offsets, int32_t n);
 qmc_status qmc_offsets_get(qmc_ctx *ctx, int32_t heliostat, int32_t *offsets, int32_t n);
 /* Micro-facet origins and normals of one heliostat (F*3 floats each):
@@ -304,6 +313,12 @@ qmc_status qmc_facets_get(qmc_ctx *ctx, int32_t heliostat, float *origins, float
  * RectangleReceiverRectangleGridRayTracing.cu:45-57, CylinderReceiverRectangleGridRayTracing.cu:44-57.
  * Heliostats of several grids are traced grid by grid. Asynchronous. */
 qmc_status qmc_trace(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n);
+/* Which trace kernel a record trace runs (qmc_trace_records_many): AUTO = the one qmc_trace would run
+ * for the same list, PER_RAY = one lane per ray (k_trace), BANDED = the persistent LDS-banded kernel
+ * (k_trace_band), whatever the list's size. */
+#define QMC_KERNEL_AUTO    0
+#define QMC_KERNEL_PER_RAY 1
+#define QMC_KERNEL_BANDED  2
 /* Rates (G operations/s) of the atomic adds the tracer deposits with, measured on the ctx's device:
  * float and 64-bit device-scope adds scattered over a 65,536-entry map, float LDS adds scattered over
  * a 64 KB tile. A capability probe (about 0.1 s) to state the trace's deposit rates against; no
@@ -320,6 +335,13 @@ qmc_status qmc_probe_atomic_rates(qmc_ctx *ctx, qmc_atomic_rates *rates);
 qmc_status qmc_cull_flags(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n, uint8_t *flags);
 /* Same trace with one record per ray of one heliostat copied to host (F*L records). Synchronous. */
 qmc_status qmc_trace_records(qmc_ctx *ctx, int32_t heliostat, qmc_ray_record *records, int64_t n);
+/* One trace of n distinct heliostats (one launch per grid, as qmc_trace) with the chosen kernel
+ * (QMC_KERNEL_*), writing every ray's record: heliostat ids[i]'s F*L records, indexed by ray id
+ * (micro-facet * L + ray), follow those of ids[0..i-1]; n_records = sum of F*L. Flux is accumulated
+ * as by qmc_trace. Synchronous. Parity checks of each kernel's own indexing against the host oracle:
+ * map_raytracing, RectangleReceiverRectangleGridRayTracing.cu:6-41. */
+qmc_status qmc_trace_records_many(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n, int32_t kernel,
+                                  qmc_ray_record *records, int64_t n_records);
 /* Trace and count the algorithmic work (instrumented kernel; flux is accumulated too). Synchronous. */
 qmc_status qmc_trace_stats_run(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n,
                                qmc_trace_stats *stats);

@@ -21,6 +21,7 @@ QMC_E_PARSE = -7
 QMC_E_STATE = -8
 
 RAY_SHADOWED, RAY_BLOCKED, RAY_MISSED, RAY_DEPOSITED = 0, 1, 2, 3
+KERNEL_AUTO, KERNEL_PER_RAY, KERNEL_BANDED = 0, 1, 2
 RAYF_EDGE_CLAMPED = 1
 
 
@@ -167,11 +168,13 @@ SIGNATURES = {
     "qmc_offsets_generate": [_P, C.c_uint64],
     "qmc_offsets_set": [_P, _I, _IP, _I],
     "qmc_offsets_get": [_P, _I, _IP, _I],
+    "qmc_offsets_resample": [_P, _I, C.c_uint64],
     "qmc_facets_get": [_P, _I, _FP, _FP, _I],
     "qmc_trace": [_P, _IP, _I],
     "qmc_cull_flags": [_P, _IP, _I, C.POINTER(C.c_uint8)],
     "qmc_probe_atomic_rates": [_P, C.POINTER(AtomicRates)],
     "qmc_trace_records": [_P, _I, C.POINTER(RayRecord), C.c_int64],
+    "qmc_trace_records_many": [_P, _IP, _I, _I, C.POINTER(RayRecord), C.c_int64],
     "qmc_trace_stats_run": [_P, _IP, _I, C.POINTER(TraceStats)],
     "qmc_flux_clear": [_P, _I],
     "qmc_flux_read": [_P, _I, _FP, C.c_int64],

@@ -23,8 +23,8 @@ LIB = LIBDIR / "libqmcrt.so"
 BINDIR = PKG / "bin"
 PIPELINE = BINDIR / "qmc_pipeline"
 
-HIP_SOURCES = ["qmc_device.hip", "qmc_cull.hip", "qmc_batch.hip", "qmc_adjoint.hip", "qmc_smooth.hip",
-               "qmc_probe.hip"]
+# every HIP source of csrc/ goes into the one library
+HIP_SOURCES = sorted(p.name for p in CSRC.glob("*.hip"))
 HOST_SOURCES = ["scene_host.cpp", "image_saver.cpp"]
 # The persistent kernels (batched sweep, banded plain trace) loop over rays; machine-level
 # loop-invariant code motion hoists copies of wave-uniform operands out of those loops into vector

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "qmc_device.h"
@@ -21,6 +22,13 @@ struct GridBuf {
     int n_sub = 0;
     int n_groups = 0;
     int receiver = 0;
+};
+
+// One launch's share of a trace call: list entries [begin, begin + count) of pass `pass`, all of grid
+// `grid`, with at most max_slots ray slots per heliostat.
+struct TraceSpan {
+    int pass, grid, begin, count;
+    long long max_slots;
 };
 
 struct RecvBuf {
@@ -90,6 +98,18 @@ struct qmc_ctx {
     size_t wide_cap = 0;
     uint64_t *d_wmask = nullptr;      // per-wave candidate masks of the last plain trace launch
     size_t wmask_cap = 0;
+    long long *d_rec_base = nullptr;  // first record of each list entry of a record trace
+    size_t rec_base_cap = 0;
+    // per-call resampling of the micro-facet group offsets: when on, every pass of a trace call first
+    // regenerates the offsets of the heliostats it traces with seed resample_seed + resample_next++
+    bool resample = false;
+    uint64_t resample_seed = 0, resample_next = 0;
+    // host bookkeeping of a trace call (reserved at upload, so that a call within the reserved sizes
+    // allocates no host memory either)
+    std::vector<int32_t> h_pass, h_occ, h_order, h_sorted, h_tmp;
+    std::vector<long long> h_sorted_base;
+    std::vector<std::pair<double, int32_t>> h_keys;
+    std::vector<qmc::TraceSpan> h_spans;
     // adjoint
     void *adj = nullptr;
 };
@@ -137,13 +157,34 @@ inline DevFrame to_dev(const Frame &f) {
 void qmc_adjoint_release(qmc_ctx *c);  // qmc_adjoint.hip
 // Traversal culling (qmc_cull.hip), on the ctx stream:
 // candidate lists (CullList, and WideList when `wide` is set; indexed by heliostat id) of the listed
-// heliostats of a grid;
+// heliostats of a grid, into the given buffers (one entry per heliostat id; no allocation);
+qmc_status launch_cull_into(qmc_ctx *c, int grid, const int32_t *d_list, int n, CullList *lists, WideList *wide);
+// the same into the ctx-owned buffers, grown to the scene's heliostat count first;
 qmc_status launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists,
                        const WideList **wide);
+// waves per list entry of a launch over max_slots ray slots per heliostat (one mask pair each);
+inline long long wave_stride(long long max_slots) { return (max_slots + 63) / 64; }
 // per-wave candidate masks of a plain trace launch over waves of max_slots slots per list entry, after
-// launch_cull with wide lists (a.wide); sets a.wmask and a.wstride;
-qmc_status launch_cull_waves(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
-                             TraceArgs &a);
+// launch_cull with wide lists (a.wide), into masks (2 * n * wave_stride(max_slots) entries; no
+// allocation); sets a.wmask and a.wstride.
+qmc_status launch_cull_waves_into(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
+                                  uint64_t *masks, TraceArgs &a);
+// Device buffers a plain trace uses besides the scene, pools, offsets and maps: the call's heliostat
+// list, per-heliostat candidate lists, per-wave candidate masks and the banded trace's per-workgroup
+// bands. Either ctx-owned and grown to the call (qmc_trace), or fixed (trace_fixed: a launch whose
+// masks do not fit is split into launches over fewer heliostats, and no buffer grows).
+struct TraceBufs {
+    int32_t *list = nullptr;  // one entry per list entry of the call
+    CullList *cull = nullptr; // one per heliostat id of the scene
+    WideList *wide = nullptr; // one per heliostat id of the scene
+    uint64_t *wmask = nullptr;
+    size_t wmask_cap = 0;     // uint64 entries
+    float *band = nullptr;
+    size_t band_cap = 0;      // floats
+    bool fixed = false;
+};
+// A plain trace (as qmc_trace) into fixed buffers: allocates no device memory.
+qmc_status trace_fixed(qmc_ctx *c, const int32_t *ids, int n, const TraceBufs &b);
 // the culling scan's quad groups of grid g (after its quads and the heliostat records are known).
 qmc_status upload_groups(qmc_ctx *c, int g, const std::vector<float4> &quads);
 

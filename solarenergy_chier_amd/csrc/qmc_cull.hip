@@ -318,10 +318,30 @@ qmc_status qmc::upload_groups(qmc_ctx *c, int g, const std::vector<float4> &quad
 
 
 // Traversal-culling candidate lists (k_cull) of the listed heliostats of a grid, on the ctx stream,
-// into the ctx-owned per-heliostat array.
+// into caller-provided per-heliostat arrays.
+qmc_status qmc::launch_cull_into(qmc_ctx *c, int grid, const int32_t *d_list, int n, CullList *lists, WideList *wide) {
+    const GridBuf &gb = c->grids[grid];
+    CullArgs ca;
+    ca.helios = c->d_helios;
+    ca.shapes = c->d_cull_shapes;
+    ca.list = d_list;
+    ca.quads = gb.quads;
+    ca.groups = gb.groups;
+    ca.n_groups = gb.n_groups;
+    ca.sun = c->sun;
+    ca.tan_sun = c->cull_tan_sun;
+    ca.ang_block = c->cull_ang_block;
+    ca.margin = kCullMargin;
+    ca.lists = lists;
+    ca.wide = wide;
+    hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
+    HIP_TRY(hipGetLastError());
+    return QMC_OK;
+}
+
+// The same into the ctx-owned per-heliostat arrays (grown to the scene's heliostat count).
 qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists,
                             const WideList **wide) {
-    const GridBuf &gb = c->grids[grid];
     qmc_status st;
     if (c->cull_cap < c->h_helios.size()) {
         if ((st = dev_alloc(c->d_cull, c->h_helios.size()))) return st;
@@ -334,40 +354,18 @@ qmc_status qmc::launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, 
         }
         c->wide_cap = c->h_helios.size();
     }
-    CullArgs ca;
-    ca.helios = c->d_helios;
-    ca.shapes = c->d_cull_shapes;
-    ca.list = d_list;
-    ca.quads = gb.quads;
-    ca.groups = gb.groups;
-    ca.n_groups = gb.n_groups;
-    ca.sun = c->sun;
-    ca.tan_sun = c->cull_tan_sun;
-    ca.ang_block = c->cull_ang_block;
-    ca.margin = kCullMargin;
-    ca.lists = c->d_cull;
-    ca.wide = wide ? c->d_wide : nullptr;
-    hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
-    HIP_TRY(hipGetLastError());
+    if ((st = launch_cull_into(c, grid, d_list, n, c->d_cull, wide ? c->d_wide : nullptr))) return st;
     *lists = c->d_cull;
     if (wide) *wide = c->d_wide;
     return QMC_OK;
 }
 
 // Per-wave candidate masks (k_cull_waves) of the listed heliostats of a grid, after their wide lists,
-// into the ctx-owned mask buffer (waves of max_slots slots per list entry).
-qmc_status qmc::launch_cull_waves(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
-                             TraceArgs &a) {
-    const long long wstride = (max_slots + 63) / 64;
+// into `masks` (2 * n * wave_stride(max_slots) entries).
+qmc_status qmc::launch_cull_waves_into(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
+                                       uint64_t *masks, TraceArgs &a) {
+    const long long wstride = wave_stride(max_slots);
     if (wstride > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
-    const size_t need = 2 * (size_t)n * (size_t)wstride;
-    if (need > c->wmask_cap) {
-        if (qmc_status st = dev_alloc(c->d_wmask, need)) {
-            c->wmask_cap = 0;
-            return st;
-        }
-        c->wmask_cap = need;
-    }
     WaveArgs w;
     std::memset(&w, 0, sizeof(w));
     w.helios = c->d_helios;
@@ -383,11 +381,10 @@ qmc_status qmc::launch_cull_waves(qmc_ctx *c, int grid, const int32_t *d_list, i
     w.L = c->L;
     w.fdL = make_fastdiv((uint32_t)c->L);
     w.wstride = (int)wstride;
-    w.masks = c->d_wmask;
+    w.masks = masks;
     hipLaunchKernelGGL(k_cull_waves, dim3((unsigned)((wstride + 255) / 256), (unsigned)n), dim3(256), 0, c->stream, w);
     HIP_TRY(hipGetLastError());
-    a.wmask = c->d_wmask;
+    a.wmask = masks;
     a.wstride = (int)wstride;
     return QMC_OK;
 }
-

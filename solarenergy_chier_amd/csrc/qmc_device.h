@@ -173,7 +173,8 @@ struct TraceArgs {
     const int32_t *list;       // heliostat ids of this launch
     float *flux;
     double *acc;               // the launch's deposits (double; folded into flux after the launch)
-    qmc_ray_record *records;   // nullable (one heliostat per launch when set)
+    qmc_ray_record *records;   // nullable: ray r of list entry li goes to records[rec_base[li] + r]
+    const long long *rec_base; // with records: first record of each list entry
     StatsAcc *stats;           // nullable
     const WideList *wide;      // per heliostat id: candidate quads of each traversal (null: no culling)
     const uint64_t *wmask;     // per-wave candidate masks (WaveArgs), with wide

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
             rec.index = o.index;
             rec.energy = o.energy;
             rec.flags = o.flags;
-            a.records[r] = rec;
+            a.records[a.rec_base[blockIdx.y] + r] = rec;
         }
     }
     if (STATS) {
@@ -189,8 +189,10 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
 // [row0, row0 + rows) in LDS for the whole launch: deposits inside it are LDS atomics, the others
 // device-scope atomics on the launch's double accumulator. Workgroups take BS-slot blocks
 // (heliostat, block) round robin; at the end each stores its band, with plain stores, to its own
-// slice of `partials`, which k_band_fold adds into the accumulator.
-template <int RT, int BS>
+// slice of `partials`, which k_band_fold adds into the accumulator. With REC every ray also writes its
+// record at records[rec_base[li] + ray id] (parity checks of this kernel's own slot, wave-mask and
+// band indexing).
+template <int RT, int BS, bool REC>
 __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int rows, int blocks_per_helio,
                                                     float *partials) {
     extern __shared__ float band[];
@@ -219,6 +221,14 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                     atomicAdd(band + br * W + o.col, o.energy);
                 else
                     unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
+            }
+            if (REC) {
+                qmc_ray_record rec;
+                rec.status = o.status;
+                rec.index = o.index;
+                rec.energy = o.energy;
+                rec.flags = o.flags;
+                a.records[ld_uniform(a.rec_base + __builtin_amdgcn_readfirstlane(li)) + r] = rec;
             }
         }
         bx += (int)gridDim.x;
@@ -373,6 +383,18 @@ __global__ __launch_bounds__(256) void k_offsets(int32_t *off, const DevHelio *h
     off[H.offsets_base + m] = group_offset(seed, h, m, P);
 }
 
+// The same for the heliostats of a list only (blockIdx.y = list entry): the fresh group offsets that
+// each reference rayTracing call draws for its heliostat (QuasiMonteCarloRayTracer.cpp:135-161 ->
+// Heliostat::generateDeviceMicrohelioGroup, Heliostat.cu:57-62).
+__global__ __launch_bounds__(256) void k_offsets_list(int32_t *off, const DevHelio *helios, const int32_t *list,
+                                                      uint64_t seed, int P) {
+    const int h = list[blockIdx.y];
+    const DevHelio &H = helios[h];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m > H.F) return;
+    off[H.offsets_base + m] = group_offset(seed, h, m, P);
+}
+
 __global__ __launch_bounds__(256) void k_facets(const DevHelio *helios, const DevFacet *facets, int h, float *origins,
                                                 float *normals) {
     const DevHelio &H = helios[h];
@@ -451,6 +473,7 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_cull_shapes);
     dev_free(c->d_wide);
     dev_free(c->d_wmask);
+    dev_free(c->d_rec_base);
     dev_free(c->d_band_partials);
     dev_free(c->d_buie_phys);
     if (c->d_batch_inst) (void)hipFree(c->d_batch_inst);
@@ -719,6 +742,18 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     if ((int)s->heliostats.size() > c->list_cap) {
         if ((st = dev_alloc(c->d_list, s->heliostats.size()))) return st;
         c->list_cap = (int)s->heliostats.size();
+    }
+    // host bookkeeping of trace calls of up to max(heliostats, 65536) entries (no host allocation then)
+    {
+        const size_t nres = std::max<size_t>(s->heliostats.size(), 65536);
+        c->h_occ.reserve(s->heliostats.size());
+        c->h_pass.reserve(nres);
+        c->h_order.reserve(nres);
+        c->h_sorted.reserve(nres);
+        c->h_sorted_base.reserve(nres);
+        c->h_keys.reserve(nres);
+        c->h_tmp.reserve(nres);
+        c->h_spans.reserve(nres);
     }
     // Buie table of this configuration
     if (s->cfg.inverse_transform_sampling_groups > 0 && s->cfg.csr > 0.0f) {
@@ -1047,58 +1082,89 @@ namespace {
 
 constexpr int kBandBS = 1024;
 
-template <int RT>
+template <int RT, bool REC>
 void launch_band(dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph, float *parts) {
-    hipLaunchKernelGGL((k_trace_band<RT, kBandBS>), g, dim3(kBandBS), lds, st, a, row0, rows, bph, parts);
+    hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC>), g, dim3(kBandBS), lds, st, a, row0, rows, bph, parts);
 }
 
-// The banded persistent trace when it pays. Measured on MI355X: its persistent loop runs about 15%
-// slower than one lane per ray (traversal-heavy fields lose more), so it is used only where the band
-// takes most deposits off the global atomics — a band of at least half the map's rows, one
-// workgroup per CU with the whole LDS (4 waves per SIMD measured best) — and where every workgroup
-// deposits many times its band (its band store and fold are then a small part of the launch).
-qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, long long max_slots, bool &launched) {
+template <bool REC>
+hipError_t band_occupancy(int rt, int &resident, size_t lds) {
+    return rt == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<0, kBandBS, REC>, kBandBS, lds)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<1, kBandBS, REC>, kBandBS, lds);
+}
+
+// The banded persistent trace when it pays (or when forced, for parity checks of this kernel).
+// Measured on MI355X: its persistent loop runs about 15% slower than one lane per ray (traversal-heavy
+// fields lose more), so it is used only where the band takes most deposits off the global atomics — a
+// band of at least half the map's rows, one workgroup per CU with the whole LDS (4 waves per SIMD
+// measured best) — and where every workgroup deposits many times its band (its band store and fold
+// are then a small part of the launch).
+qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, long long max_slots, TraceBufs &b,
+                           bool force, bool &launched) {
     launched = false;
     const int per_cu = 1;
     if (qmc_status st = device_limits(c)) return st;
     const int W = rb.w, Hh = rb.h;
     const size_t lds_budget = std::min(c->max_lds, c->lds_per_cu / (size_t)per_cu);
     const int rows = (int)std::min<size_t>((size_t)Hh, lds_budget / (sizeof(float) * (size_t)W));
-    if (rows < 1 || 2 * rows < Hh) return QMC_OK;
+    const bool rec = a.records != nullptr;
+    if (rows < 1) {
+        if (force) return fail(QMC_E_UNSUPPORTED, "banded trace: one map row does not fit in LDS");
+        return QMC_OK;
+    }
+    if (!force && 2 * rows < Hh) return QMC_OK;
     const size_t lds = sizeof(float) * (size_t)rows * W;
     int resident = 0;
-    const hipError_t e = a.recv.type == 0
-                             ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<0, kBandBS>, kBandBS, lds)
-                             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<1, kBandBS>, kBandBS, lds);
-    if (e != hipSuccess || resident < 1) return QMC_OK;
+    const hipError_t e = rec ? band_occupancy<true>(a.recv.type, resident, lds)
+                             : band_occupancy<false>(a.recv.type, resident, lds);
+    if (e != hipSuccess || resident < 1) {
+        if (force) return fail(QMC_E_UNSUPPORTED, "banded trace: no workgroup of the banded kernel is resident");
+        return QMC_OK;
+    }
     const int nwg = c->num_cus * std::min(per_cu, resident);
     const long long bph = (max_slots + kBandBS - 1) / kBandBS;
     const long long total = bph * kBandBS * a.n_list;
     const long long nband = (long long)rows * W;
-    if (total < 8 * (long long)nwg * nband) return QMC_OK;
-    if ((size_t)nwg * nband > c->band_cap) {
+    if (!force && total < 8 * (long long)nwg * nband) return QMC_OK;
+    if ((size_t)nwg * nband > b.band_cap) {
+        if (b.fixed) {
+            if (force) return fail(QMC_E_RANGE, "banded trace: the workgroup bands exceed the scratch");
+            return QMC_OK;
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));  // earlier launches may still write the old bands
         qmc_status st = dev_alloc(c->d_band_partials, (size_t)nwg * nband);
         if (st) {
             c->band_cap = 0;
             return st;
         }
         c->band_cap = (size_t)nwg * nband;
+        b.band = c->d_band_partials;
+        b.band_cap = c->band_cap;
     }
     const int row0 = (Hh - rows) / 2;  // flux concentrates around the aim point, the receiver centre
-    if (c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
-    if (a.recv.type == 0) launch_band<0>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
-    else launch_band<1>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, c->d_band_partials);
+    const bool mark = !rec;
+    if (mark && c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
+    if (rec) {
+        if (a.recv.type == 0) launch_band<0, true>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        else launch_band<1, true>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+    } else {
+        if (a.recv.type == 0) launch_band<0, false>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        else launch_band<1, false>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+    }
     HIP_TRY(hipGetLastError());
-    if (c->kev_after >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_after], c->stream));
-    hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256), 8), dim3(256), 0, c->stream, rb.acc,
-                       c->d_band_partials, nwg, (int)nband, row0 * W);
+    if (mark && c->kev_after >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_after], c->stream));
+    hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256), 8), dim3(256), 0, c->stream, rb.acc, b.band,
+                       nwg, (int)nband, row0 * W);
     HIP_TRY(hipGetLastError());
     launched = true;
     return QMC_OK;
 }
 
-qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots,
-                        qmc_ray_record *rec, StatsAcc *stats) {
+// One launch over n list entries of one grid (d_list: their heliostat ids): culling kernels, the trace
+// kernel (QMC_KERNEL_*) and the fold of the launch's accumulator into the map. The wave masks of the
+// launch must fit in b (the caller sizes or splits launches).
+qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots, qmc_ray_record *rec,
+                        const long long *rec_base, StatsAcc *stats, int kernel, TraceBufs &b) {
     const GridBuf &gb = c->grids[grid];
     const RecvBuf &rb = c->recvs[gb.receiver];
     TraceArgs a;
@@ -1115,6 +1181,7 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     a.flux = rb.flux;
     a.acc = rb.acc;
     a.records = rec;
+    a.rec_base = rec_base;
     a.stats = stats;
     a.P = c->P;
     a.L = c->L;
@@ -1127,17 +1194,18 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     if (n > 65535) return fail(QMC_E_UNSUPPORTED, "trace: more than 65535 heliostats in one grid launch");
     a.blocks_per_helio = (int)bx;
     if (bx == 0 || n == 0) return QMC_OK;
+    qmc_status st;
     if (!stats) {
-        const CullList *lists = nullptr;
-        qmc_status st = launch_cull(c, grid, d_list, n, &lists, &a.wide);
-        if (!st) st = launch_cull_waves(c, grid, d_list, n, max_slots, a);
-        if (st) return st;
+        if ((st = launch_cull_into(c, grid, d_list, n, b.cull, b.wide))) return st;
+        a.wide = b.wide;
+        if (2 * (size_t)n * (size_t)wave_stride(max_slots) > b.wmask_cap)
+            return fail(QMC_E_STATE, "trace: wave masks exceed their buffer");
+        if ((st = launch_cull_waves_into(c, grid, d_list, n, max_slots, b.wmask, a))) return st;
     }
     const int npix = rb.w * rb.h;
-    if (!rec && !stats) {
+    if (!stats && kernel != QMC_KERNEL_PER_RAY) {
         bool launched = false;
-        const qmc_status st = try_band_launch(c, rb, a, max_slots, launched);
-        if (st) return st;
+        if ((st = try_band_launch(c, rb, a, max_slots, b, kernel == QMC_KERNEL_BANDED, launched))) return st;
         if (launched) {
             hipLaunchKernelGGL(k_acc_fold, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, rb.flux,
                                rb.acc, npix);
@@ -1145,19 +1213,19 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
             return QMC_OK;
         }
     }
-    dim3 g((unsigned)bx, (unsigned)n), b(256);
+    dim3 g((unsigned)bx, (unsigned)n), blk(256);
     const int rt = rb.r.type;
     const bool mark = !rec && !stats;
     if (mark && c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
     if (rec) {
-        if (rt == 0) hipLaunchKernelGGL((k_trace<0, true, false>), g, b, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_trace<1, true, false>), g, b, 0, c->stream, a);
+        if (rt == 0) hipLaunchKernelGGL((k_trace<0, true, false>), g, blk, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trace<1, true, false>), g, blk, 0, c->stream, a);
     } else if (stats) {
-        if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, true>), g, b, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_trace<1, false, true>), g, b, 0, c->stream, a);
+        if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, true>), g, blk, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trace<1, false, true>), g, blk, 0, c->stream, a);
     } else {
-        if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, false>), g, b, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_trace<1, false, false>), g, b, 0, c->stream, a);
+        if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, false>), g, blk, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_trace<1, false, false>), g, blk, 0, c->stream, a);
     }
     HIP_TRY(hipGetLastError());
     if (mark && c->kev_after >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_after], c->stream));
@@ -1173,70 +1241,210 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
 // their atomic adds meet on the same pixels less often (2650-heliostat 512x256 field: 6.92 ms against
 // 7.47 in layout order and 7.76 sorted by azimuth). Only the order of the double-precision
 // accumulation changes.
-void interleave_by_azimuth(const qmc_ctx *c, const DevReceiver &R, int32_t *ids, int n) {
+void interleave_by_azimuth(qmc_ctx *c, const DevReceiver &R, int32_t *ids, int n) {
     constexpr int kSectors = 64;
-    std::vector<std::pair<double, int32_t>> key((size_t)n);
+    std::vector<std::pair<double, int32_t>> &key = c->h_keys;
+    std::vector<int32_t> &was = c->h_tmp;
+    key.resize((size_t)n);
+    was.assign(ids, ids + n);
     for (int i = 0; i < n; ++i) {
         const DevHelio &H = c->h_helios[ids[i]];
-        key[(size_t)i] = {std::atan2((double)H.pos[2] - R.pos[2], (double)H.pos[0] - R.pos[0]), ids[i]};
+        key[(size_t)i] = {std::atan2((double)H.pos[2] - R.pos[2], (double)H.pos[0] - R.pos[0]), i};
     }
-    std::stable_sort(key.begin(), key.end(), [](const std::pair<double, int32_t> &x,
-                                                const std::pair<double, int32_t> &y) { return x.first < y.first; });
+    // azimuth order, task order among equal azimuths (std::sort: no temporary buffer)
+    std::sort(key.begin(), key.end());
     const int q = n / kSectors;  // heliostats per sector; the last n % kSectors follow in azimuth order
     for (int i = 0; i < n; ++i)
-        ids[i] = i < kSectors * q ? key[(size_t)((i % kSectors) * q + i / kSectors)].second : key[(size_t)i].second;
+        ids[i] = was[(size_t)(i < kSectors * q ? key[(size_t)((i % kSectors) * q + i / kSectors)].second
+                                               : key[(size_t)i].second)];
 }
 
-qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *rec, StatsAcc *stats) {
+// Ctx-owned buffers of a trace call of n entries (grown as needed; record bases when rec).
+qmc_status ctx_trace_bufs(qmc_ctx *c, int n, bool rec, size_t wmask_need, TraceBufs &b) {
+    qmc_status st;
+    const size_t nh = c->h_helios.size();
+    // launches queued by earlier calls may still read the buffers that are about to be replaced
+    if (n > c->list_cap || c->cull_cap < nh || c->wide_cap < nh || wmask_need > c->wmask_cap ||
+        (rec && (size_t)n > c->rec_base_cap))
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    if (n > c->list_cap) {
+        if ((st = dev_alloc(c->d_list, (size_t)n))) {
+            c->list_cap = 0;
+            return st;
+        }
+        c->list_cap = n;
+    }
+    if (c->cull_cap < nh) {
+        if ((st = dev_alloc(c->d_cull, nh))) {
+            c->cull_cap = 0;
+            return st;
+        }
+        c->cull_cap = nh;
+    }
+    if (c->wide_cap < nh) {
+        if ((st = dev_alloc(c->d_wide, nh))) {
+            c->wide_cap = 0;
+            return st;
+        }
+        c->wide_cap = nh;
+    }
+    if (wmask_need > c->wmask_cap) {
+        if ((st = dev_alloc(c->d_wmask, wmask_need))) {
+            c->wmask_cap = 0;
+            return st;
+        }
+        c->wmask_cap = wmask_need;
+    }
+    if (rec && (size_t)n > c->rec_base_cap) {
+        if ((st = dev_alloc(c->d_rec_base, (size_t)n))) {
+            c->rec_base_cap = 0;
+            return st;
+        }
+        c->rec_base_cap = (size_t)n;
+    }
+    b.list = c->d_list;
+    b.cull = c->d_cull;
+    b.wide = c->d_wide;
+    b.wmask = c->d_wmask;
+    b.wmask_cap = c->wmask_cap;
+    b.band = c->d_band_partials;
+    b.band_cap = c->band_cap;
+    b.fixed = false;
+    return QMC_OK;
+}
+
+// Ray slots of a heliostat in a plain trace: rays, or tiled ray slots of flat heliostats (k_trace).
+long long heliostat_slots(const qmc_ctx *c, const DevHelio &H) {
+    return H.type == 0 ? (long long)facet_tiling(H.rows * H.sub_rows, H.cols * H.sub_cols, c->L).slots
+                       : (long long)H.F * c->L;
+}
+
+// A trace call over n list entries. Each occurrence of a heliostat in the list is one reference
+// rayTracing call (QuasiMonteCarloRayTracer.cpp:15-91); occurrences are traced in passes — pass p
+// holds every heliostat's (p+1)-th occurrence — so that no launch lists a heliostat twice (its
+// candidate lists are written per heliostat id). Within a pass heliostats are traced grid by grid in
+// task order (a plain trace of a cylinder grid interleaves them). With resampling on, each pass first
+// draws fresh group offsets for its heliostats. rec_base_in (with rec): first record of each entry.
+// fixed: buffers from the scratch array (b), no allocation.
+qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *rec, const long long *rec_base_in,
+                        StatsAcc *stats, int kernel, TraceBufs *fixed) {
     if (!c->uploaded) return fail(QMC_E_STATE, "trace: upload a scene first");
     if (!c->pool_ready) return fail(QMC_E_STATE, "trace: set or generate the sample pools first");
     if (!c->offsets_ready) return fail(QMC_E_STATE, "trace: set or generate the micro-facet group offsets first");
     if (n < 0 || (n > 0 && !ids)) return fail(QMC_E_INVALID_ARG, "trace: bad heliostat list");
+    if (kernel != QMC_KERNEL_AUTO && kernel != QMC_KERNEL_PER_RAY && kernel != QMC_KERNEL_BANDED)
+        return fail(QMC_E_INVALID_ARG, "trace: unknown kernel choice " + std::to_string(kernel));
     const int nh = (int)c->h_helios.size();
     for (int i = 0; i < n; ++i)
         if (ids[i] < 0 || ids[i] >= nh)
             return fail(QMC_E_RANGE, "The ray tracing heliostat index " + std::to_string(ids[i]) +
                                          " is invalid.The heliostat index should between 0 and " + std::to_string(nh) + ".");
+    if (n == 0) return QMC_OK;
     qmc_status st = set_device(c);
     if (st) return st;
-    // group by grid, keep task order inside a grid (a plain trace of a cylinder grid interleaves it)
-    std::vector<int32_t> sorted;
-    sorted.reserve(n);
-    std::vector<std::pair<int, int>> spans;  // (grid, count)
-    for (size_t g = 0; g < c->grids.size(); ++g) {
-        int before = (int)sorted.size();
-        for (int i = 0; i < n; ++i)
-            if (c->h_helios[ids[i]].grid == (int)g) sorted.push_back(ids[i]);
-        spans.emplace_back((int)g, (int)sorted.size() - before);
-        const DevReceiver &R = c->recvs[c->grids[g].receiver].r;
-        if (!rec && !stats && R.type == 1)
-            interleave_by_azimuth(c, R, sorted.data() + before, (int)sorted.size() - before);
+    // pass of each entry: the number of earlier occurrences of its heliostat
+    std::vector<int32_t> &occ = c->h_occ, &pass = c->h_pass, &order = c->h_order, &sorted = c->h_sorted;
+    std::vector<long long> &sbase = c->h_sorted_base;
+    std::vector<TraceSpan> &spans = c->h_spans;
+    occ.assign((size_t)nh, 0);
+    pass.resize((size_t)n);
+    order.resize((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        pass[i] = occ[ids[i]]++;
+        order[i] = i;
     }
-    if (sorted.empty()) return QMC_OK;
-    HIP_TRY(hipMemcpyAsync(c->d_list, sorted.data(), sizeof(int32_t) * sorted.size(), hipMemcpyHostToDevice, c->stream));
-    int at = 0;
-    for (auto &sp : spans) {
-        if (sp.second == 0) continue;
-        long long max_slots = 0;  // rays, or tiled ray slots of flat heliostats (see k_trace)
-        for (int i = at; i < at + sp.second; ++i) {
-            const DevHelio &H = c->h_helios[sorted[i]];
-            const long long slots = H.type == 0 ? (long long)facet_tiling(H.rows * H.sub_rows, H.cols * H.sub_cols,
-                                                                          c->L).slots
-                                                : (long long)H.F * c->L;
-            max_slots = std::max(max_slots, slots);
-        }
+    // entries ordered by (pass, grid, task order) (std::sort: no temporary buffer)
+    std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+        if (pass[x] != pass[y]) return pass[x] < pass[y];
+        const int gx = c->h_helios[ids[x]].grid, gy = c->h_helios[ids[y]].grid;
+        return gx != gy ? gx < gy : x < y;
+    });
+    sorted.resize((size_t)n);
+    sbase.resize(rec ? (size_t)n : 0);
+    spans.clear();
+    for (int i = 0; i < n; ++i) {
+        sorted[i] = ids[order[i]];
+        if (rec) sbase[i] = rec_base_in[order[i]];
+    }
+    size_t wmask_need = 0;
+    for (int i = 0; i < n;) {
+        const int p = pass[order[i]], g = c->h_helios[sorted[i]].grid;
+        int e = i + 1;
+        while (e < n && pass[order[e]] == p && c->h_helios[sorted[e]].grid == g) ++e;
+        const DevReceiver &R = c->recvs[c->grids[g].receiver].r;
+        if (!rec && !stats && R.type == 1) interleave_by_azimuth(c, R, sorted.data() + i, e - i);
+        long long max_slots = 0;
+        for (int k = i; k < e; ++k) max_slots = std::max(max_slots, heliostat_slots(c, c->h_helios[sorted[k]]));
         if (max_slots > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
-        if ((st = launch_trace(c, sp.first, c->d_list + at, sp.second, max_slots, rec, stats))) return st;
-        at += sp.second;
+        spans.push_back(TraceSpan{p, g, i, e - i, max_slots});
+        wmask_need = std::max(wmask_need, 2 * (size_t)(e - i) * (size_t)wave_stride(max_slots));
+        i = e;
+    }
+    TraceBufs b;
+    if (fixed) {
+        b = *fixed;
+    } else if ((st = ctx_trace_bufs(c, n, rec != nullptr, stats ? 0 : wmask_need, b))) {
+        return st;
+    }
+    HIP_TRY(hipMemcpyAsync(b.list, sorted.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    if (rec)
+        HIP_TRY(hipMemcpyAsync(c->d_rec_base, sbase.data(), sizeof(long long) * (size_t)n, hipMemcpyHostToDevice,
+                               c->stream));
+    int fresh_pass = -1;
+    for (size_t si = 0; si < spans.size(); ++si) {
+        const TraceSpan &sp = spans[si];
+        if (c->resample && sp.pass != fresh_pass) {
+            // the pass's entries are contiguous in the list: spans si.. of this pass
+            int end = sp.begin + sp.count;
+            for (size_t sj = si + 1; sj < spans.size() && spans[sj].pass == sp.pass; ++sj) end = spans[sj].begin + spans[sj].count;
+            int maxf = 0;
+            for (int k = sp.begin; k < end; ++k) maxf = std::max(maxf, c->h_helios[sorted[k]].F);
+            const uint64_t seed = c->resample_seed + c->resample_next++;
+            hipLaunchKernelGGL(k_offsets_list, dim3((unsigned)((maxf + 1 + 255) / 256), (unsigned)(end - sp.begin)),
+                               dim3(256), 0, c->stream, c->d_offsets, c->d_helios, b.list + sp.begin, seed, c->P);
+            HIP_TRY(hipGetLastError());
+            fresh_pass = sp.pass;
+        }
+        // a fixed mask buffer takes the span in chunks of whole heliostats
+        int chunk = sp.count;
+        if (b.fixed && !stats) {
+            const size_t per = 2 * (size_t)wave_stride(sp.max_slots);
+            const size_t fit = b.wmask_cap / per;
+            if (fit == 0)
+                return fail(QMC_E_RANGE, "trace: one heliostat's wave masks (" + std::to_string(per * 8) +
+                                             " B) exceed the scratch");
+            chunk = (int)std::min<size_t>((size_t)sp.count, fit);
+        }
+        for (int k = 0; k < sp.count; k += chunk) {
+            const int m = std::min(chunk, sp.count - k);
+            if ((st = launch_trace(c, sp.grid, b.list + sp.begin + k, m, sp.max_slots, rec,
+                                   rec ? c->d_rec_base + sp.begin + k : nullptr, stats, kernel, b)))
+                return st;
+        }
     }
     return QMC_OK;
 }
 
 }  // namespace
 
+qmc_status qmc::trace_fixed(qmc_ctx *c, const int32_t *ids, int n, const TraceBufs &b) {
+    TraceBufs fb = b;
+    fb.fixed = true;
+    return trace_common(c, ids, n, nullptr, nullptr, nullptr, QMC_KERNEL_AUTO, &fb);
+}
+
 extern "C" qmc_status qmc_trace(qmc_ctx *c, const int32_t *ids, int32_t n) {
     if (!c) return fail(QMC_E_INVALID_ARG, "qmc_trace: null ctx");
-    return trace_common(c, ids, n, nullptr, nullptr);
+    return trace_common(c, ids, n, nullptr, nullptr, nullptr, QMC_KERNEL_AUTO, nullptr);
+}
+
+extern "C" qmc_status qmc_offsets_resample(qmc_ctx *c, int32_t enable, uint64_t seed) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_offsets_resample: null ctx");
+    if (enable != 0 && enable != 1) return fail(QMC_E_INVALID_ARG, "qmc_offsets_resample: enable must be 0 or 1");
+    c->resample = enable == 1;
+    c->resample_seed = seed;
+    c->resample_next = 0;
+    return QMC_OK;
 }
 
 extern "C" qmc_status qmc_cull_flags(qmc_ctx *c, const int32_t *ids, int32_t n, uint8_t *flags) {
@@ -1249,20 +1457,60 @@ extern "C" qmc_status qmc_cull_flags(qmc_ctx *c, const int32_t *ids, int32_t n, 
     qmc_status st = set_device(c);
     if (st) return st;
     std::vector<CullList> all((size_t)nh);
+    std::vector<char> seen((size_t)nh, 0);
     for (size_t g = 0; g < c->grids.size(); ++g) {
+        // each heliostat once per launch: k_cull writes its lists per heliostat id
         std::vector<int32_t> list;
         for (int i = 0; i < n; ++i)
-            if (c->h_helios[ids[i]].grid == (int)g) list.push_back(ids[i]);
+            if (c->h_helios[ids[i]].grid == (int)g && !seen[ids[i]]) {
+                seen[ids[i]] = 1;
+                list.push_back(ids[i]);
+            }
         if (list.empty()) continue;
-        HIP_TRY(hipMemcpyAsync(c->d_list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, c->stream));
-        const CullList *d_lists = nullptr;
-        if ((st = launch_cull(c, (int)g, c->d_list, (int)list.size(), &d_lists, nullptr))) return st;
-        HIP_TRY(hipMemcpyAsync(all.data(), d_lists, sizeof(CullList) * (size_t)nh, hipMemcpyDeviceToHost, c->stream));
+        TraceBufs b;
+        if ((st = ctx_trace_bufs(c, (int)list.size(), false, 0, b))) return st;
+        HIP_TRY(hipMemcpyAsync(b.list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, c->stream));
+        if ((st = launch_cull_into(c, (int)g, b.list, (int)list.size(), b.cull, nullptr))) return st;
+        HIP_TRY(hipMemcpyAsync(all.data(), b.cull, sizeof(CullList) * (size_t)nh, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));  // the list and candidate buffers are reused per grid
         for (int i = 0; i < n; ++i)
             if (c->h_helios[ids[i]].grid == (int)g)
                 flags[i] = (uint8_t)((all[ids[i]].n[0] > 0 ? 1 : 0) | (all[ids[i]].n[1] > 0 ? 2 : 0));
     }
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_trace_records_many(qmc_ctx *c, const int32_t *ids, int32_t n, int32_t kernel,
+                                             qmc_ray_record *rec, int64_t n_records) {
+    if (!c || !rec || (n > 0 && !ids)) return fail(QMC_E_INVALID_ARG, "qmc_trace_records_many: null argument");
+    if (!c->uploaded) return fail(QMC_E_STATE, "qmc_trace_records_many: upload a scene first");
+    const int nh = (int)c->h_helios.size();
+    std::vector<long long> base((size_t)std::max(n, 0));
+    std::vector<char> seen((size_t)nh, 0);
+    long long need = 0;
+    for (int i = 0; i < n; ++i) {
+        if (ids[i] < 0 || ids[i] >= nh) return fail(QMC_E_RANGE, "qmc_trace_records_many: heliostat out of range");
+        if (seen[ids[i]]) return fail(QMC_E_INVALID_ARG, "qmc_trace_records_many: heliostat " +
+                                                             std::to_string(ids[i]) + " listed twice");
+        seen[ids[i]] = 1;
+        base[i] = need;
+        need += (long long)c->h_helios[ids[i]].F * c->L;
+    }
+    if (n_records != need)
+        return fail(QMC_E_INVALID_ARG, "qmc_trace_records_many: need sum(F*L) = " + std::to_string(need) + " records");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    if ((size_t)need > c->rec_cap) {
+        if ((st = dev_alloc(c->d_rec, (size_t)std::max<long long>(need, 1)))) {
+            c->rec_cap = 0;
+            return st;
+        }
+        c->rec_cap = (size_t)need;
+    }
+    if ((st = trace_common(c, ids, n, c->d_rec, base.data(), nullptr, kernel, nullptr))) return st;
+    if (need > 0)
+        HIP_TRY(hipMemcpyAsync(rec, c->d_rec, sizeof(qmc_ray_record) * (size_t)need, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return QMC_OK;
 }
 
@@ -1272,17 +1520,7 @@ extern "C" qmc_status qmc_trace_records(qmc_ctx *c, int32_t h, qmc_ray_record *r
     if (h < 0 || h >= (int)c->h_helios.size()) return fail(QMC_E_RANGE, "qmc_trace_records: heliostat out of range");
     const int64_t need = (int64_t)c->h_helios[h].F * c->L;
     if (n != need) return fail(QMC_E_INVALID_ARG, "qmc_trace_records: need F*L = " + std::to_string(need) + " records");
-    qmc_status st = set_device(c);
-    if (st) return st;
-    if ((size_t)need > c->rec_cap) {
-        if ((st = dev_alloc(c->d_rec, (size_t)need))) return st;
-        c->rec_cap = (size_t)need;
-    }
-    int32_t id = h;
-    if ((st = trace_common(c, &id, 1, c->d_rec, nullptr))) return st;
-    HIP_TRY(hipMemcpyAsync(rec, c->d_rec, sizeof(qmc_ray_record) * need, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return QMC_OK;
+    return qmc_trace_records_many(c, &h, 1, QMC_KERNEL_AUTO, rec, n);
 }
 
 extern "C" qmc_status qmc_trace_stats_run(qmc_ctx *c, const int32_t *ids, int32_t n, qmc_trace_stats *out) {
@@ -1290,7 +1528,7 @@ extern "C" qmc_status qmc_trace_stats_run(qmc_ctx *c, const int32_t *ids, int32_
     qmc_status st = set_device(c);
     if (st) return st;
     HIP_TRY(hipMemsetAsync(c->d_stats, 0, sizeof(StatsAcc), c->stream));
-    if ((st = trace_common(c, ids, n, nullptr, c->d_stats))) return st;
+    if ((st = trace_common(c, ids, n, nullptr, nullptr, c->d_stats, QMC_KERNEL_AUTO, nullptr))) return st;
     StatsAcc h;
     HIP_TRY(hipMemcpyAsync(&h, c->d_stats, sizeof(StatsAcc), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

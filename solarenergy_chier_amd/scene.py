@@ -343,6 +343,12 @@ class Device:
     def offsets_generate(self, seed: int = 3) -> None:
         N.call("qmc_offsets_generate", self.handle, C.c_uint64(seed))
 
+    def offsets_resample(self, enable: bool, seed: int = 0) -> None:
+        """Per-call resampling of the group offsets (the reference draws fresh offsets on every
+        rayTracing call): when on, trace number k since this call first regenerates the offsets of the
+        heliostats it traces with seed + k (one k per pass: a heliostat listed twice is traced twice)."""
+        N.call("qmc_offsets_resample", self.handle, 1 if enable else 0, C.c_uint64(seed))
+
     def offsets_set(self, h: int, offsets: np.ndarray) -> None:
         a = np.ascontiguousarray(offsets, np.int32)
         N.call("qmc_offsets_set", self.handle, h, N.iptr(a), len(a))
@@ -379,6 +385,18 @@ class Device:
     def trace_records(self, h: int, n_rays: int) -> np.ndarray:
         rec = np.zeros(n_rays, dtype=N.RECORD_DTYPE)
         N.call("qmc_trace_records", self.handle, h, rec.ctypes.data_as(C.POINTER(N.RayRecord)), n_rays)
+        return rec
+
+    def trace_records_many(self, ids: Iterable[int], kernel: int = N.KERNEL_AUTO) -> np.ndarray:
+        """One trace of distinct heliostats with the chosen kernel (KERNEL_AUTO: the one trace() would
+        run); returns every ray's record, heliostat after heliostat in list order, each F*L records
+        indexed by ray id."""
+        a = np.ascontiguousarray(list(ids), np.int32)
+        L = self.scene.config.num_sunshape_lights_per_group
+        n = sum(self.scene.heliostat_info(int(h)).n_micro for h in a) * L
+        rec = np.zeros(n, dtype=N.RECORD_DTYPE)
+        N.call("qmc_trace_records_many", self.handle, N.iptr(a), len(a), kernel,
+               rec.ctypes.data_as(C.POINTER(N.RayRecord)), n)
         return rec
 
     def trace_stats(self, ids: Iterable[int]) -> dict:
