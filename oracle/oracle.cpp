@@ -7,7 +7,8 @@
 //
 // Function -> reference (JachyLikeCoding/SolarEnergy_Chier) -> what pins it (tests/):
 //   vec math (dot, cross, normalize = v*(1/sqrtf), reflect)  Util/vector_arithmetic.cuh:50-53,379-413
-//                                                   -> closed forms in test_oracle_primitives.py
+//                                                   -> the Moeller-Trumbore and energy closed forms
+//                                                   of test_oracle_invariants.py that use them
 //   local2world / focus rotate        Util/global_function.cuh:19-36; FocusFlatRectangleHelio.cuh:15-46
 //                                                   -> orthonormality + KATs below
 //   rotateY                           Util/global_function.cuh:51-59 -> receiver vertex closed forms
@@ -15,19 +16,27 @@
 //                                                   closed forms and brute-force grids
 //   receiver init (rect / cylinder)   RectangleReceiver.cu:9-101; CylinderReceiver.cu:4-31
 //                                                   -> CylinderReceiverTest KATs (resolution 57x12,
-//                                                   focus centres), rectangle closed forms
+//                                                   focus centres), rectangle closed forms; the tilt
+//                                                   and resolution-override extensions -> vertex
+//                                                   closed forms of the tilted 12 x 12 m receiver and
+//                                                   a parallel-sun energy balance independent of the
+//                                                   pixel area (test_oracle_pins.py)
 //   cylinder GIntersect               CylinderReceiver.cuh:39-101 -> CylinderReceiverTest KATs
 //   grid Cinit + CGridHelioMatch      RectangleGrid.cu:8-117 -> RectangleGridTest KATs
 //   CSetNormal / CSetWorldVertex      Heliostat.cu:15-40 -> QuasiMonteCarloRayTracerTest vertex KATs
 //   flat micro-facets                 RectangleHelio.cu:54-116 -> RectangleHelioTest KATs,
 //                                                   QuasiMonteCarloRayTracerTest origin KATs
 //   focus-flat facets / micro-facets  FocusFlatRectangleHelio.cu:12-227,333-339
-//                                                   -> FocusFlatRectangleHelioTest KATs
+//                                                   -> FocusFlatRectangleHelioTest KATs, read method
+//                                                   -1 (analytic) and 1 (subhelio data: the same KAT
+//                                                   from its centres and normals, test_oracle_pins.py)
 //   setFlatRectangleHeliostatVertexs + subHeliostat_id  QuasiMonteCarloRayTracer.cpp:135-209
 //                                                   -> subHeliostat_id == 7 KAT
 //   factor                            QuasiMonteCarloRayTracer.cpp:58-60 -> energy conservation
 //   Buie table + inverse CDF          Scene/SceneProcessor.cu:7-81,196-243 -> numeric integration of
-//                                                   the Buie profile, KS test of the sampled theta
+//                                                   the Buie profile, KS tests of the sampled theta:
+//                                                   core against the table, circumsolar tail (after
+//                                                   undoing the remap) against e^k/(g+1) t^(g+1)
 //   angle2xyz / map_permuation        Scene/SceneProcessor.cu:83-103; global_function.cuh:211-216
 //                                                   -> unit norm, phi uniformity, slope moments
 //   map_float2int                     Tool/RandomNumberGenerator/RandomGenerator.cu:20-25 -> range
@@ -39,7 +48,14 @@
 //   receiver_drawing / calEnergy      Receiver/*.cu; receiverIntersectionUtil.cuh:17-34
 //                                                   -> energy conservation closed form
 //   trimmed_mean + heaps              Tool/Smoother/ImageSmoother.cu:13-64; heap.cuh:16-225
-//                                                   -> sort-based textbook trimmed mean (k <= 1)
+//                                                   -> sort-based textbook trimmed mean (k <= 1; any
+//                                                   k on windows read in ascending or descending
+//                                                   order) and a hand-derived 5x5 KAT on which the
+//                                                   reference's `else if` departs from the textbook
+//                                                   (test_oracle_pins.py)
+// test_oracle_pins.py also plants a slip in a copy of this file for each of these pins (the `else
+// if`, a heap child, the tilt sign, the pixel area, the tail exponent, read method 1) and checks
+// that the pin rejects it.
 //   physical Buie sampler (QMC_SUNSHAPE_BUIE, no reference counterpart) -> KS test of sampled angles
 //                                     against quadrature of Buie's profile; the table itself against
 //                                     independent quadrature

@@ -27,44 +27,63 @@ def lib():
         if not LIB.exists():
             from oracle import build as ob
             ob.build()
-        l = C.CDLL(str(LIB))
-        sig = {
-            "orc_scene_build": (_P, [C.POINTER(N.Config), C.POINTER(N.ReceiverDesc), C.c_int, C.POINTER(N.GridDesc),
-                                     C.c_int, C.POINTER(N.HeliostatDesc), C.c_int, C.c_char_p, C.c_int]),
-            "orc_scene_free": (None, [_P]),
-            "orc_receiver_info": (C.c_int, [_P, C.c_int, C.POINTER(N.ReceiverInfo)]),
-            "orc_heliostat_info": (C.c_int, [_P, C.c_int, C.POINTER(N.HeliostatInfo)]),
-            "orc_grid_info": (C.c_int, [_P, C.c_int, C.POINTER(N.GridInfo)]),
-            "orc_grid_cells": (C.c_int, [_P, C.c_int, _IP, _IP]),
-            "orc_subheliostat_vertices": (C.c_int, [_P, C.c_int, _FP]),
-            "orc_factor": (C.c_float, [_P, C.c_int]),
-            "orc_facets": (C.c_int, [_P, C.c_int, _FP, _FP]),
-            "orc_micro_count": (C.c_int, [_P, C.c_int]),
-            "orc_sunshape_table": (None, [C.POINTER(N.Config), _FP, _FP, _FP]),
-            "orc_pool_from_uniforms": (None, [C.POINTER(N.Config), _FP, _FP, _FP, _FP, C.c_int, _FP, _FP]),
-            "orc_pool_generate": (None, [C.POINTER(N.Config), C.c_uint64, C.c_uint64, _FP, _FP, _FP, _FP, _FP, _FP]),
-            "orc_table_theta": (None, [_DP, _DP, _DP, _DP, C.c_int, _DP, C.c_int, _DP]),
-            "orc_pool_generate_buie": (None, [C.POINTER(N.Config), C.c_uint64, C.c_uint64, _DP, _DP, _DP, _DP,
-                                              C.c_int, _FP, _FP]),
-            "orc_offsets_generate": (None, [_P, C.c_uint64, C.c_int, C.c_int, _IP]),
-            "orc_trace_heliostat": (C.c_int, [_P, C.c_int, _FP, _FP, C.c_int, C.c_int, _IP, _DP,
-                                              C.POINTER(N.RayRecord), C.POINTER(N.TraceStats)]),
-            "orc_trace_many": (C.c_int, [_P, _IP, C.c_int, _FP, _FP, C.c_int, C.c_int, _IP, _DP, C.c_int,
-                                         C.c_int64, C.POINTER(N.TraceStats)]),
-            "orc_smooth": (None, [_FP, C.c_int, C.c_int, C.c_int, C.c_float]),
-            "orc_ray_parallelogram": (C.c_int, [_FP, _FP, _FP, _FP, _FP, _FP]),
-            "orc_cylinder_intersect": (C.c_int, [_FP, _FP, _FP, _FP, _FP]),
-            "orc_cylinder_focus": (None, [_FP, _FP, _FP, _FP]),
-            "orc_cylinder_resolution": (C.c_int, [_FP, C.c_int, _IP]),
-            "orc_dda": (C.c_int, [_P, C.c_int, _FP, _FP, C.c_int, C.c_int, _IP]),
-            "orc_brute_any_hit": (C.c_int, [_P, C.c_int, _FP, _FP, C.c_int, C.c_int]),
-        }
-        for name, (res, args) in sig.items():
-            fn = getattr(l, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = l
+        _lib = load(LIB)
     return _lib
+
+
+def use_library(path):
+    """Make the oracle functions of this module call the library at `path` (a build of oracle.cpp, for
+    instance one with a planted slip); returns the library in use before, for restoring."""
+    global _lib
+    before = lib()
+    _lib = load(path)
+    return before
+
+
+def restore_library(before):
+    global _lib
+    _lib = before
+
+
+def load(path):
+    """Load a build of oracle.cpp and declare its signatures."""
+    l = C.CDLL(str(path))
+    sig = {
+        "orc_scene_build": (_P, [C.POINTER(N.Config), C.POINTER(N.ReceiverDesc), C.c_int, C.POINTER(N.GridDesc),
+                                 C.c_int, C.POINTER(N.HeliostatDesc), C.c_int, C.c_char_p, C.c_int]),
+        "orc_scene_free": (None, [_P]),
+        "orc_receiver_info": (C.c_int, [_P, C.c_int, C.POINTER(N.ReceiverInfo)]),
+        "orc_heliostat_info": (C.c_int, [_P, C.c_int, C.POINTER(N.HeliostatInfo)]),
+        "orc_grid_info": (C.c_int, [_P, C.c_int, C.POINTER(N.GridInfo)]),
+        "orc_grid_cells": (C.c_int, [_P, C.c_int, _IP, _IP]),
+        "orc_subheliostat_vertices": (C.c_int, [_P, C.c_int, _FP]),
+        "orc_factor": (C.c_float, [_P, C.c_int]),
+        "orc_facets": (C.c_int, [_P, C.c_int, _FP, _FP]),
+        "orc_micro_count": (C.c_int, [_P, C.c_int]),
+        "orc_sunshape_table": (None, [C.POINTER(N.Config), _FP, _FP, _FP]),
+        "orc_pool_from_uniforms": (None, [C.POINTER(N.Config), _FP, _FP, _FP, _FP, C.c_int, _FP, _FP]),
+        "orc_pool_generate": (None, [C.POINTER(N.Config), C.c_uint64, C.c_uint64, _FP, _FP, _FP, _FP, _FP, _FP]),
+        "orc_table_theta": (None, [_DP, _DP, _DP, _DP, C.c_int, _DP, C.c_int, _DP]),
+        "orc_pool_generate_buie": (None, [C.POINTER(N.Config), C.c_uint64, C.c_uint64, _DP, _DP, _DP, _DP,
+                                          C.c_int, _FP, _FP]),
+        "orc_offsets_generate": (None, [_P, C.c_uint64, C.c_int, C.c_int, _IP]),
+        "orc_trace_heliostat": (C.c_int, [_P, C.c_int, _FP, _FP, C.c_int, C.c_int, _IP, _DP,
+                                          C.POINTER(N.RayRecord), C.POINTER(N.TraceStats)]),
+        "orc_trace_many": (C.c_int, [_P, _IP, C.c_int, _FP, _FP, C.c_int, C.c_int, _IP, _DP, C.c_int,
+                                     C.c_int64, C.POINTER(N.TraceStats)]),
+        "orc_smooth": (None, [_FP, C.c_int, C.c_int, C.c_int, C.c_float]),
+        "orc_ray_parallelogram": (C.c_int, [_FP, _FP, _FP, _FP, _FP, _FP]),
+        "orc_cylinder_intersect": (C.c_int, [_FP, _FP, _FP, _FP, _FP]),
+        "orc_cylinder_focus": (None, [_FP, _FP, _FP, _FP]),
+        "orc_cylinder_resolution": (C.c_int, [_FP, C.c_int, _IP]),
+        "orc_dda": (C.c_int, [_P, C.c_int, _FP, _FP, C.c_int, C.c_int, _IP]),
+        "orc_brute_any_hit": (C.c_int, [_P, C.c_int, _FP, _FP, C.c_int, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(l, name)
+        fn.restype = res
+        fn.argtypes = args
+    return l
 
 
 def f32(a) -> np.ndarray:

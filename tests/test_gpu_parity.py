@@ -167,9 +167,15 @@ def test_rectangle_reference_scene(device):
 
 
 def test_focus_flat_reference_scenes(device):
+    """The reference's focus-flat inputs, analytic facets (read method -1) and facets read from the
+    file's subhelio blocks (read method 1: PS10_2_typical_4x7, focusflat_example_2x4)."""
     for scn, cfg in [("inputs/PS10_typical/PS10_typical_4x7_scene.scn",
                       "inputs/PS10_typical/PS10_typical_4x7_configuration.json"),
                      ("inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_scene.scn",
+                      "inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_configuration.json"),
+                     ("inputs/PS10_typical/PS10_2_typical_4x7_scene.scn",
+                      "inputs/PS10_typical/PS10_2_typical_configuration.json"),
+                     ("inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_2x4_scene.scn",
                       "inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_configuration.json")]:
         scene = scene_from_files(scn, cfg)
         assert parity_on_scene(device, scene, range(scene.counts.n_heliostats)) == 0

@@ -78,6 +78,10 @@ def test_cylinder_field():
      "inputs/CompleteInputFilesExample/twoReceiverConfiguration.json"),
     ("inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_scene.scn",
      "inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_configuration.json"),
+    # read method 1: subhelio centres and normals from the file
+    ("inputs/PS10_typical/PS10_2_typical_4x7_scene.scn", "inputs/PS10_typical/PS10_2_typical_configuration.json"),
+    ("inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_2x4_scene.scn",
+     "inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_configuration.json"),
 ])
 def test_reference_scene_files(scn, cfg):
     s = Scene.from_files(str(K.FIX / scn), str(K.FIX / cfg))

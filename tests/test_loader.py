@@ -1,5 +1,6 @@
 """Input formats kept from the reference: .scn scenes (SceneLoaderTest), .json configurations
 (SceneConfigurationTest), task files (TaskLoaderTest) and the tracer's index validation."""
+import numpy as np
 import pytest
 
 import kat_scenes as K
@@ -48,12 +49,38 @@ def test_focus_flat_scene_loads():
 
 
 def test_subhelio_block_read_method_1():
+    """Scene/SceneLoader.cpp:311-333: the subhelio block after `surface_property -80 1 ...` gives the
+    next heliostat rows*cols local centres, then rows*cols local normals, as the file lists them; the
+    block is consumed by that heliostat (SceneLoader.cpp:300-308 clears it), so the next heliostat,
+    whose read method stays 1, has none."""
     text = (F / "inputs/FocusFlatRectangleHeliostatInputs/focusflat_example_2x4_scene.scn").read_text()
     s = Scene()
     N.call("qmc_scene_load_text", s.handle, text.encode())
     d = s.heliostat_desc(0)
-    if int(d.surface_property[1]) == 1:
-        assert bool(d.local_centers) and bool(d.local_normals)
+    assert (d.rows, d.cols) == (2, 4)
+    assert int(d.surface_property[1]) == 1 and d.surface_property[0] == -80.0
+    assert bool(d.local_centers) and bool(d.local_normals)
+    lc = np.ctypeslib.as_array(d.local_centers, shape=(8, 3))
+    ln = np.ctypeslib.as_array(d.local_normals, shape=(8, 3))
+    np.testing.assert_array_equal(lc[0], np.float32([-4.8525, 0.122037, -2.3875]))
+    np.testing.assert_array_equal(lc[7], np.float32([4.8525, 0.122037, 2.3875]))
+    np.testing.assert_array_equal(ln[1], np.float32([0.00796718, 0.999899, 0.0117599]))
+    np.testing.assert_array_equal(ln[7], np.float32([-0.0238955, 0.999645, -0.0117569]))
+    d1 = s.heliostat_desc(1)
+    assert (d1.rows, d1.cols) == (1, 1) and int(d1.surface_property[1]) == 1
+    assert not bool(d1.local_centers) and not bool(d1.local_normals)
+    ps = Scene()
+    N.call("qmc_scene_load_text", ps.handle,
+           (F / "inputs/PS10_typical/PS10_2_typical_4x7_scene.scn").read_text().encode())
+    # heliostat 0 reads its facets from the file; heliostat 1 (surface_property -1 -1 ...) lists a
+    # subhelio block too, but read method -1 computes its facets from the paraboloid
+    assert ps.counts.n_heliostats == 2
+    d0, d1 = ps.heliostat_desc(0), ps.heliostat_desc(1)
+    assert (d0.rows, d0.cols, int(d0.surface_property[1])) == (4, 7, 1)
+    assert bool(d0.local_centers) and bool(d0.local_normals)
+    np.testing.assert_allclose(np.ctypeslib.as_array(d0.local_centers, shape=(28, 3))[0],
+                               [-4.35, 0.0151471312655876, -4.965], rtol=1e-6)
+    assert (d1.rows, d1.cols, int(d1.surface_property[1])) == (4, 7, -1)
 
 
 # Tests/DataStructureTests/SceneConfigurationTest.cpp:32-68
