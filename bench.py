@@ -1,12 +1,13 @@
 """Benchmark: rays/s of the plain tracer on one GPU, PS10-like field (624 heliostats, 483M rays per
-step, 256x256 flux map), with the trace kernel's algorithmic-byte roofline and the CPU oracle timed
-on a bounded sample of the same workload.
+step, 256x256 flux map), with the trace kernel's vector-issue roofline and the CPU oracle timed on a
+bounded sample of the same workload.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ps10|small] [--no-cpu-baseline]
 
-A step = clear the flux map + trace every heliostat of the field (one trace call: the culling
-kernel, the trace kernel and its folds) with the sample pools and micro-facet offsets resident in
-HBM. Prints ONE JSON line on stdout.
+A step = clear the flux map + trace every heliostat of the field in one trace call: fresh micro-facet
+group offsets for every heliostat (as the reference draws them on each rayTracing call), the culling
+kernels, the trace kernel and its folds, with the scene and sample pools resident in HBM. Prints ONE
+JSON line on stdout.
 """
 from __future__ import annotations
 
@@ -21,6 +22,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# MI355X vector issue: 256 CUs x 4 SIMDs, a wave64 VALU instruction occupies a 16-lane SIMD for 4
+# cycles, 2.4 GHz peak engine clock -> 614.4 G wave instructions per second
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
 
 
 def log(msg: str) -> None:
@@ -47,6 +51,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched sun x aim measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement")
     ap.add_argument("--no-cylinder", action="store_true", help="skip the config-3 (cylinder) field measurement")
+    ap.add_argument("--no-low-sun", action="store_true", help="skip the PS10 point under a 15-degree sun")
     ap.add_argument("--batched-suns", type=int, default=4000)
     ap.add_argument("--loop-instances", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -86,61 +91,47 @@ def main() -> int:
     alg_bytes = algorithmic_bytes(st)
     log(f"counted work: {st}; algorithmic bytes per launch {alg_bytes / 1e9:.2f} GB")
 
+    def timed(steps: int) -> tuple[float, float, float]:
+        """K steps between events on the trace stream, and the library's events around each launch's
+        trace kernel alone (the offsets, culling kernels and folds of a call are in the step time, not
+        in the kernel time): (ms per step, trace-kernel ms per step, host wall seconds)."""
+        kernel_ms = 0.0
+        dev.sync()
+        dev.kernel_events(2, 3)
+        wall0 = time.perf_counter()
+        dev.event_record(0)
+        for _ in range(steps):
+            dev.flux_clear(0)
+            dev.trace(ids)
+            kernel_ms += dev.event_elapsed_ms(2, 3)
+        dev.event_record(1)
+        dev.sync()
+        dev.kernel_events(-1, -1)
+        return dev.event_elapsed_ms(0, 1) / steps, kernel_ms / steps, time.perf_counter() - wall0
+
+    # the reference's semantics: every trace call draws fresh group offsets for its heliostats
+    dev.offsets_resample(True, 1000)
     for _ in range(args.warmup):
         dev.flux_clear(0)
         dev.trace(ids)
     dev.sync()
     log(f"warmup done ({args.warmup} steps)")
-
-    # timed region: K steps, events around the whole region and, recorded by the library on the
-    # trace stream, around each launch's trace kernel alone (the culling kernels and folds of the
-    # call are in the step time, not in the kernel time)
-    kernel_ms = 0.0
-    dev.sync()
-    dev.kernel_events(2, 3)
-    wall0 = time.perf_counter()
-    dev.event_record(0)
-    for k in range(args.steps):
-        dev.flux_clear(0)
-        dev.trace(ids)
-        kernel_ms += dev.event_elapsed_ms(2, 3)
-    dev.event_record(1)
-    dev.sync()
-    dev.kernel_events(-1, -1)
-    wall = time.perf_counter() - wall0
-    total_ms = dev.event_elapsed_ms(0, 1)
-    ms_per_step = total_ms / args.steps
-    kernel_avg_ms = kernel_ms / args.steps
+    ms_per_step, kernel_avg_ms, wall = timed(args.steps)
     rays_per_s = rays_per_step / (ms_per_step / 1e3)
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall:.3f}s), trace kernel {kernel_avg_ms:.3f} ms, "
         f"{rays_per_s / 1e9:.3f} G rays/s")
     flux = dev.flux(0)
     log(f"flux map sum {float(flux.sum(dtype=np.float64)):.6e}, max {float(flux.max()):.3f}")
+    # the same steps with the offsets kept (no resampling kernel in the step)
+    dev.offsets_resample(False)
+    ms_fixed, kernel_fixed_ms, _ = timed(args.steps)
+    log(f"fixed offsets: {ms_fixed:.3f} ms/step, trace kernel {kernel_fixed_ms:.3f} ms")
 
-    achieved = alg_bytes / (kernel_avg_ms / 1e3) / 1e9
-    # HBM bytes per launch and VALU busy fraction of this kernel, from the committed rocprofv3 PMC
-    # summaries of the same workload (profiles/trace_<workload>_{hbm_traffic,valu}.md)
-    traffic = valu_busy = atom = None
-    tfile = ROOT / "profiles" / f"trace_{args.workload}_hbm_traffic.json"
-    vfile = ROOT / "profiles" / f"trace_{args.workload}_valu.json"
-    afile = ROOT / "profiles" / f"trace_{args.workload}_atomics.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-    if vfile.exists():
-        valu_busy = json.loads(vfile.read_text()).get("valu_busy")
-    if afile.exists():
-        atom = json.loads(afile.read_text())
+    roofline = valu_roofline(args.workload, kernel_avg_ms, alg_bytes)
     # deposit rates of the trace against the device's measured atomic ceilings (capability probe)
     ceil = dev.probe_atomic_rates()
     log(f"atomic ceilings (G ops/s): {ceil}")
-    atomics = {"ceiling_gops": ceil}
-    if atom:
-        lds_rate = atom["lds_atomic_lanes_per_launch"] / (kernel_avg_ms / 1e3) / 1e9
-        glob_rate = atom["global_atomics_per_launch"] / (kernel_avg_ms / 1e3) / 1e9
-        atomics.update({"lds_atomic_lanes_per_launch": atom["lds_atomic_lanes_per_launch"],
-                        "global_atomics_per_launch": atom["global_atomics_per_launch"],
-                        "lds_gops": lds_rate, "lds_frac": lds_rate / ceil["lds_f32"],
-                        "global_gops": glob_rate, "global_frac": glob_rate / ceil["global_f64"]})
+    roofline["atomics"] = atomic_rates(args.workload, kernel_avg_ms, ceil)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -157,6 +148,9 @@ def main() -> int:
     cylinder = None
     if not args.no_cylinder:
         cylinder = bench_cylinder(dev)
+    low_sun = None
+    if args.workload == "ps10" and not args.no_low_sun:
+        low_sun = bench_low_sun(dev)
 
     line = {
         "metric": "rays/s on 1 GPU (PS10-like); batched scenes/s; adjoint:forward cost",
@@ -180,65 +174,175 @@ def main() -> int:
             "pool": P,
             "flux_map": list(flux.shape),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": "k_trace_band (rectangle receiver; persistent, LDS band of map rows)",
-            "kernel_avg_ms": kernel_avg_ms,
-            "algorithmic_bytes_per_launch": alg_bytes,
-            "hbm_gbs_measured": traffic / (kernel_avg_ms / 1e3) / 1e9 if traffic else None,
-            "binding_resource": "vector issue (VALU about 80% busy) at 4 waves per SIMD, the occupancy the LDS band allows",
-            "valu_busy": valu_busy,
-            "atomics": atomics,
-        },
+        "roofline": roofline,
+        "fixed_offsets": {"ms_per_step": ms_fixed, "kernel_avg_ms": kernel_fixed_ms,
+                          "rays_per_s": rays_per_step / (ms_fixed / 1e3),
+                          "note": "the same step without the per-call resampling of the group offsets"},
         "cpu_baseline": cpu,
         "batched": batched,
         "batched_ps10_sampled": batched_ps10,
         "adjoint": adjoint,
         "config3_cylinder_field": cylinder,
+        "ps10_low_sun": low_sun,
     }
     print(json.dumps(line), flush=True)
     dev.close()
     return 0
 
 
+def profile_json(name: str):
+    """A committed rocprofv3 summary (profiles/<name>.json) and whether it was measured on the
+    sources this library is built from."""
+    from solarenergy_chier_amd import build as b
+
+    f = ROOT / "profiles" / f"{name}.json"
+    if not f.exists():
+        return None, None
+    d = json.loads(f.read_text())
+    here = b.source_digest()
+    tag = (f"profiles/{name}.json: rocprofv3 run recorded there (sources {d.get('source_digest')}; "
+           + ("the sources of this build)" if d.get("source_digest") == here else f"this build's sources are {here}: stale)"))
+    return d, tag
+
+
+def valu_roofline(workload: str, kernel_ms: float, alg_bytes: int) -> dict:
+    """Roofline of the trace kernel against vector issue, the resource that binds it: achieved = its
+    VALU wave instructions per launch (rocprofv3 SQ_INSTS_VALU, a count fixed by the code and the
+    workload) / this run's kernel time, peak = 614.4 G/s. HBM and the reference-equivalent
+    algorithmic-byte rate are reported beside it, labelled."""
+    sec = kernel_ms / 1e3
+    out = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINST, "unit": "G VALU wave64 instructions/s",
+           "frac": None, "traffic": None,
+           "kernel": "k_trace_band<0,1024> (persistent, LDS band of map rows)" if workload == "ps10" else "trace kernel",
+           "kernel_avg_ms": kernel_ms, "kernel_time_source": "HIP events on the trace stream around each launch",
+           "algorithmic_bytes_per_launch": alg_bytes,
+           "algorithmic_gbs_reference_equivalent": alg_bytes / sec / 1e9,
+           "algorithmic_gbs_note": "bytes the reference's control flow would move with no on-chip reuse and no "
+                                   "culling, over this kernel's time; not a roofline (culling skips that work)",
+           "provenance": {}}
+    v, vtag = profile_json(f"trace_{workload}_valu")
+    if v and "valu_insts_per_launch" in v:
+        out["valu_insts_per_launch"] = v["valu_insts_per_launch"]
+        out["valu_insts_per_64_rays"] = v.get("valu_insts_per_64_rays")
+        out["salu_insts_per_64_rays"] = v.get("salu_insts_per_64_rays")
+        out["achieved"] = v["valu_insts_per_launch"] / sec / 1e9
+        out["frac"] = out["achieved"] / VALU_PEAK_GINST
+        out["valu_busy_profiled"] = v.get("valu_busy")
+        out["provenance"]["valu_insts_per_launch"] = vtag
+    t, ttag = profile_json(f"trace_{workload}_hbm_traffic")
+    if t and "bytes_per_launch" in t:
+        out["traffic"] = t["bytes_per_launch"]
+        out["hbm_gbs"] = t["bytes_per_launch"] / sec / 1e9
+        out["hbm_frac"] = out["hbm_gbs"] / HBM_PEAK_GBS
+        out["provenance"]["traffic"] = ttag
+        out["provenance"]["hbm_gbs"] = "profiled bytes per launch / this run's kernel time"
+    return out
+
+
+def atomic_rates(workload: str, kernel_ms: float, ceil: dict) -> dict:
+    """The trace kernel's deposit rates (profiled atomic counts / this run's kernel time) against the
+    device's atomic ceilings measured in this run (qmc_probe_atomic_rates)."""
+    out = {"ceiling_gops": ceil, "ceiling_source": "qmc_probe_atomic_rates in this run"}
+    a, tag = profile_json(f"trace_{workload}_atomics")
+    if a and "lds_atomic_lanes_per_launch" in a:
+        sec = kernel_ms / 1e3
+        lds_rate = a["lds_atomic_lanes_per_launch"] / sec / 1e9
+        glob_rate = a["global_atomics_per_launch"] / sec / 1e9
+        out.update({"lds_atomic_lanes_per_launch": a["lds_atomic_lanes_per_launch"],
+                    "global_atomics_per_launch": a["global_atomics_per_launch"],
+                    "lds_gops": lds_rate, "lds_frac": lds_rate / ceil["lds_f32"],
+                    "global_gops": glob_rate, "global_frac": glob_rate / ceil["global_f64"],
+                    "provenance": tag})
+    return out
+
+
+def bench_low_sun(dev, steps: int = 3) -> dict:
+    """The PS10-like field under a 15-degree sun (azimuth 180): neighbours shade and block each other,
+    so shadow traversals run (the 50-degree headline sun culls every one of them)."""
+    from solarenergy_chier_amd import Scene, workloads
+
+    sun = (180.0, 15.0)
+    wl = workloads.ps10_field(sun_deg=sun)
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    dev.upload(scene)
+    dev.pool_generate(2, 4)
+    dev.offsets_generate(3)
+    ids = list(range(scene.counts.n_heliostats))
+    rays = sum(scene.heliostat_info(h).n_micro for h in ids) * scene.config.num_sunshape_lights_per_group
+    st = dev.trace_stats(ids)
+    flags = dev.cull_flags(ids)
+    dev.offsets_resample(True, 2000)
+    dev.flux_clear(0)
+    dev.trace(ids)
+    dev.sync()
+    total = kern = 0.0
+    dev.kernel_events(2, 3)
+    for _ in range(steps):
+        dev.flux_clear(0)
+        dev.event_record(12)
+        dev.trace(ids)
+        dev.event_record(13)
+        dev.sync()
+        total += dev.event_elapsed_ms(12, 13)
+        kern += dev.event_elapsed_ms(2, 3)
+    dev.kernel_events(-1, -1)
+    dev.offsets_resample(False)
+    ms, kms = total / steps, kern / steps
+    out = {"workload": f"{wl.name}, sun azimuth {sun[0]:.0f} deg, elevation {sun[1]:.0f} deg", "rays": rays,
+           "ms_per_step": ms, "kernel_avg_ms": kms, "rays_per_s": rays / (ms / 1e3),
+           "shadowed_fraction": 1.0 - st["unshadowed"] / st["rays"],
+           "blocked_fraction": (st["unshadowed"] - st["unblocked"]) / st["rays"],
+           "heliostats_needing_shadow_traversal": int((flags & 1).sum()),
+           "heliostats_needing_blocking_traversal": int(((flags >> 1) & 1).sum()),
+           "algorithmic_bytes_per_launch": algorithmic_bytes(st), "counted_work": st}
+    log(f"PS10 low sun {sun}: {ms:.3f} ms/step (kernel {kms:.3f} ms) = {rays / ms / 1e6:.2f} G rays/s; "
+        f"shadowed {out['shadowed_fraction']:.3f}, blocked {out['blocked_fraction']:.3f}, shadow traversals on "
+        f"{out['heliostats_needing_shadow_traversal']} heliostats")
+    return out
+
+
 def cpu_baseline(scene, dev, ids, L, P, F, seconds):
-    """The oracle (CPU restatement) on a strided sample of the same rays, all host threads (<= 16)."""
+    """The oracle (CPU restatement) on a strided sample of the same rays: every host thread of the
+    process's CPU share (the GPU box gives one GPU's process 16, whatever os.cpu_count() reports), and
+    one thread on a smaller sample. The stride is set from one calibration run (every 64th ray) so the
+    all-thread sample takes about `seconds`."""
     import numpy as np
 
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O  # the checker / CPU baseline only
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host_cpus = os.cpu_count() or 1
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = host_cpus
+    threads = max(1, min(16, share))
     orc = O.OracleScene.from_product_scene(scene)
     dirs, perts = dev.pool_get(P)
     offs = np.concatenate([dev.offsets_get(h, F[h]) for h in ids])
-    # two calibration samples separate the per-heliostat facet set-up from the per-ray cost, then
-    # the stride is chosen for about `seconds` of CPU work
     total = sum(F) * L
-    runs = []
-    for stride in (4096, 1024):
+
+    def run(stride, nthreads):
         t = time.perf_counter()
-        _, st = orc.trace_many(ids, dirs, perts, L, offs, nthreads=threads, stride=stride)
-        runs.append((st["rays"], time.perf_counter() - t))
-    (r1, t1), (r2, t2) = runs
-    rate = (r2 - r1) / max(t2 - t1, 1e-6)
-    setup = max(t1 - r1 / rate, 0.0)
-    want = max(rate * max(seconds - setup, 1.0), 1.0)
-    stride = max(1, int(total / want))
-    log(f"cpu baseline: calibration {rate / 1e6:.2f} M rays/s on {threads} threads (+{setup:.2f}s set-up); "
-        f"sample stride {stride}")
-    t = time.perf_counter()
-    _, st = orc.trace_many(ids, dirs, perts, L, offs, nthreads=threads, stride=stride)
-    dt = time.perf_counter() - t
-    value = st["rays"] / dt
-    log(f"cpu baseline: {st['rays']} rays in {dt:.2f}s = {value / 1e6:.3f} M rays/s")
+        _, st = orc.trace_many(ids, dirs, perts, L, offs, nthreads=nthreads, stride=stride)
+        dt = time.perf_counter() - t
+        return st["rays"], dt
+
+    r0, t0 = run(64, threads)
+    rate = r0 / max(t0, 1e-6)
+    stride = max(1, int(total / max(rate * seconds, 1.0)))
+    log(f"cpu baseline: calibration {rate / 1e6:.2f} M rays/s on {threads} threads (every 64th ray: {r0} rays "
+        f"in {t0:.2f}s); sample stride {stride}")
+    rays, dt = run(stride, threads)
+    value = rays / dt
+    log(f"cpu baseline: {rays} rays in {dt:.2f}s = {value / 1e6:.3f} M rays/s on {threads} threads")
+    stride1 = max(stride, int(total / max(value / threads * 2.0, 1.0)))
+    rays1, dt1 = run(stride1, 1)
+    log(f"cpu baseline: one thread {rays1} rays in {dt1:.2f}s = {rays1 / dt1 / 1e6:.3f} M rays/s")
     return {"value": value, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"every {stride}-th ray of all {len(ids)} heliostats ({st['rays']} rays, {dt:.1f} s)"}
+            "sample": f"every {stride}-th ray of all {len(ids)} heliostats ({rays} rays, {dt:.1f} s)",
+            "host_cpus": host_cpus, "cpu_share": share,
+            "single_thread": {"value": rays1 / dt1, "sample": f"every {stride1}-th ray ({rays1} rays, {dt1:.1f} s)"}}
 
 
 def bench_batched(dev, field: str, n_suns: int, n_loop: int) -> dict:
@@ -401,7 +505,9 @@ def bench_adjoint(dev, steps: int = 64) -> dict:
         t_bwd += dev.event_elapsed_ms(9, 10)
     t_plain, t_fwd, t_bwd = t_plain / reps, t_fwd / reps, t_bwd / reps
     log(f"adjoint: {rays} rays; plain trace {t_plain:.3f} ms, differentiable forward {t_fwd:.3f} ms, "
-        f"backward {t_bwd:.3f} ms -> adjoint:forward {t_bwd / t_plain:.2f}")
+        f"backward {t_bwd:.3f} ms -> adjoint:forward {t_bwd / t_plain:.2f}, gradient (forward + backward) "
+        f"{(t_fwd + t_bwd) / t_plain:.2f}")
+    fd = crn_finite_differences(dev, scene, offs, buf, npix, T, PK, LAM)
     # aim-point optimisation: normalised gradient steps, offsets kept within +-2 m
     o = offs.copy()
     losses = []
@@ -420,8 +526,55 @@ def bench_adjoint(dev, steps: int = 64) -> dict:
     return {"metric": "adjoint:forward cost", "workload": wl.name, "parameters": 2 * nh + 3, "rays": rays,
             "plain_forward_ms": t_plain, "differentiable_forward_ms": t_fwd, "backward_ms": t_bwd,
             "adjoint_to_forward": t_bwd / t_plain, "adjoint_plus_forward_to_forward": (t_bwd + t_fwd) / t_plain,
+            "cost_note": "adjoint_to_forward = backward / plain forward trace of the same rays; one gradient "
+                         "evaluation costs the differentiable forward plus the backward",
+            "crn_finite_differences": fd,
             "optimisation_steps": steps, "loss_first": losses[0], "loss_last": losses[-1],
             "optimisation_s": opt_s}
+
+
+def crn_finite_differences(dev, scene, offs, buf, npix, T, PK, LAM, n_check: int = 8) -> dict:
+    """Central differences with common random numbers (same pools, offsets, frozen visibility; the
+    masked forward accumulates in double) of the loss w.r.t. the n_check largest aim-offset gradient
+    components, against the adjoint, at the configuration's 1e-3 m step and at 1e-6 m. The bilinear
+    splat is piecewise linear in each hit position, kinked where a hit crosses a pixel centre, so a
+    step that is not small against the pixel averages the derivative over those kinks."""
+    import numpy as np
+
+    scene.set_aim_offsets(offs)
+    scene.process()
+    dev.update(scene)
+    dev.adjoint_forward(0, buf)
+    dev.flux_loss(buf, npix, T, PK, LAM)
+    ga, _ = dev.adjoint_backward(scene, 0)
+
+    def dloss(fp, fm):
+        return float(((fp - fm) * (fp + fm - 2 * T)).sum()
+                     + LAM * (np.maximum(fp - PK, 0) ** 2 - np.maximum(fm - PK, 0) ** 2).sum())
+
+    def flux_at(o):
+        scene.set_aim_offsets(o)
+        scene.process()
+        dev.update(scene)
+        return dev.adjoint_forward_masked(0)
+
+    order = np.argsort(-np.abs(ga).ravel())[:n_check]
+    out = {"components": int(n_check), "bar": 1e-3}
+    for h in (1e-3, 1e-4, 1e-6):
+        fd_v, ad_v = [], []
+        for flat in order:
+            hh, k = divmod(int(flat), 2)
+            op, om = offs.copy(), offs.copy()
+            op[hh, k] += h
+            om[hh, k] -= h
+            step = float(op[hh, k]) - float(om[hh, k])
+            fd_v.append(dloss(flux_at(op), flux_at(om)) / step)
+            ad_v.append(ga[hh, k])
+        err = float(np.linalg.norm(np.array(fd_v) - np.array(ad_v)) / np.linalg.norm(ad_v))
+        out[f"rel_err_step_{h:g}m"] = err
+        log(f"adjoint: CRN central differences, step {h:g} m: relative error {err:.2e} over {n_check} components")
+    flux_at(offs)
+    return out
 
 
 def copy_probe(mib: int) -> int:

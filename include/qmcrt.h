@@ -404,6 +404,10 @@ qmc_status qmc_batch_set_aims(qmc_ctx *ctx, const qmc_scene *scene, const float 
 qmc_status qmc_trace_batched(qmc_ctx *ctx, int32_t grid, const qmc_instance *instances, int32_t n,
                              const qmc_schedule *schedule, float *d_maps);
 qmc_status qmc_batch_read(qmc_ctx *ctx, int32_t first, int32_t count, float *out);
+/* Bytes of per-instance device data (frames, factors, quads, candidate lists) a batched trace keeps at
+ * once: a sweep needing more is traced in chunks of instances, one launch each. 0 restores the default
+ * (2 GiB). No reference counterpart. */
+qmc_status qmc_batch_set_work_budget(qmc_ctx *ctx, uint64_t bytes);
 /* Time the candidate schedules on (a prefix of) the instances and return the fastest; results are
  * cached per (device, field shape) for the process and, when QMC_TUNE_CACHE names a file, across
  * processes. */

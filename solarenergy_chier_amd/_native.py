@@ -187,6 +187,7 @@ SIGNATURES = {
     "qmc_batch_set_aims": [_P, _P, _FP, _I],
     "qmc_trace_batched": [_P, _I, C.POINTER(Instance), _I, C.POINTER(Schedule), _FP],
     "qmc_batch_read": [_P, _I, _I, _FP],
+    "qmc_batch_set_work_budget": [_P, C.c_uint64],
     "qmc_batch_autotune": [_P, _I, C.POINTER(Instance), _I, C.POINTER(Schedule)],
     "qmc_adjoint_forward": [_P, _I, _FP],
     "qmc_adjoint_forward_masked": [_P, _I, C.POINTER(C.c_double), C.c_int64],

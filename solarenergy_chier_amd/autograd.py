@@ -37,12 +37,11 @@ torch.cuda.init()
 class _FluxFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, offsets: torch.Tensor, sun_dir: torch.Tensor, fm: "FluxMap") -> torch.Tensor:
-        fm._set_parameters(offsets.detach().cpu().numpy(), sun_dir.detach().cpu().numpy())
-        flux = torch.empty(fm.shape, dtype=torch.float32, device=fm.torch_device)
-        torch.cuda.synchronize(fm.torch_device)
-        fm.device.adjoint_forward(fm.grid, flux.data_ptr())
-        fm.device.sync()
+        params = (np.array(offsets.detach().cpu().numpy(), np.float32), np.array(sun_dir.detach().cpu().numpy(), np.float32))
+        flux = fm._forward(*params)
         ctx.fm = fm
+        ctx.params = params
+        ctx.stamp = fm._stamp
         ctx.shapes = (offsets.shape, offsets.dtype, offsets.device, sun_dir.dtype, sun_dir.device)
         return flux
 
@@ -50,6 +49,11 @@ class _FluxFn(torch.autograd.Function):
     def backward(ctx, grad_flux: torch.Tensor):
         fm: FluxMap = ctx.fm
         oshape, odtype, odev, sdtype, sdev = ctx.shapes
+        if fm._stamp != ctx.stamp:
+            # another forward ran since this one (e.g. loss = f(x1) + f(x2)): the device holds its
+            # parameters and visibility mask. Restore this forward's: the same parameters give the same
+            # rays, so the recomputed mask is the one this forward recorded.
+            fm._forward(*ctx.params)
         g = grad_flux.detach().to(device=fm.torch_device, dtype=torch.float32).contiguous()
         torch.cuda.synchronize(fm.torch_device)
         grad_aim, grad_sun = fm.device.adjoint_backward(fm.scene, fm.grid, g.data_ptr())
@@ -75,6 +79,17 @@ class FluxMap:
         self._all = np.zeros((scene.counts.n_heliostats, 2), np.float32)
         for h in range(scene.counts.n_heliostats):
             self._all[h] = scene.heliostat_desc(h).aim_offset[:]
+        self._stamp = 0  # counts forwards: which one the device state (parameters, mask) belongs to
+
+    def _forward(self, offsets: np.ndarray, sun_dir: np.ndarray) -> torch.Tensor:
+        """Set the parameters, run the differentiable forward (recording the visibility mask)."""
+        self._set_parameters(offsets, sun_dir)
+        flux = torch.empty(self.shape, dtype=torch.float32, device=self.torch_device)
+        torch.cuda.synchronize(self.torch_device)
+        self.device.adjoint_forward(self.grid, flux.data_ptr())
+        self.device.sync()
+        self._stamp += 1
+        return flux
 
     def _set_parameters(self, offsets: np.ndarray, sun_dir: np.ndarray) -> None:
         self._all[self.helio_slice] = np.asarray(offsets, np.float32).reshape(-1, 2)

@@ -36,6 +36,22 @@ EXTRA_FLAGS = {"qmc_batch.hip": ["-mllvm", "-disable-machine-licm"],
 ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")
 
 
+def source_digest() -> str:
+    """SHA-256 (first 16 hex digits) of everything the library is built from: the csrc/ sources, the
+    public header, the per-file compile flags and the target. Profiles record it, so that numbers
+    read from them can be told apart from those of other sources."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for p in sorted(CSRC.iterdir()) + [ROOT / "include" / "qmcrt.h"]:
+        if p.is_file():
+            h.update(p.name.encode())
+            h.update(p.read_bytes())
+    h.update(repr(sorted(EXTRA_FLAGS.items())).encode())
+    h.update(ARCH.encode())
+    return h.hexdigest()[:16]
+
+
 def hipcc() -> str:
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
         if cand and Path(cand).exists():

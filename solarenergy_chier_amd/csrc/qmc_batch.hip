@@ -395,7 +395,8 @@ qmc_status launch_batched(qmc_ctx *c, const BatchPlan &plan, const qmc_schedule 
     const bool lds_map = s.lds_flux_tile != 0;
     const size_t lds = lds_bytes(a, lds_map);
     if (lds > c->max_lds) return fail(QMC_E_UNSUPPORTED, "batched trace: the flux map does not fit in LDS");
-    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n_inst, kWorkBudget / work_bytes_per_instance(a)));
+    const size_t budget = c->batch_work_budget ? c->batch_work_budget : kWorkBudget;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n_inst, budget / work_bytes_per_instance(a)));
     qmc_status st = ensure_work(c, a, chunk);
     if (st) return st;
     if (!lds_map) HIP_TRY(hipMemsetAsync(a.maps, 0, sizeof(float) * (size_t)n_inst * a.npix, c->stream));
@@ -649,5 +650,11 @@ extern "C" qmc_status qmc_batch_read(qmc_ctx *c, int32_t first, int32_t count, f
     HIP_TRY(hipMemcpyAsync(out, c->d_batch_maps + (size_t)first * c->batch_npix,
                            sizeof(float) * (size_t)count * c->batch_npix, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_batch_set_work_budget(qmc_ctx *c, uint64_t bytes) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_batch_set_work_budget: null ctx");
+    c->batch_work_budget = (size_t)bytes;
     return QMC_OK;
 }

@@ -80,6 +80,7 @@ struct qmc_ctx {
     size_t batch_inst_cap = 0;
     char *d_batch_work = nullptr;  // per-instance frames, factors and quads of a chunk
     size_t batch_work_cap = 0;     // bytes
+    size_t batch_work_budget = 0;  // bytes of per-instance data per chunk (0: the 2 GiB default)
     bool batch_maps_caller = false; // the last batched trace wrote into the caller's buffer
     int32_t batch_n = 0;           // instances of the last batched trace
     int32_t batch_npix = 0;

@@ -471,6 +471,11 @@ class Device:
         N.call("qmc_batch_read", self.handle, first, count, N.fptr(out))
         return out
 
+    def batch_set_work_budget(self, nbytes: int) -> None:
+        """Per-instance device data a batched trace keeps at once (0: the 2 GiB default); a larger
+        sweep is traced in chunks of instances."""
+        N.call("qmc_batch_set_work_budget", self.handle, C.c_uint64(nbytes))
+
     def batch_autotune(self, grid: int, inst) -> dict:
         s = N.Schedule()
         N.call("qmc_batch_autotune", self.handle, grid, inst, len(inst), C.byref(s))

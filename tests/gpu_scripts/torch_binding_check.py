@@ -44,8 +44,29 @@ def main() -> int:
     ok = flux.shape == ref_flux.shape and e_flux < 1e-6 and e_aim < 1e-5 and e_sun < 1e-5
     print(f"torch binding vs adjoint: flux {e_flux:.2e}, aim {e_aim:.2e}, sun {e_sun:.2e} -> {'ok' if ok else 'MISMATCH'}",
           flush=True)
+
+    # two forwards before one backward: each term's gradients are those of its own parameters
+    offs1 = (offs0 + 0.5).clip(-2.0, 2.0).astype(np.float32)
+    scene.set_aim_offsets(offs1)
+    scene.process()
+    dev.update(scene)
+    dev.adjoint_forward(0, buf)
+    dev.flux_loss(buf, ref_flux.size, T, PK, LAM)
+    ga1_ref, gs1_ref = dev.adjoint_backward(scene, 0)
+    x0 = torch.tensor(offs0, dtype=torch.float32, requires_grad=True)
+    x1 = torch.tensor(offs1, dtype=torch.float32, requires_grad=True)
+    s0 = torch.tensor(list(sun.detach().numpy()), dtype=torch.float32, requires_grad=True)
+    f0, f1 = fm(x0, s0), fm(x1, s0)
+    total = sum(((f - T) ** 2).sum() + LAM * (torch.clamp(f - PK, min=0.0) ** 2).sum() for f in (f0, f1))
+    total.backward()
+    e0 = float(np.linalg.norm(x0.grad.numpy() - ga_ref) / np.linalg.norm(ga_ref))
+    e1 = float(np.linalg.norm(x1.grad.numpy() - ga1_ref) / np.linalg.norm(ga1_ref))
+    es = float(np.linalg.norm(s0.grad.numpy() - (gs_ref + gs1_ref)) / np.linalg.norm(gs_ref + gs1_ref))
+    ok2 = e0 < 1e-5 and e1 < 1e-5 and es < 1e-5
+    print(f"two forwards, one backward: aim {e0:.2e} / {e1:.2e}, sun {es:.2e} -> {'ok' if ok2 else 'MISMATCH'}",
+          flush=True)
     dev.close()
-    return 0 if ok else 1
+    return 0 if ok and ok2 else 1
 
 
 if __name__ == "__main__":

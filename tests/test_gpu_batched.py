@@ -162,3 +162,40 @@ def test_ps10_sampled_batched_equals_looped_plain_calls(device, schedule):
         print(f"ps10 instance {i} (aim set {aim[i]}): rel L1 {err:.2e}, sum {ref.sum():.4e}", flush=True)
         assert ref.sum() > 0
         assert err < TOL
+
+
+def test_full_sweep_start_middle_end_and_chunks(device):
+    """The whole sweep of the batched configuration (config-1 field, 4000 suns x 4 aim strategies =
+    16,000 instances of 2.56M rays): instances from its start, middle and end equal the plain call
+    re-processed for their sun, DNI and aim set; then the same sweep with the per-chunk work budget
+    lowered to 8 MiB (3,740 B per instance: 8 chunks of 2,243 instances, one launch each) gives the
+    same maps."""
+    wl = workloads.small_field()
+    scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
+    device.upload(scene)
+    device.pool_generate(2, 4)
+    device.offsets_generate(3)
+    nh = scene.counts.n_heliostats
+    aims = workloads.aim_strategies(nh)
+    device.batch_set_aims(scene, aims)
+    sun, dni, aim = workloads.batched_instances(n_suns=4000)
+    inst = device.instances(sun, dni, aim)
+    n = len(inst)
+    assert n == 16000
+    shape = device.flux_shape(0)
+    sched = device.batch_autotune(0, inst)
+    device.trace_batched(0, inst, sched)
+    picks = [0, 1, n // 2 - 1, n // 2, n - 2, n - 1]
+    maps = {i: device.batch_read(i, 1, shape)[0].copy() for i in picks}
+    device.batch_set_work_budget(8 << 20)
+    try:
+        device.trace_batched(0, inst, sched)
+        chunked = {i: device.batch_read(i, 1, shape)[0].copy() for i in picks}
+    finally:
+        device.batch_set_work_budget(0)
+    for i in picks:
+        ref = plain_map(device, scene, wl.config, sun[i], dni[i], aims[aim[i]])
+        e1, e2 = rel_l1(maps[i], ref), rel_l1(chunked[i], ref)
+        print(f"instance {i}: batched rel L1 {e1:.2e}, chunked {e2:.2e}, sum {ref.sum():.4e}", flush=True)
+        assert ref.sum() > 0
+        assert e1 < TOL and e2 < TOL
