@@ -313,6 +313,14 @@ qmc_status qmc_facets_get(qmc_ctx *ctx, int32_t heliostat, float *origins, float
  * RectangleReceiverRectangleGridRayTracing.cu:45-57, CylinderReceiverRectangleGridRayTracing.cu:44-57.
  * Heliostats of several grids are traced grid by grid. Asynchronous. */
 qmc_status qmc_trace(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n);
+/* The same trace as qmc_trace, allocating no device memory. No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The list, per-heliostat candidate lists, workgroup bands and per-wave masks live in a
+ * file-scope device array of 128 MiB per device (held until the process ends); a call whose wave masks
+ * do not fit in it is split into launches over fewer heliostats. Limits: n <= 65536 entries, and the
+ * scene's candidate lists (576 B per heliostat) plus one band per CU must fit (QMC_E_RANGE otherwise).
+ * Calls that overlap on one device (contexts on different streams) share the array and race. */
+qmc_status qmc_trace_noalloc(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n);
 /* Which trace kernel a record trace runs (qmc_trace_records_many): AUTO = the one qmc_trace would run
  * for the same list, PER_RAY = one lane per ray (k_trace), BANDED = the persistent LDS-banded kernel
  * (k_trace_band), whatever the list's size. */

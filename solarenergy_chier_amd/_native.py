@@ -171,6 +171,7 @@ SIGNATURES = {
     "qmc_offsets_resample": [_P, _I, C.c_uint64],
     "qmc_facets_get": [_P, _I, _FP, _FP, _I],
     "qmc_trace": [_P, _IP, _I],
+    "qmc_trace_noalloc": [_P, _IP, _I],
     "qmc_cull_flags": [_P, _IP, _I, C.POINTER(C.c_uint8)],
     "qmc_probe_atomic_rates": [_P, C.POINTER(AtomicRates)],
     "qmc_trace_records": [_P, _I, C.POINTER(RayRecord), C.c_int64],
