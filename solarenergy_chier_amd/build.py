@@ -37,16 +37,16 @@ ARCH = os.environ.get("QMC_OFFLOAD_ARCH", "gfx950")
 
 
 def source_digest() -> str:
-    """SHA-256 (first 16 hex digits) of everything the library is built from: the csrc/ sources, the
-    public header, the per-file compile flags and the target. Profiles record it, so that numbers
-    read from them can be told apart from those of other sources."""
+    """SHA-256 (first 16 hex digits) of everything libqmcrt.so is built from: its csrc/ sources and
+    headers, the public header, the per-file compile flags and the target. Profiles record it, so that
+    numbers read from them can be told apart from those of other sources."""
     import hashlib
 
     h = hashlib.sha256()
-    for p in sorted(CSRC.iterdir()) + [ROOT / "include" / "qmcrt.h"]:
-        if p.is_file():
-            h.update(p.name.encode())
-            h.update(p.read_bytes())
+    files = [CSRC / f for f in HIP_SOURCES + HOST_SOURCES] + sorted(CSRC.glob("*.h")) + [ROOT / "include" / "qmcrt.h"]
+    for p in files:
+        h.update(p.name.encode())
+        h.update(p.read_bytes())
     h.update(repr(sorted(EXTRA_FLAGS.items())).encode())
     h.update(ARCH.encode())
     return h.hexdigest()[:16]
@@ -99,15 +99,27 @@ def build_qmcrt(verbose: bool = False) -> Path:
     return LIB
 
 
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+FACADE_CHECK = BINDIR / "qmc_facade_check"
+
+
+def facade_program_cmd(src: Path, out: Path) -> list[str]:
+    """g++ command line of a host program on the C++ facade (include/qmcrt_facade.hpp): the C ABI of
+    libqmcrt.so (found through an $ORIGIN-relative run path) and the HIP runtime's vector types."""
+    return ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I", str(ROOT / "include"),
+            "-I", str(ROCM / "include"), str(src), "-L", str(LIBDIR), "-lqmcrt", "-L", str(ROCM / "lib"),
+            "-lamdhip64", "-Wl,-rpath,$ORIGIN/../lib", "-Wl,-rpath," + str(ROCM / "lib"), "-o", str(out)]
+
+
 def build_pipeline(verbose: bool = False) -> Path:
-    """The `qmc_pipeline` command-line program (the reference's RayTracingPipeline executable) on the
-    C ABI: host C++ linked against libqmcrt.so, found through an $ORIGIN-relative run path."""
+    """The `qmc_pipeline` command-line program (the reference's RayTracingPipeline executable) and the
+    facade's call-pattern check (tests/facade_check.cpp), both host C++ on the facade."""
     BINDIR.mkdir(parents=True, exist_ok=True)
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", str(ROOT / "include"), str(CSRC / "pipeline_main.cpp"),
-           "-L", str(LIBDIR), "-lqmcrt", "-Wl,-rpath,$ORIGIN/../lib", "-o", str(PIPELINE)]
-    if verbose:
-        print(" ".join(cmd))
-    _run(cmd)
+    for src, out in ((CSRC / "pipeline_main.cpp", PIPELINE), (ROOT / "tests" / "facade_check.cpp", FACADE_CHECK)):
+        cmd = facade_program_cmd(src, out)
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
     return PIPELINE
 
 

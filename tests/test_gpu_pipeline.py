@@ -1,7 +1,8 @@
 """End to end: the qmc_pipeline program on the reference's example inputs (InputFiles/example: 3
 heliostats, 737M rays, 320x200 map) writes, per receiver, the before- and after-smoothing text maps
 of RayTracingPipeline.cpp:145-175; they equal the library's own trace and smoothing of the same
-scene with the same seeds, up to the 2-decimal text rounding and float atomic order."""
+scene with the same seeds (pools 2, 4; offsets 3, drawn afresh from seed 1000 for the trace), up to
+the 2-decimal text rounding and float atomic order."""
 import subprocess
 from pathlib import Path
 
@@ -31,8 +32,14 @@ def library_maps(device):
     device.upload(s)
     device.pool_generate(2, 4)
     device.offsets_generate(3)
+    # the pipeline's SceneProcessor turns on per-call resampling from seed 1000 (one pass: every
+    # heliostat of the task list once)
+    device.offsets_resample(True, 1000)
     device.flux_clear(0)
-    device.trace(range(s.counts.n_heliostats))
+    try:
+        device.trace(range(s.counts.n_heliostats))
+    finally:
+        device.offsets_resample(False)
     before = device.flux(0).astype(np.float64)
     device.smooth(0, 5, 0.05)
     after = device.flux(0).astype(np.float64)
