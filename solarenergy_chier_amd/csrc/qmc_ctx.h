@@ -18,6 +18,7 @@ struct GridBuf {
     int32_t *cell_start = nullptr;
     int32_t *match = nullptr;
     float4 *quads = nullptr;
+    int4 *boxes = nullptr;        // cells listing each sub-heliostat: lo, hi
     QuadGroup *groups = nullptr;  // culling scan groups of the quads
     int n_sub = 0;
     int n_groups = 0;
@@ -195,6 +196,7 @@ inline void free_scene(qmc_ctx *c) {
         dev_free(g.cell_start);
         dev_free(g.match);
         dev_free(g.quads);
+        dev_free(g.boxes);
         dev_free(g.groups);
     }
     c->grids.clear();

@@ -74,6 +74,7 @@ struct DevGrid {
     const int32_t *cell_start;
     const int32_t *match;
     const float4 *quads;
+    const int4 *boxes;         // per sub-heliostat id: the cells listing it, (x0, y0, z0) then (x1, y1, z1)
 };
 
 // Buie sun-shape table on the device (Scene/SceneProcessor.cu:23-44,237-239)

@@ -36,17 +36,6 @@ using namespace qmc;
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-// Ray slot -> ray id of heliostat H: flat heliostats take their rays in facet tiles (a wave's origins
-// form a compact patch); -1 for slots past the heliostat's rays.
-__device__ __forceinline__ long long slot_ray(const DevHelio &H, int L, long long slot) {
-    if (H.type == 0) {
-        const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
-        const FacetTiling tl = facet_tiling(R, C, L);
-        return slot < tl.slots ? facet_tiled_ray(tl, tile_divisor(H, tl), R, C, (int)slot) : -1;
-    }
-    return slot < (long long)H.F * L ? slot : -1;
-}
-
 // Outcome of one ray: record fields, the deposit's pixel, and what the work counters need.
 struct RayOut {
     int status, index, flags, row, col;
@@ -88,7 +77,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     // ray's candidate tests found (never while counting work)
     const bool culling = !STATS && wl;
     RayHits rh;
-    int plan = culling ? traversal_plan(wl, wm, 0, origin, dir, a.grid.quads, rh) : 2;
+    int plan = culling ? traversal_plan(wl, wm, 0, origin, dir, a.grid, rh) : 2;
     if (plan != 0 &&
         (plan == 2 ? dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)
                    : dda_known<STATS>(rh, origin, dir, a.grid, own_lo, own_hi, hit, st))) {
@@ -100,7 +89,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     // Step 2: perturbed facet normal, reflection, blocking (:24-36)
     V3 normal = normalize(fr_apply(pf, V3{q.x, q.y, q.z}));
     dir = normalize(reflect(-dir, normal));
-    plan = culling ? traversal_plan(wl, wm, 1, origin, dir, a.grid.quads, rh) : 2;
+    plan = culling ? traversal_plan(wl, wm, 1, origin, dir, a.grid, rh) : 2;
     if (plan != 0 &&
         (plan == 2 ? dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)
                    : dda_known<STATS>(rh, origin, dir, a.grid, own_lo, own_hi, hit, st))) {
@@ -691,6 +680,16 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         if (!cells.empty())
             HIP_TRY(hipMemcpyAsync(gb.cells, cells.data(), cells.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    c->stream));
+        std::vector<int4> boxes(2 * (size_t)hg.n_sub);
+        for (int q = 0; q < hg.n_sub; ++q) {
+            const int *b = &hg.sub_box[6 * (size_t)q];
+            boxes[2 * (size_t)q] = make_int4(b[0], b[2], b[4], 0);
+            boxes[2 * (size_t)q + 1] = make_int4(b[1], b[3], b[5], 0);
+        }
+        if ((st = dev_alloc(gb.boxes, boxes.size()))) return st;
+        if (!boxes.empty())
+            HIP_TRY(hipMemcpyAsync(gb.boxes, boxes.data(), boxes.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                   c->stream));
         if ((st = dev_alloc(gb.cell_start, hg.cell_start.size()))) return st;
         if ((st = dev_alloc(gb.match, hg.match.size()))) return st;
         if ((st = dev_alloc(gb.quads, quads.size()))) return st;
@@ -715,6 +714,7 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         g.cell_start = gb.cell_start;
         g.match = gb.match;
         g.quads = gb.quads;
+        g.boxes = gb.boxes;
         gb.n_sub = hg.n_sub;
         gb.receiver = hg.d.receiver;
     }
@@ -1314,10 +1314,7 @@ qmc_status ctx_trace_bufs(qmc_ctx *c, int n, bool rec, size_t wmask_need, TraceB
 }
 
 // Ray slots of a heliostat in a plain trace: rays, or tiled ray slots of flat heliostats (k_trace).
-long long heliostat_slots(const qmc_ctx *c, const DevHelio &H) {
-    return H.type == 0 ? (long long)facet_tiling(H.rows * H.sub_rows, H.cols * H.sub_cols, c->L).slots
-                       : (long long)H.F * c->L;
-}
+long long heliostat_slots(const qmc_ctx *c, const DevHelio &H) { return heliostat_slot_count(H, c->L); }
 
 // A trace call over n list entries. Each occurrence of a heliostat in the list is one reference
 // rayTracing call (QuasiMonteCarloRayTracer.cpp:15-91); occurrences are traced in passes — pass p

@@ -1,7 +1,8 @@
 // qmc_trace_core.h — per-ray device functions shared by the plain and the batched trace kernels, so
 // both follow exactly the same arithmetic (and so stay bit-identical to the host oracle):
 //   fr_apply         local2world_rotate with a precomputed frame (Util/global_function.cuh:19-36)
-//   dda_collision    rectGridDDA::collision + intersect (QuasiMonteCarloRayTracing/Grid/rectGridDDA.cu:7-119)
+//   dda_walk         rectGridDDA::collision's cell walk (QuasiMonteCarloRayTracing/Grid/rectGridDDA.cu:31-119)
+//   dda_collision    the walk with rectGridDDA::intersect on every candidate (rectGridDDA.cu:7-25)
 //   facet_geometry   micro-facet origin / normal in registers (RectangleHelio.cu:54-80,
 //                    FocusFlatRectangleHelio.cu:12-64)
 //   receiver_hit     RectangleReceiver / CylinderReceiver GIntersect
@@ -24,16 +25,16 @@ struct RayStats {
     unsigned cells, candidates, tests;
 };
 
-// Walk of the 3D-DDA over the uniform grid: cells in the reference's order, each cell's candidate
-// ids in list order, own sub-heliostats skipped; test(id) says whether the ray hits quad id. True at
-// the first hit, with its id in hit_id.
-template <bool STATS, class Test>
-__device__ __forceinline__ bool dda_walk(V3 o, V3 d, const DevGrid &g, int own_lo, int own_hi, int &hit_id,
-                                         RayStats &st, Test test) {
+// Walk of the 3D-DDA over the uniform grid: cells in the reference's order (rectGridDDA.cu:31-119).
+// cell(addr, ix, iy, iz, sx, sy, sz, hit_id, st) is called for every visited cell (address in x-major
+// order, indices, step signs): it returns 1 at a hit (hit_id set), -1 when no later cell can hold one
+// (the walk ends without a hit) and 0 to go on.
+template <class Cell>
+__device__ __forceinline__ bool dda_walk(V3 o, V3 d, const DevGrid &g, int &hit_id, RayStats &st, Cell cell) {
     const V3 gpos{g.pos[0], g.pos[1], g.pos[2]};
     const V3 gint{g.interval[0], g.interval[1], g.interval[2]};
-    const V3 cell = (o - gpos) / gint;
-    int ix = (int)cell.x, iy = (int)cell.y, iz = (int)cell.z;
+    const V3 cellf = (o - gpos) / gint;
+    int ix = (int)cellf.x, iy = (int)cellf.y, iz = (int)cellf.z;
     const int sx = (d.x >= 0) ? 1 : -1;
     const int sy = (d.y >= 0) ? 1 : -1;
     const int sz = (d.z >= 0) ? 1 : -1;
@@ -49,22 +50,8 @@ __device__ __forceinline__ bool dda_walk(V3 o, V3 d, const DevGrid &g, int own_l
     const int stride_x = sx * ny * nz, stride_y = sy * nz;
     int addr = ix * ny * nz + iy * nz + iz;
     while (((unsigned)ix < (unsigned)nx) & ((unsigned)iy < (unsigned)ny) & ((unsigned)iz < (unsigned)nz)) {
-        const int4 rec = g.cells[addr];  // count, then the cell's first three candidates
-        const int n = rec.x;
-        if (STATS) {
-            st.cells += 1;
-            st.candidates += (unsigned)n;
-        }
-        for (int i = 0; i < n; ++i) {
-            const int id = i == 0 ? rec.y : i == 1 ? rec.z : i == 2 ? rec.w : g.match[g.cell_start[addr] + i];
-            if (id < own_lo || id >= own_hi) {
-                if (STATS) st.tests += 1;
-                if (test(id)) {
-                    hit_id = id;
-                    return true;
-                }
-            }
-        }
+        const int r = cell(addr, ix, iy, iz, sx, sy, sz, hit_id, st);
+        if (r) return r > 0;
         // Step rule of rectGridDDA (tie tolerance kEps, compared in double): if tx is clearly below
         // ty, step x and/or z by comparing tx with tz; else if ty is clearly below tx, step y and/or
         // z by comparing ty with tz; else step x and y together and/or z by comparing tx with tz.
@@ -91,17 +78,43 @@ __device__ __forceinline__ bool dda_walk(V3 o, V3 d, const DevGrid &g, int own_l
     return false;
 }
 
+// The candidates of one cell in list order (the first three from the packed record, the rest from
+// the cell list), own sub-heliostats skipped; test(id) says whether the ray hits quad id.
+template <bool STATS, class Test>
+__device__ __forceinline__ int cell_candidates(const DevGrid &g, int addr, int own_lo, int own_hi, int &hit_id,
+                                               RayStats &st, Test test) {
+    const int4 rec = g.cells[addr];  // count, then the cell's first three candidates
+    const int n = rec.x;
+    if (STATS) {
+        st.cells += 1;
+        st.candidates += (unsigned)n;
+    }
+    for (int i = 0; i < n; ++i) {
+        const int id = i == 0 ? rec.y : i == 1 ? rec.z : i == 2 ? rec.w : g.match[g.cell_start[addr] + i];
+        if (id < own_lo || id >= own_hi) {
+            if (STATS) st.tests += 1;
+            if (test(id)) {
+                hit_id = id;
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
 // 3D-DDA over the uniform grid; true when a non-own sub-heliostat quad is hit (first hit in
 // traversal and cell-list order; its id goes to hit_id).
 template <bool STATS>
 __device__ __forceinline__ bool dda_collision(V3 o, V3 d, const DevGrid &g, const float4 *quads, int own_lo,
                                               int own_hi, int &hit_id, RayStats &st) {
-    return dda_walk<STATS>(o, d, g, own_lo, own_hi, hit_id, st, [&](int id) {
-        const float4 qa = quads[3 * id];
-        const float4 qb = quads[3 * id + 1];
-        const float4 qc = quads[3 * id + 2];
-        float t, u, v;
-        return ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v);
+    return dda_walk(o, d, g, hit_id, st, [&](int addr, int, int, int, int, int, int, int &hid, RayStats &s) {
+        return cell_candidates<STATS>(g, addr, own_lo, own_hi, hid, s, [&](int id) {
+            const float4 qa = quads[3 * id];
+            const float4 qb = quads[3 * id + 1];
+            const float4 qc = quads[3 * id + 2];
+            float t, u, v;
+            return ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v);
+        });
     });
 }
 
@@ -164,6 +177,23 @@ __host__ __device__ inline int facet_tiled_ray(const FacetTiling &tl, FastDiv fd
 // The multiply-high divisor of a heliostat for tiles of 2^ltw facets across.
 __device__ __forceinline__ FastDiv tile_divisor(const DevHelio &H, const FacetTiling &tl) {
     return tl.ltw == 3 ? H.fd_ntx8 : H.fd_ntx4;
+}
+
+// Ray slot -> ray id of heliostat H: flat heliostats take their rays in facet tiles (a wave's origins
+// form a compact patch); -1 for slots past the heliostat's rays.
+__device__ __forceinline__ long long slot_ray(const DevHelio &H, int L, long long slot) {
+    if (H.type == 0) {
+        const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
+        const FacetTiling tl = facet_tiling(R, C, L);
+        return slot < tl.slots ? facet_tiled_ray(tl, tile_divisor(H, tl), R, C, (int)slot) : -1;
+    }
+    return slot < (long long)H.F * L ? slot : -1;
+}
+
+// Ray slots of a heliostat: its rays, or the tiled ray slots of a flat heliostat (slot_ray).
+__host__ __device__ inline long long heliostat_slot_count(const DevHelio &H, int L) {
+    return H.type == 0 ? (long long)facet_tiling(H.rows * H.sub_rows, H.cols * H.sub_cols, L).slots
+                       : (long long)H.F * L;
 }
 
 // Culling shape of a flat heliostat with normal frame f: its micro-facet origins lie in the rectangle
@@ -273,62 +303,81 @@ __device__ __forceinline__ bool traversal_needed(const CullList *cl, int kind, V
     return false;
 }
 
-// A ray's tests of its wave's masked candidates (traversal_plan): their ids, wave-uniform, and which
-// of them the ray hits.
+// A ray's tests of its wave's masked candidates (traversal_plan): the (one or two) quads it hits and
+// the union of the cell ranges whose lists hold them (x0..x1, y0..y1, z0..z1).
 struct RayHits {
     int n;
-    int ids[kRayTests];
-    uint32_t bits;  // bit j: ray_quad hits ids[j]
+    int id0, id1;
+    int x0, y0, z0, x1, y1, z1;
 };
 
 // How the plain trace runs a ray's traversal of kind `kind`, given its heliostat's wide list and the
 // mask of its wave (wm: the wave's two masks, from k_cull_waves): every quad outside the mask is out
 // of reach of every ray of the wave, so ray_quad misses it. Returns
 //   0 — no traversal: no candidate, or the ray misses every masked one, so no quad can be hit;
-//   1 — the ray hits a masked candidate: the traversal's cell walk runs, and since every quad's
-//       ray_quad result is known (rh: the masked candidates' results, a miss for all others), it
-//       looks ids up instead of testing them; its first hit is the reference traversal's;
-//   2 — the traversal runs with its own tests (too many candidates to list or mask).
+//   1 — the ray hits one or two masked candidates (rh): the traversal's cell walk runs (dda_known),
+//       and since every quad's ray_quad result is known (a hit for those, a miss for all others), it
+//       only visits the cells listing them; its first hit is the reference traversal's;
+//   2 — the traversal runs with its own tests (too many candidates to list or mask, or more than two
+//       hit).
 // Records are read with scalar loads.
 __device__ __forceinline__ int traversal_plan(const WideList *wl, const uint64_t *wm, int kind, V3 o, V3 d,
-                                              const float4 *quads, RayHits &rh) {
+                                              const DevGrid &g, RayHits &rh) {
     const int n = ld_uniform(&wl->n[kind]);
     if (n == 0) return 0;
     if (n > kCullWide) return 2;
     uint64_t m = ld_uniform(wm + kind);
     if (__builtin_popcountll(m) > kRayTests) return 2;
     rh.n = 0;
-    rh.bits = 0u;
+    rh.id0 = rh.id1 = -1;
 #pragma unroll
     for (int j = 0; j < kRayTests; ++j) {
         if (!m) break;
         const int i = __builtin_ctzll(m);
         m &= m - 1;
         const int id = ld_uniform(&wl->ids[kind][i]);
-        const float4 qa = ld_uniform(quads + 3 * id), qb = ld_uniform(quads + 3 * id + 1),
-                     qc = ld_uniform(quads + 3 * id + 2);
+        const float4 qa = ld_uniform(g.quads + 3 * id), qb = ld_uniform(g.quads + 3 * id + 1),
+                     qc = ld_uniform(g.quads + 3 * id + 2);
         float t, u, v;
-        if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v))
-            rh.bits |= 1u << j;
-        rh.ids[j] = id;
-        rh.n = j + 1;
+        if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) {
+            if (rh.n == 0) rh.id0 = id;
+            else rh.id1 = id;
+            ++rh.n;
+        }
     }
-    return rh.bits ? 1 : 0;
+    if (rh.n == 0) return 0;
+    if (rh.n > 2) return 2;
+    // the cells that list the hit quads (grid_match's ranges; the union when there are two)
+    const int4 lo0 = g.boxes[2 * rh.id0], hi0 = g.boxes[2 * rh.id0 + 1];
+    rh.x0 = lo0.x, rh.y0 = lo0.y, rh.z0 = lo0.z, rh.x1 = hi0.x, rh.y1 = hi0.y, rh.z1 = hi0.z;
+    if (rh.n == 2) {
+        const int4 lo1 = g.boxes[2 * rh.id1], hi1 = g.boxes[2 * rh.id1 + 1];
+        rh.x0 = min(rh.x0, lo1.x), rh.y0 = min(rh.y0, lo1.y), rh.z0 = min(rh.z0, lo1.z);
+        rh.x1 = max(rh.x1, hi1.x), rh.y1 = max(rh.y1, hi1.y), rh.z1 = max(rh.z1, hi1.z);
+    }
+    return 1;
 }
 
-// The traversal of a ray with plan 1 (traversal_plan): the reference's cell walk, taking each quad's
-// hit from rh (a masked candidate's ray_quad result, a miss for every other quad) instead of testing
-// it. Kept apart from dda_collision's walk, which plan 2 runs: one loop with both tests holds more
-// registers through the whole trace.
+// The traversal of a ray with plan 1 (traversal_plan): the reference's cell walk in its order, where
+// only cells inside the hit quads' range can list a quad the ray hits. There the cell's candidates are
+// checked in list order against the hit ids; elsewhere the cell is stepped over without reading it;
+// once the walk has passed the range on any axis (indices only grow along the ray) no later cell is in
+// it and the walk ends without a hit. Its first hit is the reference traversal's.
 template <bool STATS>
 __device__ __forceinline__ bool dda_known(const RayHits &rh, V3 o, V3 d, const DevGrid &g, int own_lo, int own_hi,
                                           int &hit_id, RayStats &st) {
-    return dda_walk<STATS>(o, d, g, own_lo, own_hi, hit_id, st, [&](int id) {
-        bool hit = false;
-#pragma unroll
-        for (int j = 0; j < kRayTests; ++j) hit |= (j < rh.n) & (id == rh.ids[j]) & (((rh.bits >> j) & 1u) != 0u);
-        return hit;
-    });
+    if ((rh.x0 > rh.x1) | (rh.y0 > rh.y1) | (rh.z0 > rh.z1)) return false;  // listed in no cell
+    return dda_walk(o, d, g, hit_id, st,
+                    [&](int addr, int ix, int iy, int iz, int sx, int sy, int sz, int &hid, RayStats &s) {
+                        const bool inside = (ix >= rh.x0) & (ix <= rh.x1) & (iy >= rh.y0) & (iy <= rh.y1) &
+                                            (iz >= rh.z0) & (iz <= rh.z1);
+                        if (inside)
+                            return cell_candidates<STATS>(g, addr, own_lo, own_hi, hid, s,
+                                                          [&](int id) { return (id == rh.id0) | (id == rh.id1); });
+                        const bool past = (sx > 0 ? ix > rh.x1 : ix < rh.x0) | (sy > 0 ? iy > rh.y1 : iy < rh.y0) |
+                                          (sz > 0 ? iz > rh.z1 : iz < rh.z0);
+                        return past ? -1 : 0;
+                    });
 }
 
 // Half-angles (radians) at or above which a cone is not culled: the bound needs theta < 90 deg.

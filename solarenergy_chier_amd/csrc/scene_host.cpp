@@ -752,6 +752,12 @@ void grid_match(HostGrid &g, const std::vector<HostHeliostat> &hs) {
                                start_sub, ss});
         start_sub += ss;
     }
+    g.sub_box.assign(6 * (size_t)start_sub, 0);
+    for (const Range &r : ranges)
+        for (int k = 0; k < r.n; ++k) {
+            int *b = &g.sub_box[6 * (size_t)(r.first + k)];
+            b[0] = r.x0, b[1] = r.x1, b[2] = r.y0, b[3] = r.y1, b[4] = r.z0, b[5] = r.z1;
+        }
     g.cell_start.assign(ncell + 1, 0);
     for (const Range &r : ranges)
         for (int x = r.x0; x <= r.x1; ++x)

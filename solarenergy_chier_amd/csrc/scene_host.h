@@ -33,6 +33,9 @@ struct HostGrid {
     std::vector<int> cell_start;  // nx*ny*nz + 1
     std::vector<int> match;       // grid-relative sub-heliostat ids
     std::vector<V3> sub_verts;    // 3 per sub-heliostat (v0, v1, v2 order of the reference)
+    // per listed sub-heliostat id: the cell range whose lists hold it (x0, x1, y0, y1, z0, z1; empty
+    // when x0 > x1 or y0 > y1 or z0 > z1)
+    std::vector<int> sub_box;
     int n_sub = 0;
 };
 
