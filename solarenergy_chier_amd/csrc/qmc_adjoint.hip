@@ -41,7 +41,6 @@ __device__ __forceinline__ D3 operator+(D3 a, D3 b) { return D3{a.x + b.x, a.y +
 __device__ __forceinline__ D3 operator-(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 __device__ __forceinline__ D3 operator-(D3 a) { return D3{-a.x, -a.y, -a.z}; }
 __device__ __forceinline__ D3 operator*(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ D3 operator/(D3 a, double s) { return D3{a.x / s, a.y / s, a.z / s}; }
 __device__ __forceinline__ double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ D3 dcross(D3 a, D3 b) {
     return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
@@ -50,14 +49,14 @@ __device__ __forceinline__ D3 ldd(const double *p) { return D3{p[0], p[1], p[2]}
 
 struct AdjHelio {
     double u[3], n[3], v[3];
-    double nw1, nw2;   // |Y x n|, |u x n|
+    double nw1, nw2;   // 1/|Y x n|, 1/|u x n|
     double p[3];       // position
-    int64_t ray_base;  // first mask bit of this heliostat (multiple of 64)
+    int64_t ray_base;  // first mask bit of this heliostat (multiple of 64); bit ray_base + slot per ray slot
 };
 
 struct AdjSun {
     double us[3], m[3], vs[3];  // frame of m = -s
-    double nw1, nw2;
+    double nw1, nw2;            // 1/|Y x m|, 1/|us x m|
 };
 
 struct AdjArgs {
@@ -73,19 +72,21 @@ struct AdjArgs {
     const int32_t *offsets;
     int32_t P, L;
     FastDiv fdP, fdL;
+    int32_t n_list;
     float *flux;          // bilinear map (forward)
     double *flux64;       // bilinear map (masked forward)
-    uint32_t *mask;       // visibility bits
-    const CullList *cull; // per heliostat id: candidate quads of each traversal (null: no culling)
+    uint32_t *mask;       // visibility bits, one per ray slot (slot_ray order)
+    const WideList *wide; // per heliostat id: candidate quads of each traversal
+    const uint64_t *wmask;// per-wave candidate masks of the list's slots (k_cull_waves)
+    int32_t wstride;      // waves per list entry in wmask
     const float *G;       // dL/dflux (backward)
-    double *g_n;          // per heliostat dL/dn (3 doubles), indexed by heliostat id
-    double *g_m;          // dL/dm (3 doubles), m = -s
+    double *g_part;       // backward: per block, its rays' dL/dn then dL/dm (6 doubles), m = -s
 };
 
 // Forward values of the differentiated model for one ray (kept for the reverse sweep).
 struct RayModel {
     D3 lf, N, ii, d, pv, T, q;
-    double lN, k, lr, det, uu, vv, t, c, eta, deta, E;
+    double ilN, k, ilr, idet, uu, vv, t, c, eta, deta, E;  // ilN, ilr, idet: reciprocals
 };
 
 __device__ __forceinline__ void ray_model(const AdjArgs &a, const DevHelio &H, const AdjHelio &Ah, int m, float4 qi,
@@ -102,22 +103,23 @@ __device__ __forceinline__ void ray_model(const AdjArgs &a, const DevHelio &H, c
     const D3 O = ldd(Ah.p) + u * r.lf.x + n * r.lf.y + v * r.lf.z;
     const D3 din = us * (double)qi.x + mm * (double)qi.y + vs * (double)qi.z;
     const D3 Nraw = u * (double)qp.x + n * (double)qp.y + v * (double)qp.z;
-    r.lN = sqrt(ddot(Nraw, Nraw));
-    r.N = Nraw / r.lN;
+    // divisions by a vector length or the determinant are products with one reciprocal
+    r.ilN = 1.0 / sqrt(ddot(Nraw, Nraw));
+    r.N = Nraw * r.ilN;
     r.ii = -din;
     r.k = ddot(r.N, r.ii);
     const D3 rr = r.ii - r.N * (r.k * 2.0);
-    r.lr = sqrt(ddot(rr, rr));
-    r.d = rr / r.lr;
+    r.ilr = 1.0 / sqrt(ddot(rr, rr));
+    r.d = rr * r.ilr;
     const DevReceiver &R = a.recv;
     const D3 A0{R.A[0], R.A[1], R.A[2]}, E1{R.E1[0], R.E1[1], R.E1[2]}, E2{R.E2[0], R.E2[1], R.E2[2]};
     r.pv = dcross(r.d, E2);
-    r.det = ddot(E1, r.pv);
+    r.idet = 1.0 / ddot(E1, r.pv);
     r.T = O - A0;
-    r.uu = ddot(r.T, r.pv) / r.det;
+    r.uu = ddot(r.T, r.pv) * r.idet;
     r.q = dcross(r.T, E1);
-    r.vv = ddot(r.d, r.q) / r.det;
-    r.t = ddot(E2, r.q) / r.det;
+    r.vv = ddot(r.d, r.q) * r.idet;
+    r.t = ddot(E2, r.q) * r.idet;
     // atmospheric attenuation (global_function.cuh eta_aAlpha), with its derivative
     if (r.t <= 1000.0) {
         r.eta = 0.99331 - r.t * 0.0001176 + r.t * r.t * 1.97e-8;
@@ -130,8 +132,62 @@ __device__ __forceinline__ void ray_model(const AdjArgs &a, const DevHelio &H, c
     r.E = fabs(r.c) * r.eta * (double)H.factor;
 }
 
-template <bool F64>
-__device__ __forceinline__ void bilinear_splat(const AdjArgs &a, double uu, double vv, double E) {
+// Bilinear deposit of E at (vv*res_x - 0.5, uu*res_y - 0.5) into the double map (masked forward).
+__device__ __forceinline__ void bilinear_splat_f64(const AdjArgs &a, double uu, double vv, double E) {
+    const int resx = a.recv.res_x, resy = a.recv.res_y;
+    const double x = vv * (double)resx - 0.5, y = uu * (double)resy - 0.5;
+    const double fx0 = floor(x), fy0 = floor(y);
+    const int i0 = (int)fx0, j0 = (int)fy0;
+    const double fx = x - fx0, fy = y - fy0;
+    const double wx[2] = {1.0 - fx, fx}, wy[2] = {1.0 - fy, fy};
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int i = i0 + c, j = j0 + b;
+            if (i >= 0 && i < resx && j >= 0 && j < resy) unsafeAtomicAdd(a.flux64 + j * resx + i, E * wx[c] * wy[b]);
+        }
+}
+
+// Visibility of ray r (slot `slot` of list entry li, heliostat h) by the plain trace's float rules
+// (RectangleReceiverRectangleGridRayTracing.cu:14-40) with its culling: traversals proved hit-free
+// are skipped, the others walk only the cells of the masked candidates the ray hits. True when the
+// ray reaches the receiver, with its micro-facet m and pool samples s (sun shape) and q (slope error).
+__device__ __forceinline__ bool adj_visible(const AdjArgs &a, const DevHelio &H, int h, int li, long long slot,
+                                            long long r, int &m, float4 &s, float4 &q) {
+    const int L = a.L, P = a.P;
+    m = (int)fast_div((uint32_t)r, a.fdL);
+    const int j = (int)r - m * L;
+    const int32_t *grp = a.offsets + H.offsets_base;
+    s = a.pool[fast_mod((uint32_t)(grp[m] + j), P, a.fdP)];
+    q = a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)];
+    V3 dir = fr_apply(a.sunf, V3{s.x, s.y, s.z});
+    const V3 origin = flat_facet_origin(H, H.fr, m);
+    const WideList *wl = a.wide + h;
+    // the caller's slot blocks hold whole waves, so the wave's masks are uniform
+    const uint64_t *wm = a.wmask + 2 * ((size_t)li * a.wstride + (size_t)(slot >> 6));
+    RayStats st{0u, 0u, 0u};
+    RayHits rh;
+    int hit;
+    const int lo = H.sub_begin, hi = H.sub_begin + H.sub_count;
+    int plan = traversal_plan(wl, wm, 0, origin, dir, a.grid, rh);
+    if (plan != 0 && (plan == 2 ? dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st)
+                                : dda_known<false>(rh, origin, dir, a.grid, lo, hi, hit, st)))
+        return false;
+    const V3 normal = normalize(fr_apply(H.fr, V3{q.x, q.y, q.z}));
+    dir = normalize(reflect(-dir, normal));
+    plan = traversal_plan(wl, wm, 1, origin, dir, a.grid, rh);
+    if (plan != 0 && (plan == 2 ? dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st)
+                                : dda_known<false>(rh, origin, dir, a.grid, lo, hi, hit, st)))
+        return false;
+    float t, u, v;
+    return receiver_hit<0>(a.recv, origin, dir, t, u, v);
+}
+
+// Bilinear deposit of E at (vv*res_x - 0.5, uu*res_y - 0.5): rows [row0, row0 + rows) into the
+// workgroup's LDS band, the others as device-scope float atomics into the map.
+__device__ __forceinline__ void bilinear_splat_band(const AdjArgs &a, float *band, int row0, int rows, double uu,
+                                                    double vv, double E) {
     const int resx = a.recv.res_x, resy = a.recv.res_y;
     const double x = vv * (double)resx - 0.5, y = uu * (double)resy - 0.5;
     const double fx0 = floor(x), fy0 = floor(y);
@@ -144,58 +200,83 @@ __device__ __forceinline__ void bilinear_splat(const AdjArgs &a, double uu, doub
         for (int c = 0; c < 2; ++c) {
             const int i = i0 + c, j = j0 + b;
             if (i >= 0 && i < resx && j >= 0 && j < resy) {
-                const double w = E * wx[c] * wy[b];
-                if (F64)
-                    unsafeAtomicAdd(a.flux64 + j * resx + i, w);
+                const float w = (float)(E * wx[c] * wy[b]);
+                const int br = j - row0;
+                if ((unsigned)br < (unsigned)rows)
+                    atomicAdd(band + br * resx + i, w);
                 else
-                    unsafeAtomicAdd(a.flux + j * resx + i, (float)w);
+                    unsafeAtomicAdd(a.flux + j * resx + i, w);
             }
         }
 }
 
-// Forward with full visibility: float visibility trace (the plain tracer's), double bilinear
-// deposit, one visibility bit per ray.
-__global__ __launch_bounds__(256) void k_adj_forward(AdjArgs a) {
-    const int h = a.list[blockIdx.y];
-    const DevHelio &H = a.helios[h];
-    const int L = a.L, P = a.P;
-    const long long nrays = (long long)H.F * L;
-    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool visible = false;
-    if (r < nrays) {
-        const int m = (int)fast_div((uint32_t)r, a.fdL), j = (int)r - m * L;
-        const int32_t *grp = a.offsets + H.offsets_base;
-        const float4 s = a.pool[fast_mod((uint32_t)(grp[m] + j), P, a.fdP)];
-        V3 dir = fr_apply(a.sunf, V3{s.x, s.y, s.z});
-        const V3 origin = flat_facet_origin(H, H.fr, m);
-        RayStats st{0u, 0u, 0u};
-        int hit;
-        const int lo = H.sub_begin, hi = H.sub_begin + H.sub_count;
-        // traversals proved hit-free for this ray are skipped (same visibility)
-        if (!((!a.cull || traversal_needed(a.cull + h, 0, origin, dir, a.grid.quads)) &&
-              dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
-            const float4 q = a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)];
-            const V3 normal = normalize(fr_apply(H.fr, V3{q.x, q.y, q.z}));
-            dir = normalize(reflect(-dir, normal));
-            if (!((!a.cull || traversal_needed(a.cull + h, 1, origin, dir, a.grid.quads)) &&
-                  dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))) {
-                float t, u, v;
-                if (receiver_hit<0>(a.recv, origin, dir, t, u, v)) {
-                    visible = true;
-                    RayModel rm;
-                    ray_model(a, H, a.ah[h], m, s, q, rm);
-                    bilinear_splat<false>(a, rm.uu, rm.vv, rm.E);
-                }
+// Forward with full visibility: visibility by adj_visible, double bilinear deposit, one visibility
+// bit per ray slot. Persistent like the plain banded trace: each of a deposit's four pixels is an
+// atomic, so the workgroup keeps map rows [row0, row0 + rows) in LDS (the whole map when it fits)
+// and takes BS-slot blocks (list entry, block) round robin; at the end it stores its band to its own
+// slice of `partials` (k_adj_band_fold adds them).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_adj_forward(AdjArgs a, int row0, int rows, int blocks_per_helio,
+                                                    float *partials) {
+    extern __shared__ float band[];
+    const int W = a.recv.res_x, nband = rows * W;
+    for (int p = threadIdx.x; p < nband; p += BS) band[p] = 0.0f;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    int li = (int)(blockIdx.x / (unsigned)blocks_per_helio);
+    int bx = (int)blockIdx.x - li * blocks_per_helio;
+#pragma nounroll
+    while (li < a.n_list) {
+        // the launch writes none of these records, so they are read with scalar loads
+        const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
+        const DevHelio H = ld_uniform(a.helios + h);
+        const long long slot = (long long)__builtin_amdgcn_readfirstlane(bx) * BS + threadIdx.x;
+        const long long r = slot_ray(H, a.L, slot);
+        bool visible = false;
+        if (r >= 0) {
+            int m;
+            float4 s, q;
+            visible = adj_visible(a, H, h, li, slot, r, m, s, q);
+            if (visible) {
+                RayModel rm;
+                ray_model(a, H, a.ah[h], m, s, q, rm);
+                bilinear_splat_band(a, band, row0, rows, rm.uu, rm.vv, rm.E);
             }
         }
+        // ray_base is a multiple of 64 and blocks hold whole waves: each wave owns two mask words
+        const unsigned long long bits = __ballot(visible);
+        if ((lane & 31) == 0 && slot - lane < heliostat_slot_count(H, a.L)) {
+            const long long bit = a.ah[h].ray_base + slot;
+            a.mask[bit >> 5] = (uint32_t)(bits >> lane);
+        }
+        bx += (int)gridDim.x;
+        while (bx >= blocks_per_helio) {
+            bx -= blocks_per_helio;
+            ++li;
+        }
     }
-    // ray_base is a multiple of 64 and blocks hold whole waves: each wave owns two mask words
-    const unsigned long long bits = __ballot(visible);
-    const int lane = threadIdx.x & 63;
-    if ((lane & 31) == 0 && r - lane < nrays) {
-        const long long bit = a.ah[h].ray_base + r;
-        a.mask[bit >> 5] = (uint32_t)(bits >> lane);
+    __syncthreads();
+    float *mine = partials + (size_t)blockIdx.x * nband;
+    for (int p = threadIdx.x; p < nband; p += BS) mine[p] = band[p];
+}
+
+// Map rows [row0, row0 + nband / W) of the forward: the sum (in double) of the workgroups' bands. A
+// block takes 64 pixels; its wave w adds the bands w, w + 4, ... for them (coalesced rows of
+// partials), and the four wave sums are added in a fixed order. Those rows take no device-scope
+// deposits, so the sum is stored.
+__global__ __launch_bounds__(256) void k_adj_band_fold(float *flux, const float *partials, int n_parts, int nband,
+                                                       int offset) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    double s = 0.0;
+    if (p < nband) {
+#pragma unroll 8
+        for (int k = w; k < n_parts; k += 4) s += (double)partials[(size_t)k * nband + p];
     }
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && p < nband) flux[offset + p] = (float)(((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]);
 }
 
 // Forward of the frozen-visibility model in double (finite differences with common random numbers).
@@ -203,21 +284,21 @@ __global__ __launch_bounds__(256) void k_adj_forward_masked(AdjArgs a) {
     const int h = a.list[blockIdx.y];
     const DevHelio &H = a.helios[h];
     const int L = a.L, P = a.P;
-    const long long nrays = (long long)H.F * L;
-    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrays) return;
-    const long long bit = a.ah[h].ray_base + r;
+    const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long r = slot_ray(H, L, slot);
+    if (r < 0) return;
+    const long long bit = a.ah[h].ray_base + slot;
     if (!((a.mask[bit >> 5] >> (bit & 31)) & 1u)) return;
     const int m = (int)fast_div((uint32_t)r, a.fdL), j = (int)r - m * L;
     const int32_t *grp = a.offsets + H.offsets_base;
     RayModel rm;
     ray_model(a, H, a.ah[h], m, a.pool[fast_mod((uint32_t)(grp[m] + j), P, a.fdP)],
               a.pert[fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP)], rm);
-    bilinear_splat<true>(a, rm.uu, rm.vv, rm.E);
+    bilinear_splat_f64(a, rm.uu, rm.vv, rm.E);
 }
 
-__device__ __forceinline__ D3 norm_bwd(D3 y, double len, D3 g) {  // y = x/|x|
-    return (g - y * ddot(y, g)) / len;
+__device__ __forceinline__ D3 norm_bwd(D3 y, double inv_len, D3 g) {  // y = x/|x|, inv_len = 1/|x|
+    return (g - y * ddot(y, g)) * inv_len;
 }
 
 __global__ __launch_bounds__(256) void k_adj_backward(AdjArgs a) {
@@ -225,12 +306,12 @@ __global__ __launch_bounds__(256) void k_adj_backward(AdjArgs a) {
     const int h = a.list[blockIdx.y];
     const DevHelio &H = a.helios[h];
     const int L = a.L, P = a.P;
-    const long long nrays = (long long)H.F * L;
-    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long r = slot_ray(H, L, slot);
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // dL/dn (3), dL/dm (3)
     bool vis = false;
-    if (r < nrays) {
-        const long long bit = a.ah[h].ray_base + r;
+    if (r >= 0) {
+        const long long bit = a.ah[h].ray_base + slot;
         vis = (a.mask[bit >> 5] >> (bit & 31)) & 1u;
     }
     if (vis) {
@@ -270,20 +351,20 @@ __global__ __launch_bounds__(256) void k_adj_backward(AdjArgs a) {
         D3 g_N = f.d * g_c;
         const DevReceiver &R = a.recv;
         const D3 E1{R.E1[0], R.E1[1], R.E1[2]}, E2{R.E2[0], R.E2[1], R.E2[2]};
-        const double g_a1 = g_uu / f.det, g_a2 = g_vv / f.det, g_a3 = g_t / f.det;
-        const double g_det = -(g_uu * f.uu + g_vv * f.vv + g_t * f.t) / f.det;
+        const double g_a1 = g_uu * f.idet, g_a2 = g_vv * f.idet, g_a3 = g_t * f.idet;
+        const double g_det = -(g_uu * f.uu + g_vv * f.vv + g_t * f.t) * f.idet;
         D3 g_T = f.pv * g_a1;
         const D3 g_pv = f.T * g_a1 + E1 * g_det;
         g_d = g_d + dcross(E2, g_pv) + f.q * g_a2;
         const D3 g_q = f.d * g_a2 + E2 * g_a3;
         g_T = g_T + dcross(E1, g_q);
         const D3 g_O = g_T;
-        const D3 g_r = norm_bwd(f.d, f.lr, g_d);
+        const D3 g_r = norm_bwd(f.d, f.ilr, g_d);
         const double grN = ddot(f.N, g_r);
         const D3 g_i = g_r - f.N * (2.0 * grN);
         g_N = g_N - (g_r * f.k + f.ii * grN) * 2.0;
         const D3 g_din = -g_i;
-        const D3 g_Nraw = norm_bwd(f.N, f.lN, g_N);
+        const D3 g_Nraw = norm_bwd(f.N, f.ilN, g_N);
         const D3 u = ldd(Ah.u), n = ldd(Ah.n), v = ldd(Ah.v);
         D3 g_u = g_Nraw * (double)qp4.x + g_O * f.lf.x;
         D3 gn = g_Nraw * (double)qp4.y + g_O * f.lf.y;
@@ -307,7 +388,8 @@ __global__ __launch_bounds__(256) void k_adj_backward(AdjArgs a) {
         acc[0] = gn.x; acc[1] = gn.y; acc[2] = gn.z;
         acc[3] = gm.x; acc[4] = gm.y; acc[5] = gm.z;
     }
-    // one block = a slice of one heliostat's rays: wave sums, block sum, six atomics
+    // one block = a slice of one heliostat's rays: wave sums, then the block's sum in a fixed order
+    // into its own partial (no atomics: every block shares dL/dm, and the result is reproducible)
 #pragma unroll
     for (int k = 0; k < 6; ++k)
         for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
@@ -320,7 +402,29 @@ __global__ __launch_bounds__(256) void k_adj_backward(AdjArgs a) {
         const int k = threadIdx.x;
         double s = 0.0;
         for (int ww = 0; ww < (int)(blockDim.x >> 6); ++ww) s += red[ww][k];
-        unsafeAtomicAdd(k < 3 ? a.g_n + 3 * h + k : a.g_m + (k - 3), s);
+        a.g_part[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 6 + k] = s;
+    }
+}
+
+// Per list entry li: the sum of its n_blocks backward partials (lane-strided, then a fixed shuffle
+// tree), dL/dn into g_n[3 * list[li]] and the heliostat's share of dL/dm into g_m[3 * li].
+__global__ __launch_bounds__(64) void k_adj_reduce(const double *part, int n_blocks, const int32_t *list, double *g_n,
+                                                   double *g_m) {
+    const int li = blockIdx.x, lane = threadIdx.x;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int b = lane; b < n_blocks; b += 64)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) acc[k] += part[((size_t)li * n_blocks + b) * 6 + k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        for (int o = 32; o > 0; o >>= 1) acc[k] += __shfl_xor(acc[k], o);
+    if (lane == 0) {
+        const int h = list[li];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            g_n[3 * h + k] = acc[k];
+            g_m[3 * li + k] = acc[3 + k];
+        }
     }
 }
 
@@ -354,24 +458,73 @@ struct AdjState {
     std::vector<int32_t> list;
     std::vector<AdjHelio> h_ah;  // indexed by heliostat id
     AdjHelio *d_ah = nullptr;
+    size_t ah_cap = 0;
     int32_t *d_list = nullptr;
+    size_t list_cap = 0;
+    // pinned staging of the uploads (read by the copies until up_done) and of the gradient results
+    AdjHelio *p_ah = nullptr;
+    size_t p_ah_cap = 0;
+    int32_t *p_list = nullptr;
+    size_t p_list_cap = 0;
+    double *p_out = nullptr;
+    size_t p_out_cap = 0;
+    hipEvent_t up_done = nullptr;
     uint32_t *d_mask = nullptr;
     size_t mask_words = 0;
-    double *d_gn = nullptr;
-    double *d_gm = nullptr;
+    double *d_gout = nullptr;    // per heliostat id dL/dn (3 doubles), then per list entry its share of dL/dm
+    size_t gout_cap = 0;
+    double *d_gpart = nullptr;   // backward partials, 6 doubles per block
+    size_t gpart_cap = 0;
     float *d_G = nullptr;
     double *d_partial = nullptr;
     double *d_flux64 = nullptr;
     size_t flux64_cap = 0;
     size_t npix = 0;
+    uint64_t *d_wmask = nullptr;
+    size_t wmask_cap = 0;  // uint64 entries
+    float *d_parts = nullptr;
+    size_t parts_cap = 0;  // floats
     bool mask_ready = false;
     AdjSun sun{};
-    int maxF = 0;
+    long long max_slots = 0;
 };
 
 AdjState *state(qmc_ctx *c) {
     if (!c->adj) c->adj = new AdjState();
     return static_cast<AdjState *>(c->adj);
+}
+
+// Grows a device buffer to at least `need` elements, without keeping its contents; earlier launches
+// may still use the old buffer, so the stream is drained first.
+template <class T>
+qmc_status grow(qmc_ctx *c, T *&p, size_t &cap, size_t need) {
+    if (p && need <= cap) return QMC_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    cap = 0;
+    if (qmc_status st = dev_alloc(p, need)) return st;
+    cap = std::max<size_t>(need, 1);
+    return QMC_OK;
+}
+
+// The same for pinned host memory (the caller makes sure that no copy still reads or writes it).
+template <class T>
+qmc_status grow_pinned(T *&p, size_t &cap, size_t need) {
+    if (p && need <= cap) return QMC_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    need = std::max<size_t>(need, 1);
+    HIP_TRY(hipHostMalloc((void **)&p, need * sizeof(T)));
+    cap = need;
+    return QMC_OK;
+}
+
+// Slots per block of the persistent forward (one workgroup per CU with the LDS band).
+constexpr int kAdjBS = 1024;
+
+// Grid of the masked forward and the backward: 256-slot blocks (x) of each list entry (y).
+dim3 adj_grid(qmc_ctx *, AdjState *s) {
+    return dim3((unsigned)((s->max_slots + 255) / 256), (unsigned)s->list.size());
 }
 
 struct H3 {
@@ -390,11 +543,15 @@ void put(double *dst, H3 a) {
     dst[2] = a.z;
 }
 
-// (u, v) of the frame of n with u = normalize(Y x n), v = normalize(u x n); false when n is
-// (anti)parallel to Y, where the frame is not differentiable.
-bool frame_f64(H3 n, H3 &u, H3 &v, double &nw1, double &nw2) {
+// (u, v) of the frame of n with u = normalize(Y x n), v = normalize(u x n), and the reciprocals of
+// the two normalised lengths; false when n is (anti)parallel to Y, where the frame is not
+// differentiable.
+bool frame_f64(H3 n, H3 &u, H3 &v, double &inw1, double &inw2) {
+    double nw1, nw2;
     u = hnorm(hcross(H3{0.0, 1.0, 0.0}, n), nw1);
     v = hnorm(hcross(u, n), nw2);
+    inw1 = 1.0 / nw1;
+    inw2 = 1.0 / nw2;
     return nw1 > 1e-9 && nw2 > 1e-9;
 }
 
@@ -417,7 +574,7 @@ qmc_status adj_prepare(qmc_ctx *c, int grid) {
     s->list.clear();
     s->h_ah.assign(c->h_helios.size(), AdjHelio{});
     int64_t base = 0;
-    s->maxF = 0;
+    s->max_slots = 0;
     for (size_t h = 0; h < c->h_helios.size(); ++h) {
         const DevHelio &H = c->h_helios[h];
         if (H.grid != grid) continue;
@@ -434,9 +591,13 @@ qmc_status adj_prepare(qmc_ctx *c, int grid) {
         put(A.v, v);
         for (int k = 0; k < 3; ++k) A.p[k] = H.pos[k];
         A.ray_base = base;
-        base += (((int64_t)H.F * c->L + 63) / 64) * 64;
-        s->maxF = std::max(s->maxF, H.F);
+        const long long slots = heliostat_slot_count(H, c->L);
+        base += ((slots + 63) / 64) * 64;
+        s->max_slots = std::max(s->max_slots, slots);
     }
+    // slots and ray ids are divided by multiply-high constants, exact below 2^31
+    if (s->max_slots >= 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "adjoint: 2^31 or more rays per heliostat");
+    if (s->list.size() > 65535) return fail(QMC_E_UNSUPPORTED, "adjoint: more than 65535 heliostats in one grid");
     const H3 mvec{-(double)c->cfg.sun_dir[0], -(double)c->cfg.sun_dir[1], -(double)c->cfg.sun_dir[2]};
     H3 us, vs;
     if (!frame_f64(mvec, us, vs, s->sun.nw1, s->sun.nw2))
@@ -445,27 +606,29 @@ qmc_status adj_prepare(qmc_ctx *c, int grid) {
     put(s->sun.m, mvec);
     put(s->sun.vs, vs);
     qmc_status st;
-    const size_t words = (size_t)(base / 32) + 2;
-    if (words > s->mask_words) {
-        if ((st = dev_alloc(s->d_mask, words))) return st;
-        s->mask_words = words;
-    }
-    if ((st = dev_alloc(s->d_ah, s->h_ah.size()))) return st;
-    if ((st = dev_alloc(s->d_list, std::max<size_t>(s->list.size(), 1)))) return st;
-    if ((st = dev_alloc(s->d_gn, 3 * s->h_ah.size()))) return st;
-    if (!s->d_gm && (st = dev_alloc(s->d_gm, 3))) return st;
+    const size_t nh = s->h_ah.size(), nl = s->list.size();
+    if ((st = grow(c, s->d_mask, s->mask_words, (size_t)(base / 32) + 2))) return st;
+    if ((st = grow(c, s->d_ah, s->ah_cap, nh))) return st;
+    if ((st = grow(c, s->d_list, s->list_cap, nl))) return st;
+    if ((st = grow(c, s->d_gout, s->gout_cap, 3 * (nh + nl)))) return st;
+    if ((st = grow(c, s->d_gpart, s->gpart_cap, 6 * (size_t)adj_grid(c, s).x * nl))) return st;
     const size_t npix = (size_t)rb.w * rb.h;
     if (npix != s->npix) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
         if ((st = dev_alloc(s->d_G, npix))) return st;
         if ((st = dev_alloc(s->d_partial, (npix + 255) / 256))) return st;
         s->npix = npix;
     }
-    HIP_TRY(hipMemcpyAsync(s->d_ah, s->h_ah.data(), sizeof(AdjHelio) * s->h_ah.size(), hipMemcpyHostToDevice,
-                           c->stream));
-    if (!s->list.empty())
-        HIP_TRY(hipMemcpyAsync(s->d_list, s->list.data(), sizeof(int32_t) * s->list.size(), hipMemcpyHostToDevice,
-                               c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    // the records go up through pinned staging: the previous upload must have read it
+    if (!s->up_done) HIP_TRY(hipEventCreateWithFlags(&s->up_done, hipEventDisableTiming));
+    else HIP_TRY(hipEventSynchronize(s->up_done));
+    if ((st = grow_pinned(s->p_ah, s->p_ah_cap, nh))) return st;
+    if ((st = grow_pinned(s->p_list, s->p_list_cap, nl))) return st;
+    std::memcpy(s->p_ah, s->h_ah.data(), sizeof(AdjHelio) * nh);
+    std::memcpy(s->p_list, s->list.data(), sizeof(int32_t) * nl);
+    HIP_TRY(hipMemcpyAsync(s->d_ah, s->p_ah, sizeof(AdjHelio) * nh, hipMemcpyHostToDevice, c->stream));
+    if (nl) HIP_TRY(hipMemcpyAsync(s->d_list, s->p_list, sizeof(int32_t) * nl, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipEventRecord(s->up_done, c->stream));
     return QMC_OK;
 }
 
@@ -483,6 +646,7 @@ AdjArgs adj_args(qmc_ctx *c, AdjState *s, float *flux) {
     a.pool = c->d_pool;
     a.pert = c->d_pert;
     a.offsets = c->d_offsets;
+    a.n_list = (int32_t)s->list.size();
     a.P = c->P;
     a.L = c->L;
     a.fdP = make_fastdiv((uint32_t)c->P);
@@ -491,15 +655,10 @@ AdjArgs adj_args(qmc_ctx *c, AdjState *s, float *flux) {
     a.flux64 = s->d_flux64;
     a.mask = s->d_mask;
     a.G = s->d_G;
-    a.g_n = s->d_gn;
-    a.g_m = s->d_gm;
+    a.g_part = s->d_gpart;
     return a;
 }
 
-dim3 adj_grid(qmc_ctx *c, AdjState *s) {
-    const long long rays = (long long)s->maxF * c->L;
-    return dim3((unsigned)((rays + 255) / 256), (unsigned)s->list.size());
-}
 
 }  // namespace
 
@@ -510,11 +669,20 @@ void qmc_adjoint_release(qmc_ctx *c) {
     dev_free(s->d_ah);
     dev_free(s->d_list);
     dev_free(s->d_mask);
-    dev_free(s->d_gn);
-    dev_free(s->d_gm);
+    dev_free(s->d_gout);
+    if (s->up_done) {
+        (void)hipEventSynchronize(s->up_done);
+        (void)hipEventDestroy(s->up_done);
+    }
+    if (s->p_ah) (void)hipHostFree(s->p_ah);
+    if (s->p_list) (void)hipHostFree(s->p_list);
+    if (s->p_out) (void)hipHostFree(s->p_out);
     dev_free(s->d_G);
     dev_free(s->d_partial);
     dev_free(s->d_flux64);
+    dev_free(s->d_wmask);
+    dev_free(s->d_parts);
+    dev_free(s->d_gpart);
     delete s;
     c->adj = nullptr;
 }
@@ -528,12 +696,57 @@ extern "C" qmc_status qmc_adjoint_forward(qmc_ctx *c, int32_t grid, float *d_flu
     AdjState *s = state(c);
     if (s->list.empty()) return fail(QMC_E_INVALID_ARG, "qmc_adjoint_forward: grid has no heliostats");
     const RecvBuf &rb = c->recvs[c->grids[grid].receiver];
-    HIP_TRY(hipMemsetAsync(d_flux, 0, sizeof(float) * (size_t)rb.w * rb.h, c->stream));
-    HIP_TRY(hipMemsetAsync(s->d_mask, 0, sizeof(uint32_t) * s->mask_words, c->stream));
     AdjArgs a = adj_args(c, s, d_flux);
-    if ((st = launch_cull(c, grid, s->d_list, (int)s->list.size(), &a.cull, nullptr))) return st;
-    hipLaunchKernelGGL(k_adj_forward, adj_grid(c, s), dim3(256), 0, c->stream, a);
+    // the plain trace's culling: wide candidate lists, then per-wave masks over the same slots
+    const int n = (int)s->list.size();
+    const CullList *cull = nullptr;
+    TraceArgs ta;
+    std::memset(&ta, 0, sizeof(ta));
+    if ((st = launch_cull(c, grid, s->d_list, n, &cull, &ta.wide))) return st;
+    const size_t need = 2 * (size_t)n * (size_t)wave_stride(s->max_slots);
+    if (need > s->wmask_cap) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if ((st = dev_alloc(s->d_wmask, need))) {
+            s->wmask_cap = 0;
+            return st;
+        }
+        s->wmask_cap = need;
+    }
+    if ((st = launch_cull_waves_into(c, grid, s->d_list, n, s->max_slots, s->d_wmask, ta))) return st;
+    a.wide = ta.wide;
+    a.wmask = ta.wmask;
+    a.wstride = ta.wstride;
+    // the band: as many map rows as the LDS holds, centred (flux concentrates around the aim points)
+    if ((st = device_limits(c))) return st;
+    const int W = rb.w, Hh = rb.h;
+    const int rows = (int)std::min<size_t>((size_t)Hh, std::min(c->max_lds, c->lds_per_cu) / (sizeof(float) * W));
+    const size_t lds = sizeof(float) * (size_t)rows * W;
+    int resident = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_adj_forward<kAdjBS>, kAdjBS, lds));
+    if (resident < 1) return fail(QMC_E_UNSUPPORTED, "qmc_adjoint_forward: no workgroup is resident");
+    const long long bph = (s->max_slots + kAdjBS - 1) / kAdjBS;
+    const int nwg = (int)std::min<long long>(c->num_cus, bph * n);
+    const long long nband = (long long)rows * W;
+    if ((size_t)(nwg * nband) > s->parts_cap) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if ((st = dev_alloc(s->d_parts, (size_t)(nwg * nband)))) {
+            s->parts_cap = 0;
+            return st;
+        }
+        s->parts_cap = (size_t)(nwg * nband);
+    }
+    const int row0 = (Hh - rows) / 2;
+    // band rows are stored by the fold; the others take atomics (every mask word of the list's slots
+    // is written by the forward)
+    if (rows < Hh) HIP_TRY(hipMemsetAsync(d_flux, 0, sizeof(float) * (size_t)W * Hh, c->stream));
+    hipLaunchKernelGGL(k_adj_forward<kAdjBS>, dim3((unsigned)nwg), dim3(kAdjBS), lds, c->stream, a, row0, rows,
+                       (int)bph, s->d_parts);
     HIP_TRY(hipGetLastError());
+    if (nband > 0) {
+        hipLaunchKernelGGL(k_adj_band_fold, dim3((unsigned)((nband + 63) / 64)), dim3(256), 0, c->stream, d_flux,
+                           s->d_parts, nwg, (int)nband, row0 * W);
+        HIP_TRY(hipGetLastError());
+    }
     s->mask_ready = true;
     return QMC_OK;
 }
@@ -607,15 +820,23 @@ extern "C" qmc_status qmc_adjoint_backward(qmc_ctx *c, const qmc_scene *scene, i
     if (st) return st;
     if (d_dL_dflux && d_dL_dflux != s->d_G)
         HIP_TRY(hipMemcpyAsync(s->d_G, d_dL_dflux, sizeof(float) * s->npix, hipMemcpyDeviceToDevice, c->stream));
-    HIP_TRY(hipMemsetAsync(s->d_gn, 0, sizeof(double) * 3 * s->h_ah.size(), c->stream));
-    HIP_TRY(hipMemsetAsync(s->d_gm, 0, sizeof(double) * 3, c->stream));
-    hipLaunchKernelGGL(k_adj_backward, adj_grid(c, s), dim3(256), 0, c->stream, adj_args(c, s, nullptr));
+    const dim3 g = adj_grid(c, s);
+    hipLaunchKernelGGL(k_adj_backward, g, dim3(256), 0, c->stream, adj_args(c, s, nullptr));
     HIP_TRY(hipGetLastError());
-    std::vector<double> gn(3 * s->h_ah.size());
-    double gm[3];
-    HIP_TRY(hipMemcpyAsync(gn.data(), s->d_gn, sizeof(double) * gn.size(), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipMemcpyAsync(gm, s->d_gm, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    const int nl = (int)s->list.size();
+    const size_t nh3 = 3 * s->h_ah.size();
+    // every list entry's dL/dn is written; other heliostats' entries are never read
+    hipLaunchKernelGGL(k_adj_reduce, dim3((unsigned)nl), dim3(64), 0, c->stream, s->d_gpart, (int)g.x, s->d_list,
+                       s->d_gout, s->d_gout + nh3);
+    HIP_TRY(hipGetLastError());
+    if ((st = grow_pinned(s->p_out, s->p_out_cap, nh3 + 3 * (size_t)nl))) return st;
+    HIP_TRY(hipMemcpyAsync(s->p_out, s->d_gout, sizeof(double) * (nh3 + 3 * (size_t)nl), hipMemcpyDeviceToHost,
+                           c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    const double *gn = s->p_out, *gml = s->p_out + nh3;
+    double gm[3] = {0.0, 0.0, 0.0};
+    for (int li = 0; li < nl; ++li)
+        for (int k = 0; k < 3; ++k) gm[k] += gml[3 * (size_t)li + k];
     // chain n_h = normalize(normalize(A - p) - s), A = focus + o0*across + o1*up, in double
     double gs[3] = {-gm[0], -gm[1], -gm[2]};
     const int nh = (int)scene->heliostats.size();
