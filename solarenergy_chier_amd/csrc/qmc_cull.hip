@@ -19,13 +19,117 @@ using namespace qmc;
 
 namespace {
 
+constexpr int kCullBS = 256;            // lanes of a k_cull workgroup
+constexpr int kScanRound = 4 * kCullBS; // quad groups per round of the scan
+
+// The traversal-culling scan of one heliostat (culling shape S, own quads [own_lo, own_hi)) over a
+// grid's quad groups, by the whole workgroup: every quad that passes the bound (sphere_may_hit, then
+// cone_may_hit) for the shadow cone (kind 0: tan_sun around the sun axis, the n axis of sunf) or the
+// blocking cone (kind 1: tan_block around the sun axis reflected in the heliostat normal) is counted in
+// n[kind] and, while the count is at most cap (<= kCullWide), listed in ids[kind] (in no particular
+// order); a negative tangent skips that kind. Each round takes kScanRound groups: one lane per group
+// tests the group's ball (sphere_may_hit is monotone in the ball: a ball inside one that fails fails
+// too, so no quad that would pass is skipped); the passing groups' quads are then spread over all
+// lanes, one (group, quad) pair each, so that a group near the heliostat does not leave one lane
+// testing its quads while the others idle. A kind stops being scanned once its count exceeds cap (it
+// is then traversed for every ray and its list is not used), so a final count <= cap is complete.
+// n and ids are in LDS, n zeroed beforehand; every lane of the workgroup calls this.
+__device__ void cull_scan(const DevCullShape &S, const DevFrame &sunf, const float4 *quads, const QuadGroup *groups,
+                          int n_groups, int own_lo, int own_hi, double tan_sun, double tan_block, double margin,
+                          int cap, int *n, int (*ids)[kCullWide]) {
+    __shared__ int s_sel[kScanRound];  // passing groups of the round: index, kinds in bits 29 and 30
+    __shared__ int s_end[kScanRound];  // inclusive prefix sums of their quad counts
+    __shared__ int s_cnt;
+    __shared__ int s_wsum[kCullBS / 64];
+    constexpr int kIdx = (1 << 29) - 1, kS = 1 << 29, kB = 1 << 30;
+    const CD3 c{S.c[0], S.c[1], S.c[2]}, hu{S.hu[0], S.hu[1], S.hu[2]}, hn{S.hn[0], S.hn[1], S.hn[2]},
+        hv{S.hv[0], S.hv[1], S.hv[2]}, nrm{S.n[0], S.n[1], S.n[2]};
+    const CD3 ash = sunf.identity ? CD3{0, 1, 0} : CD3{sunf.nx, sunf.ny, sunf.nz};
+    CD3 r0 = cadd(cmul(ash, -1.0), cmul(nrm, 2.0 * cdot(ash, nrm)));  // reflect(-sun axis, n)
+    r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
+    const double rs = sqrt(cdot(hu, hu) + cdot(hn, hn) + cdot(hv, hv));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int g0 = 0; g0 < n_groups; g0 += kScanRound) {
+        // the counts were last written before a barrier: every lane reads the same values
+        const bool want_s = tan_sun >= 0.0 && n[0] <= cap;
+        const bool want_b = tan_block >= 0.0 && n[1] <= cap;
+        if (!want_s && !want_b) break;
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        const int g1 = min(n_groups, g0 + kScanRound);
+        for (int gi = g0 + (int)threadIdx.x; gi < g1; gi += kCullBS) {
+            const QuadGroup G = groups[gi];
+            if (G.begin >= own_lo && G.begin + G.count <= own_hi) continue;
+            const CD3 Dg = csub(CD3{G.c[0], G.c[1], G.c[2]}, c);
+            const bool gs = want_s && sphere_may_hit(Dg, rs + G.r, ash, tan_sun, margin);
+            const bool gb = want_b && sphere_may_hit(Dg, rs + G.r, r0, tan_block, margin);
+            if (gs || gb) s_sel[atomicAdd(&s_cnt, 1)] = gi | (gs ? kS : 0) | (gb ? kB : 0);
+        }
+        __syncthreads();
+        const int cnt = s_cnt;
+        // prefix sums of the selected groups' quad counts: four entries per lane, then across lanes
+        int loc[4], run = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = 4 * (int)threadIdx.x + k;
+            run += e < cnt ? groups[s_sel[e] & kIdx].count : 0;
+            loc[k] = run;
+        }
+        int incl = run;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        int base = incl - run;
+        for (int w = 0; w < wave; ++w) base += s_wsum[w];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = 4 * (int)threadIdx.x + k;
+            if (e < cnt) s_end[e] = base + loc[k];
+        }
+        __syncthreads();
+        const int total = cnt ? s_end[cnt - 1] : 0;
+        for (int t = (int)threadIdx.x; t < total; t += kCullBS) {
+            // the selected group holding pair t: the first entry whose prefix sum exceeds t
+            int lo = 0, hi = cnt - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (s_end[mid] > t) hi = mid;
+                else lo = mid + 1;
+            }
+            const int sel = s_sel[lo];
+            const QuadGroup &G = groups[sel & kIdx];
+            const int q = G.begin + t - (s_end[lo] - G.count);
+            if (q >= own_lo && q < own_hi) continue;
+            const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
+            const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
+            const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
+            const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
+            const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
+            if ((sel & kS) && sphere_may_hit(D, R, ash, tan_sun, margin) &&
+                cone_may_hit(c, hu, hn, hv, ash, tan_sun, A, E1, E2, margin)) {
+                const int k = atomicAdd(n, 1);
+                if (k < cap) ids[0][k] = q;
+            }
+            if ((sel & kB) && sphere_may_hit(D, R, r0, tan_block, margin) &&
+                cone_may_hit(c, hu, hn, hv, r0, tan_block, A, E1, E2, margin)) {
+                const int k = atomicAdd(n + 1, 1);
+                if (k < cap) ids[1][k] = q;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Per heliostat of the launch: whether any ray of its shadow cone (directions within the sun-shape
 // pool's half-angle of the sun axis) or of its blocking cone (within sun shape + twice the slope
 // error of the ideal reflection) can meet another quad of the grid (cone_may_hit, conservative).
 // When none can, every test of that traversal is a miss, so skipping it returns the same result for
 // every ray. Candidates are listed up to kCullList (CullList) or, when a.wide is set, up to kCullWide
 // (WideList, for the per-wave masks of the plain trace).
-__global__ __launch_bounds__(256) void k_cull(CullArgs a) {
+__global__ __launch_bounds__(kCullBS) void k_cull(CullArgs a) {
     __shared__ int s_n[2];
     __shared__ int s_ids[2][kCullWide];
     const int h = a.list[blockIdx.x];
@@ -36,8 +140,8 @@ __global__ __launch_bounds__(256) void k_cull(CullArgs a) {
     const double tan_block = (ang_block >= 0.0 && ang_block < kCullMaxAngle) ? tan(ang_block) : -1.0;
     if (threadIdx.x < 2) s_n[threadIdx.x] = 0;
     __syncthreads();
-    cull_collect(S, a.sun, a.quads, a.groups, a.n_groups, H.sub_begin, H.sub_begin + H.sub_count, a.tan_sun,
-                 tan_block, a.margin, threadIdx.x, blockDim.x, a.wide ? kCullWide : kCullList, s_n, s_ids);
+    cull_scan(S, a.sun, a.quads, a.groups, a.n_groups, H.sub_begin, H.sub_begin + H.sub_count, a.tan_sun, tan_block,
+              a.margin, a.wide ? kCullWide : kCullList, s_n, s_ids);
     __syncthreads();
     if (a.wide) {
         WideList *W = a.wide + h;
@@ -334,7 +438,8 @@ qmc_status qmc::launch_cull_into(qmc_ctx *c, int grid, const int32_t *d_list, in
     ca.margin = kCullMargin;
     ca.lists = lists;
     ca.wide = wide;
-    hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(256), 0, c->stream, ca);
+    if (gb.n_groups >= (1 << 29)) return fail(QMC_E_UNSUPPORTED, "trace: 2^29 or more culling groups in a grid");
+    hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(kCullBS), 0, c->stream, ca);
     HIP_TRY(hipGetLastError());
     return QMC_OK;
 }

@@ -215,59 +215,6 @@ __host__ __device__ inline DevCullShape flat_cull_shape(const DevHelio &H, const
     return s;
 }
 
-// One thread's share of the traversal-culling scan of a heliostat (culling shape S, own quads
-// [own_lo, own_hi)) over the quad groups g = first, first + step, ... of a grid: every quad that passes
-// the bound (sphere_may_hit, then cone_may_hit) for the shadow cone (kind 0: tan_sun around the sun
-// axis, the n axis of sunf) or the blocking cone (kind 1: tan_block around the sun axis reflected in
-// the heliostat normal) is counted in n[kind] and, while the count is at most cap (<= kCullWide),
-// listed in ids[kind]; a negative tangent skips that kind. A group's quads are tested only when the
-// group's ball passes sphere_may_hit: the test is monotone in the ball (a ball inside one that fails
-// fails too), so no quad that would pass is skipped. n and ids are shared by the workgroup (LDS, n
-// zeroed beforehand). A kind stops being scanned once its count exceeds cap (it is then traversed for
-// every ray and its list is not used), so a final count <= cap is complete.
-__device__ __forceinline__ void cull_collect(const DevCullShape &S, const DevFrame &sunf, const float4 *quads,
-                                             const QuadGroup *groups, int n_groups, int own_lo, int own_hi,
-                                             double tan_sun, double tan_block, double margin, int first, int step,
-                                             int cap, int *n, int (*ids)[kCullWide]) {
-    const CD3 c{S.c[0], S.c[1], S.c[2]}, hu{S.hu[0], S.hu[1], S.hu[2]}, hn{S.hn[0], S.hn[1], S.hn[2]},
-        hv{S.hv[0], S.hv[1], S.hv[2]}, nrm{S.n[0], S.n[1], S.n[2]};
-    const CD3 ash = sunf.identity ? CD3{0, 1, 0} : CD3{sunf.nx, sunf.ny, sunf.nz};
-    CD3 r0 = cadd(cmul(ash, -1.0), cmul(nrm, 2.0 * cdot(ash, nrm)));  // reflect(-sun axis, n)
-    r0 = cmul(r0, 1.0 / sqrt(cdot(r0, r0)));
-    const double rs = sqrt(cdot(hu, hu) + cdot(hn, hn) + cdot(hv, hv));
-    for (int gi = first; gi < n_groups; gi += step) {
-        const QuadGroup G = groups[gi];
-        if (G.begin >= own_lo && G.begin + G.count <= own_hi) continue;
-        const bool want_s = tan_sun >= 0.0 &&
-                            __hip_atomic_load(n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= cap;
-        const bool want_b = tan_block >= 0.0 &&
-                            __hip_atomic_load(n + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= cap;
-        if (!want_s && !want_b) break;
-        const CD3 Dg = csub(CD3{G.c[0], G.c[1], G.c[2]}, c);
-        const bool gs = want_s && sphere_may_hit(Dg, rs + G.r, ash, tan_sun, margin);
-        const bool gb = want_b && sphere_may_hit(Dg, rs + G.r, r0, tan_block, margin);
-        if (!gs && !gb) continue;
-        for (int q = G.begin; q < G.begin + G.count; ++q) {
-            if (q >= own_lo && q < own_hi) continue;
-            const float4 qa = quads[3 * q], qb = quads[3 * q + 1], qc = quads[3 * q + 2];
-            const CD3 A{qa.x, qa.y, qa.z}, E1{qb.x, qb.y, qb.z}, E2{qc.x, qc.y, qc.z};
-            const CD3 sum = cadd(E1, E2), diff = csub(E1, E2);
-            const CD3 D = csub(cadd(A, cmul(sum, 0.5)), c);
-            const double R = rs + 0.5 * sqrt(fmax(cdot(sum, sum), cdot(diff, diff)));
-            if (gs && sphere_may_hit(D, R, ash, tan_sun, margin) &&
-                cone_may_hit(c, hu, hn, hv, ash, tan_sun, A, E1, E2, margin)) {
-                const int k = atomicAdd(n, 1);
-                if (k < cap) ids[0][k] = q;
-            }
-            if (gb && sphere_may_hit(D, R, r0, tan_block, margin) &&
-                cone_may_hit(c, hu, hn, hv, r0, tan_block, A, E1, E2, margin)) {
-                const int k = atomicAdd(n + 1, 1);
-                if (k < cap) ids[1][k] = q;
-            }
-        }
-    }
-}
-
 // Load of data that no kernel of the launch writes (per-heliostat and per-instance records) through
 // the constant address space: with a wave-uniform address it becomes a scalar load into SGPRs.
 template <class T>

@@ -1,4 +1,4 @@
-// Check of the grouped culling scan's premise (cull_collect in qmc_trace_core.h): sphere_may_hit is
+// Check of the grouped culling scan's premise (cull_scan in qmc_cull.hip): sphere_may_hit is
 // monotone in its ball, so a group ball that fails the test proves that every ball inside it fails
 // too. Random cones (axis, half-angle, margin) and random groups of 1-28 balls, each group ball the
 // tightest one about the group's box centre holding them (as the upload builds it) or a looser one;
