@@ -311,7 +311,11 @@ qmc_status qmc_facets_get(qmc_ctx *ctx, int32_t heliostat, float *origins, float
  * QuasiMonteCarloRayTracer::rayTracing, QuasiMonteCarloRayTracer.cpp:15-91, and the launchers
  * RectangleReceiverRectangleGridRayTracing / CylinderReceiverRectangleGridRayTracing,
  * RectangleReceiverRectangleGridRayTracing.cu:45-57, CylinderReceiverRectangleGridRayTracing.cu:44-57.
- * Heliostats of several grids are traced grid by grid. Asynchronous. */
+ * Heliostats of several grids are traced grid by grid; a heliostat listed twice is traced twice.
+ * Group offsets: the reference draws fresh ones for its heliostat on every rayTracing call
+ * (QuasiMonteCarloRayTracer.cpp:135-161); here that needs qmc_offsets_resample(ctx, 1, seed). With it
+ * off (the default) every trace reuses the offsets generated or set before, so tracing a heliostat
+ * twice repeats the same rays. Asynchronous. */
 qmc_status qmc_trace(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n);
 /* The same trace as qmc_trace, allocating no device memory. No counterpart in the reference: for callers
  * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
