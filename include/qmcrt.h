@@ -325,12 +325,30 @@ qmc_status qmc_trace(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n);
  * scene's candidate lists (576 B per heliostat) plus one band per CU must fit (QMC_E_RANGE otherwise).
  * Calls that overlap on one device (contexts on different streams) share the array and race. */
 qmc_status qmc_trace_noalloc(qmc_ctx *ctx, const int32_t *heliostat_ids, int32_t n);
+/* The map path (qmc_trace, qmc_trace_noalloc, and records with QMC_KERNEL_MAP_DECISIONS) decides each
+ * shadow and blocking traversal from the ray's tests of its wave's candidates and a walk-free proof;
+ * the traversals these leave open (none to a few per 10^4 rays on the fields measured) defer their ray
+ * to an exact re-trace after the launch, through a queue. No counterpart in the reference.
+ * qmc_set_defer_capacity caps the queue at `rays` entries per launch (0: sized from the launch, one
+ * entry per 256 rays and at least 65536; qmc_trace_noalloc holds at most 2^20). A launch that defers
+ * more than its queue holds re-traces every ray of the launch instead and outputs those it deferred:
+ * the same results, slower. qmc_get_defer_count waits for the context's stream and returns how many
+ * rays the last such launch deferred. */
+qmc_status qmc_set_defer_capacity(qmc_ctx *ctx, int64_t rays);
+qmc_status qmc_get_defer_count(qmc_ctx *ctx, int64_t *rays);
 /* Which trace kernel a record trace runs (qmc_trace_records_many): AUTO = the one qmc_trace would run
  * for the same list, PER_RAY = one lane per ray (k_trace), BANDED = the persistent LDS-banded kernel
  * (k_trace_band), whatever the list's size. */
 #define QMC_KERNEL_AUTO    0
 #define QMC_KERNEL_PER_RAY 1
 #define QMC_KERNEL_BANDED  2
+/* Or-ed into the kernel choice: records of the map path's own decisions (the path qmc_trace runs).
+ * Without it a shadowed or blocked record names the traversal's first hit, as the reference's
+ * collision test returns it (rectGridDDA.cu:31-119), and the lookup walk that finds it always runs;
+ * with it, a walk proven to reach the cells of a quad the ray hits is skipped, and the record names
+ * that quad (the same status, pixel and energy; another hit quad than the first only when the ray
+ * hits two). */
+#define QMC_KERNEL_MAP_DECISIONS 4
 /* Rates (G operations/s) of the atomic adds the tracer deposits with, measured on the ctx's device:
  * float and 64-bit device-scope adds scattered over a 65,536-entry map, float LDS adds scattered over
  * a 64 KB tile. A capability probe (about 0.1 s) to state the trace's deposit rates against; no

@@ -508,7 +508,8 @@ bool intersect(f3 o, f3 d, const std::vector<f3> &V, const std::vector<int> &mat
     return false;
 }
 
-bool collision(f3 o, f3 d, const Grid &g, int own, int nown, int &hit, Counters *cnt) {
+bool collision(f3 o, f3 d, const Grid &g, int own, int nown, int &hit, Counters *cnt,
+               std::vector<int> *visited = nullptr) {
     f3 gp = load3(g.d.pos), gi = load3(g.d.interval);
     f3 q = divv(sub(o, gp), gi);
     int px = int(q.x), py = int(q.y), pz = int(q.z);
@@ -519,6 +520,7 @@ bool collision(f3 o, f3 d, const Grid &g, int own, int nown, int &hit, Counters 
     float tDx = absDivide(gi.x, d.x), tDy = absDivide(gi.y, d.y), tDz = absDivide(gi.z, d.z);
     while (px >= 0 && px < g.nx && py >= 0 && py < g.ny && pz >= 0 && pz < g.nz) {
         int addr = px * g.ny * g.nz + py * g.nz + pz;
+        if (visited) visited->push_back(addr);
         if (cnt) {
             cnt->cells++;
             cnt->candidates += (uint64_t)(g.index[addr + 1] - g.index[addr]);
@@ -1175,6 +1177,16 @@ extern "C" int orc_dda(const orc_scene *s, int g, const float o[3], const float 
     bool r = collision(load3(o), load3(d), s->G[g], own_lo, own_hi - own_lo, hh, nullptr);
     if (hit) *hit = hh;
     return r ? 1 : 0;
+}
+
+extern "C" int orc_dda_cells(const orc_scene *s, int g, const float o[3], const float d[3], int32_t *cells, int cap) {
+    const Grid &gg = s->G[g];
+    std::vector<int> visited;
+    int hh = -1;
+    collision(load3(o), load3(d), gg, 0, (int)gg.verts.size() / 3, hh, nullptr, &visited);
+    const int n = (int)visited.size();
+    for (int i = 0; i < n && i < cap; ++i) cells[i] = visited[(size_t)i];
+    return n;
 }
 
 extern "C" int orc_brute_any_hit(const orc_scene *s, int g, const float o[3], const float d[3], int own_lo,

@@ -90,6 +90,9 @@ int orc_cylinder_resolution(const float size[3], int ppm, int res[2]);
 int orc_dda(const orc_scene *s, int g, const float o[3], const float d[3], int own_lo, int own_hi, int *hit);
 /* Brute force: does the ray hit any non-own quad of grid g with t >= eps (any order)? */
 int orc_brute_any_hit(const orc_scene *s, int g, const float o[3], const float d[3], int own_lo, int own_hi);
+/* The cells (x-major addresses) the traversal of orc_dda visits with no quad tested, in order, up to cap of
+ * them into cells; returns how many it visits. */
+int orc_dda_cells(const orc_scene *s, int g, const float o[3], const float d[3], int32_t *cells, int cap);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,7 @@ QMC_E_STATE = -8
 
 RAY_SHADOWED, RAY_BLOCKED, RAY_MISSED, RAY_DEPOSITED = 0, 1, 2, 3
 KERNEL_AUTO, KERNEL_PER_RAY, KERNEL_BANDED = 0, 1, 2
+KERNEL_MAP_DECISIONS = 4  # or-ed into a kernel choice: records of the map path's own decisions
 RAYF_EDGE_CLAMPED = 1
 
 
@@ -172,6 +173,8 @@ SIGNATURES = {
     "qmc_facets_get": [_P, _I, _FP, _FP, _I],
     "qmc_trace": [_P, _IP, _I],
     "qmc_trace_noalloc": [_P, _IP, _I],
+    "qmc_set_defer_capacity": [_P, C.c_int64],
+    "qmc_get_defer_count": [_P, C.POINTER(C.c_int64)],
     "qmc_cull_flags": [_P, _IP, _I, C.POINTER(C.c_uint8)],
     "qmc_probe_atomic_rates": [_P, C.POINTER(AtomicRates)],
     "qmc_trace_records": [_P, _I, C.POINTER(RayRecord), C.c_int64],

@@ -151,7 +151,8 @@ __device__ __forceinline__ void bilinear_splat_f64(const AdjArgs &a, double uu, 
 
 // Visibility of ray r (slot `slot` of list entry li, heliostat h) by the plain trace's float rules
 // (RectangleReceiverRectangleGridRayTracing.cu:14-40) with its culling: traversals proved hit-free
-// are skipped, the others walk only the cells of the masked candidates the ray hits. True when the
+// are skipped, the others walk only the cells of the masked candidates the ray hits, unless the walk
+// is proven to reach a hit quad's cells (the outcome, not which quad, matters here). True when the
 // ray reaches the receiver, with its micro-facet m and pool samples s (sun shape) and q (slope error).
 __device__ __forceinline__ bool adj_visible(const AdjArgs &a, const DevHelio &H, int h, int li, long long slot,
                                             long long r, int &m, float4 &s, float4 &q) {
@@ -171,14 +172,16 @@ __device__ __forceinline__ bool adj_visible(const AdjArgs &a, const DevHelio &H,
     int hit;
     const int lo = H.sub_begin, hi = H.sub_begin + H.sub_count;
     int plan = traversal_plan(wl, wm, 0, origin, dir, a.grid, rh);
-    if (plan != 0 && (plan == 2 ? dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st)
-                                : dda_known<false>(rh, origin, dir, a.grid, lo, hi, hit, st)))
+    if (plan != 0 && ((plan == 1 && known_hit_proven(rh, origin, dir, a.grid, hit)) ||
+                      (plan == 1 && rh.n <= 2 ? dda_known<false>(rh, origin, dir, a.grid, lo, hi, hit, st)
+                                              : dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))))
         return false;
     const V3 normal = normalize(fr_apply(H.fr, V3{q.x, q.y, q.z}));
     dir = normalize(reflect(-dir, normal));
     plan = traversal_plan(wl, wm, 1, origin, dir, a.grid, rh);
-    if (plan != 0 && (plan == 2 ? dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st)
-                                : dda_known<false>(rh, origin, dir, a.grid, lo, hi, hit, st)))
+    if (plan != 0 && ((plan == 1 && known_hit_proven(rh, origin, dir, a.grid, hit)) ||
+                      (plan == 1 && rh.n <= 2 ? dda_known<false>(rh, origin, dir, a.grid, lo, hi, hit, st)
+                                              : dda_collision<false>(origin, dir, a.grid, a.grid.quads, lo, hi, hit, st))))
         return false;
     float t, u, v;
     return receiver_hit<0>(a.recv, origin, dir, t, u, v);

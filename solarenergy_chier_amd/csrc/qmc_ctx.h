@@ -19,10 +19,13 @@ struct GridBuf {
     int32_t *match = nullptr;
     float4 *quads = nullptr;
     int4 *boxes = nullptr;        // cells listing each sub-heliostat: lo, hi
+    ProofFrame *frame = nullptr;  // the grid's frame for the walk-free proof
+    float4 *pbox = nullptr;       // each sub-heliostat's cell box for the proof: mid, lim
     QuadGroup *groups = nullptr;  // culling scan groups of the quads
     int n_sub = 0;
     int n_groups = 0;
     int receiver = 0;
+    int htype = 0;                // heliostat type of every heliostat of the grid (0 flat, 1 focus-flat)
 };
 
 // One launch's share of a trace call: list entries [begin, begin + count) of pass `pass`, all of grid
@@ -100,6 +103,11 @@ struct qmc_ctx {
     size_t wide_cap = 0;
     uint64_t *d_wmask = nullptr;      // per-wave candidate masks of the last plain trace launch
     size_t wmask_cap = 0;
+    unsigned long long *d_defer = nullptr;  // rays the map path defers to k_trace_deferred
+    size_t defer_cap = 0;
+    unsigned *d_defer_n = nullptr;
+    size_t defer_limit = 0;              // qmc_set_defer_capacity (0: the launch's own sizing)
+    const unsigned *last_defer_n = nullptr;  // the counter of the last deferring launch
     long long *d_rec_base = nullptr;  // first record of each list entry of a record trace
     size_t rec_base_cap = 0;
     // per-call resampling of the micro-facet group offsets: when on, every pass of a trace call first
@@ -160,7 +168,8 @@ void qmc_adjoint_release(qmc_ctx *c);  // qmc_adjoint.hip
 // Traversal culling (qmc_cull.hip), on the ctx stream:
 // candidate lists (CullList, and WideList when `wide` is set; indexed by heliostat id) of the listed
 // heliostats of a grid, into the given buffers (one entry per heliostat id; no allocation);
-qmc_status launch_cull_into(qmc_ctx *c, int grid, const int32_t *d_list, int n, CullList *lists, WideList *wide);
+qmc_status launch_cull_into(qmc_ctx *c, int grid, const int32_t *d_list, int n, CullList *lists, WideList *wide,
+                            unsigned *zero = nullptr);
 // the same into the ctx-owned buffers, grown to the scene's heliostat count first;
 qmc_status launch_cull(qmc_ctx *c, int grid, const int32_t *d_list, int n, const CullList **lists,
                        const WideList **wide);
@@ -183,6 +192,9 @@ struct TraceBufs {
     size_t wmask_cap = 0;     // uint64 entries
     float *band = nullptr;
     size_t band_cap = 0;      // floats
+    unsigned long long *defer = nullptr;  // deferred rays of the map path (k_trace_deferred) and their count
+    unsigned *defer_n = nullptr;
+    size_t defer_cap = 0;     // entries (0: none; the trace kernels then walk every traversal themselves)
     bool fixed = false;
 };
 // A plain trace (as qmc_trace) into fixed buffers: allocates no device memory.
@@ -197,6 +209,8 @@ inline void free_scene(qmc_ctx *c) {
         dev_free(g.match);
         dev_free(g.quads);
         dev_free(g.boxes);
+        dev_free(g.frame);
+        dev_free(g.pbox);
         dev_free(g.groups);
     }
     c->grids.clear();

@@ -135,6 +135,7 @@ __global__ __launch_bounds__(kCullBS) void k_cull(CullArgs a) {
     const int h = a.list[blockIdx.x];
     const DevHelio &H = a.helios[h];
     const DevCullShape S = a.shapes[h];
+    if (a.zero && blockIdx.x == 0 && threadIdx.x == 0) *a.zero = 0u;
     // canted facets turn reflections further from the ideal one: twice the canting angle
     const double ang_block = a.ang_block < 0.0 ? -1.0 : a.ang_block + 2.0 * S.cant;
     const double tan_block = (ang_block >= 0.0 && ang_block < kCullMaxAngle) ? tan(ang_block) : -1.0;
@@ -422,8 +423,9 @@ qmc_status qmc::upload_groups(qmc_ctx *c, int g, const std::vector<float4> &quad
 
 
 // Traversal-culling candidate lists (k_cull) of the listed heliostats of a grid, on the ctx stream,
-// into caller-provided per-heliostat arrays.
-qmc_status qmc::launch_cull_into(qmc_ctx *c, int grid, const int32_t *d_list, int n, CullList *lists, WideList *wide) {
+// into caller-provided per-heliostat arrays; `zero` (nullable) is set to 0 on the way.
+qmc_status qmc::launch_cull_into(qmc_ctx *c, int grid, const int32_t *d_list, int n, CullList *lists, WideList *wide,
+                                 unsigned *zero) {
     const GridBuf &gb = c->grids[grid];
     CullArgs ca;
     ca.helios = c->d_helios;
@@ -438,6 +440,7 @@ qmc_status qmc::launch_cull_into(qmc_ctx *c, int grid, const int32_t *d_list, in
     ca.margin = kCullMargin;
     ca.lists = lists;
     ca.wide = wide;
+    ca.zero = zero;
     if (gb.n_groups >= (1 << 29)) return fail(QMC_E_UNSUPPORTED, "trace: 2^29 or more culling groups in a grid");
     hipLaunchKernelGGL(k_cull, dim3((unsigned)n), dim3(kCullBS), 0, c->stream, ca);
     HIP_TRY(hipGetLastError());

@@ -75,6 +75,8 @@ struct DevGrid {
     const int32_t *match;
     const float4 *quads;
     const int4 *boxes;         // per sub-heliostat id: the cells listing it, (x0, y0, z0) then (x1, y1, z1)
+    const ProofFrame *frame;   // the grid's frame for the walk-free proof (dda_visit_proof), in HBM
+    const float4 *pbox;        // per sub-heliostat id: its cell box as the proof reads it, mid then lim
 };
 
 // Buie sun-shape table on the device (Scene/SceneProcessor.cu:23-44,237-239)
@@ -143,6 +145,7 @@ struct CullArgs {
     double margin;
     CullList *lists;      // per heliostat id
     WideList *wide;       // per heliostat id (nullable: collect for `lists` only)
+    unsigned *zero;       // nullable: a counter the launch's trace kernel starts from, set to 0 here
 };
 
 // Per-wave candidate masks of the plain trace: wave w of list entry li (slots [64w, 64w + 64) of
@@ -184,6 +187,12 @@ struct TraceArgs {
     FastDiv fdP, fdL;
     int32_t n_list;
     int32_t blocks_per_helio;  // grid.x extent per heliostat
+    // map path: rays whose traversal outcome the walk-free proof leaves open, as (list entry << 32) |
+    // slot, for the deferred re-trace (k_trace_deferred); defer_n counts them (zeroed by k_cull before
+    // the launch). defer_cap = 0: no queue, such rays run the full traversal in place.
+    unsigned long long *defer;
+    unsigned *defer_n;
+    int32_t defer_cap;
 };
 
 }  // namespace qmc

@@ -41,13 +41,52 @@ struct RayOut {
     int status, index, flags, row, col;
     float energy;
     bool unshadowed, unblocked, first_of_facet;
+    bool replayed;  // kTmReplay: a traversal that the deferring pass left open was decided here
 };
 
+// A ray the map path leaves to the deferred re-trace (RayOut.status only; never written to a record).
+constexpr int kRayDeferred = -1;
+
+// How trace_ray decides a traversal (plan from traversal_plan):
+//   kTmExact  — the traversal's own walk: the lookup walk (dda_known) for plan 1 with at most two hits,
+//               the full walk (dda_collision) otherwise; a shadowed or blocked record names the first
+//               hit.
+//   kTmDefer  — the map path with culling: the plan tests every masked candidate; plan 0 has no hit;
+//               plan 1 is decided by the walk-free proof (known_hit_proven) where it holds; every other
+//               traversal is deferred: the ray is appended to the launch's queue (while it has room)
+//               and k_trace_deferred traces it again. The kernel carries no walk at all.
+//   kTmReplay — k_trace_deferred: the same decisions as kTmDefer, with each traversal kTmDefer would
+//               defer decided by its own walk instead (and RayOut.replayed set).
+// Statuses, pixels and energies are the same in every mode.
+constexpr int kTmExact = 0, kTmDefer = 1, kTmReplay = 2;
+
+// A traversal's outcome: 0 no hit, 1 hit (hit set), 2 deferred.
+template <bool STATS, int TM>
+__device__ __forceinline__ int traversal_outcome(const TraceArgs &a, int plan, const RayHits &rh, V3 o, V3 d,
+                                                 int own_lo, int own_hi, int li, long long slot, int &hit,
+                                                 bool &replayed, RayStats &st) {
+    if (plan == 0) return 0;
+    if (TM != kTmExact && plan == 1 && known_hit_proven(rh, o, d, a.grid, hit)) return 1;
+    if (TM == kTmDefer) {
+        const unsigned k = atomicAdd(a.defer_n, 1u);
+        if (k < (unsigned)a.defer_cap) a.defer[k] = ((unsigned long long)(unsigned)li << 32) | (unsigned long long)slot;
+        return 2;
+    }
+    if (TM == kTmReplay) replayed = true;
+    if (plan == 1 && rh.n <= 2) return dda_known<STATS>(rh, o, d, a.grid, own_lo, own_hi, hit, st) ? 1 : 0;
+    return dda_collision<STATS>(o, d, a.grid, a.grid.quads, own_lo, own_hi, hit, st) ? 1 : 0;
+}
+
 // One ray of map_raytracing (RectangleReceiverRectangleGridRayTracing.cu:6-41,
-// CylinderReceiverRectangleGridRayTracing.cu:6-40) up to, not including, its deposit.
-template <int RT, bool STATS>
+// CylinderReceiverRectangleGridRayTracing.cu:6-40) up to, not including, its deposit. The ray is ray id
+// r of list entry li, held in slot `slot`. TM: how traversals are decided (kTmExact, kTmDefer,
+// kTmReplay); a deferred ray gets status kRayDeferred (no deposit, no record). HT: the heliostats'
+// type (that of the grid). U: the wave's lanes share the heliostat and the wave's candidate masks
+// (scalar loads); off in k_trace_deferred.
+template <int RT, bool STATS, int TM, int HT, bool U = true>
 __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H, const WideList *wl,
-                                          const uint64_t *wm, long long r, RayOut &o, RayStats &st) {
+                                          const uint64_t *wm, long long r, int li, long long slot, RayOut &o,
+                                          RayStats &st) {
     const int L = a.L;
     const int m = (int)fast_div((uint32_t)r, a.fdL);
     const int j = (int)r - m * L;
@@ -60,6 +99,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     o.row = o.col = -1;
     o.unshadowed = o.unblocked = false;
     o.first_of_facet = j == 0;
+    o.replayed = false;
 
     // Step 1: incident direction and shadowing (RectangleReceiverRectangleGridRayTracing.cu:14-22)
     // both pool indices and samples are read before any test, so the four loads form two
@@ -71,16 +111,19 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
     V3 origin, fnormal;
     DevFrame pf;
-    facet_geometry(H, a.facets, m, origin, pf, fnormal);
+    facet_geometry<HT>(H, a.facets, m, origin, pf, fnormal);
     int hit = -1;
     // traversals proved hit-free for this ray are skipped, and the others look up the hits that the
     // ray's candidate tests found (never while counting work)
     const bool culling = !STATS && wl;
     RayHits rh;
-    int plan = culling ? traversal_plan(wl, wm, 0, origin, dir, a.grid, rh) : 2;
-    if (plan != 0 &&
-        (plan == 2 ? dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)
-                   : dda_known<STATS>(rh, origin, dir, a.grid, own_lo, own_hi, hit, st))) {
+    int plan = culling ? traversal_plan<U, TM != kTmExact>(wl, wm, 0, origin, dir, a.grid, rh) : 2;
+    int out = traversal_outcome<STATS, TM>(a, plan, rh, origin, dir, own_lo, own_hi, li, slot, hit, o.replayed, st);
+    if (out == 2) {
+        o.status = kRayDeferred;
+        return;
+    }
+    if (out) {
         o.status = QMC_RAY_SHADOWED;
         o.index = hit;
         return;
@@ -89,10 +132,13 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     // Step 2: perturbed facet normal, reflection, blocking (:24-36)
     V3 normal = normalize(fr_apply(pf, V3{q.x, q.y, q.z}));
     dir = normalize(reflect(-dir, normal));
-    plan = culling ? traversal_plan(wl, wm, 1, origin, dir, a.grid, rh) : 2;
-    if (plan != 0 &&
-        (plan == 2 ? dda_collision<STATS>(origin, dir, a.grid, a.grid.quads, own_lo, own_hi, hit, st)
-                   : dda_known<STATS>(rh, origin, dir, a.grid, own_lo, own_hi, hit, st))) {
+    plan = culling ? traversal_plan<U, TM != kTmExact>(wl, wm, 1, origin, dir, a.grid, rh) : 2;
+    out = traversal_outcome<STATS, TM>(a, plan, rh, origin, dir, own_lo, own_hi, li, slot, hit, o.replayed, st);
+    if (out == 2) {
+        o.status = kRayDeferred;
+        return;
+    }
+    if (out) {
         o.status = QMC_RAY_BLOCKED;
         o.index = hit;
         return;
@@ -121,11 +167,16 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
 // tens of thousands of nearly equal deposits, and one float accumulator would round each against a
 // sum thousands of times larger (a bias near 3e-4 on the reference's example inputs); 64-bit atomics
 // run at the same rate as 32-bit ones on MI355X.
-template <int RT, bool REC, bool STATS>
+// REC: 0 = no records (the map path), 1 = records naming each traversal's first hit, 2 = records of
+// the map path's own decisions (a shadowed or blocked record may name another hit quad). DEFER: the map
+// path with culling (kTmDefer), else every traversal walks (kTmExact).
+template <int RT, int REC, bool STATS, int HT, bool DEFER>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
+    static_assert(!DEFER || (!STATS && REC != 1), "deferral is for the map path");
     const int h = a.list[blockIdx.y];
     const DevHelio &H = a.helios[h];
-    const long long r = slot_ray(H, a.L, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+    const long long slot = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long r = slot_ray<HT>(H, a.L, slot);
 
     __shared__ unsigned long long s_acc[9];
     if (STATS) {
@@ -140,9 +191,10 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         // the wave's candidate masks (slots of a wave are consecutive and 64-aligned)
         const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
         const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)blockIdx.y * a.wstride + wave) : nullptr;
-        trace_ray<RT, STATS>(a, H, a.wide ? a.wide + h : nullptr, wm, r, o, st);
+        trace_ray<RT, STATS, DEFER ? kTmDefer : kTmExact, HT>(a, H, a.wide ? a.wide + h : nullptr, wm, r,
+                                                             (int)blockIdx.y, slot, o, st);
         if (o.status == QMC_RAY_DEPOSITED) unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
-        if (REC) {
+        if (REC && o.status != kRayDeferred) {
             qmc_ray_record rec;
             rec.status = o.status;
             rec.index = o.index;
@@ -180,10 +232,11 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
 // (heliostat, block) round robin; at the end each stores its band, with plain stores, to its own
 // slice of `partials`, which k_band_fold adds into the accumulator. With REC every ray also writes its
 // record at records[rec_base[li] + ray id] (parity checks of this kernel's own slot, wave-mask and
-// band indexing).
-template <int RT, int BS, bool REC>
+// band indexing). DEFER as for k_trace.
+template <int RT, int BS, int REC, int HT, bool DEFER>
 __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int rows, int blocks_per_helio,
                                                     float *partials) {
+    static_assert(!DEFER || REC != 1, "deferral is for the map path");
     extern __shared__ float band[];
     const int W = a.recv.res_x, nband = rows * W;
     for (int p = threadIdx.x; p < nband; p += BS) band[p] = 0.0f;
@@ -198,12 +251,14 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
         // the launch writes none of these records, so they are read with scalar loads
         const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
         const DevHelio H = ld_uniform(a.helios + h);
-        const long long r = slot_ray(H, a.L, (long long)__builtin_amdgcn_readfirstlane(bx) * BS + threadIdx.x);
+        const long long slot = (long long)__builtin_amdgcn_readfirstlane(bx) * BS + threadIdx.x;
+        const long long r = slot_ray<HT>(H, a.L, slot);
         if (r >= 0) {
             RayOut o;
             const int wave = __builtin_amdgcn_readfirstlane(bx * (BS / 64) + (int)(threadIdx.x >> 6));
             const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
-            trace_ray<RT, false>(a, H, a.wide ? a.wide + h : nullptr, wm, r, o, st);
+            trace_ray<RT, false, DEFER ? kTmDefer : kTmExact, HT>(a, H, a.wide ? a.wide + h : nullptr, wm, r, li,
+                                                                  slot, o, st);
             if (o.status == QMC_RAY_DEPOSITED) {
                 const int br = o.row - row0;
                 if ((unsigned)br < (unsigned)rows)
@@ -211,7 +266,7 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                 else
                     unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
             }
-            if (REC) {
+            if (REC && o.status != kRayDeferred) {
                 qmc_ray_record rec;
                 rec.status = o.status;
                 rec.index = o.index;
@@ -242,6 +297,56 @@ __global__ __launch_bounds__(256) void k_band_fold(double *acc, const float *par
     double s = 0.0;
     for (int w = w0; w < w1; ++w) s += (double)partials[(size_t)w * nband + p];
     if (s != 0.0) unsafeAtomicAdd(acc + offset + p, s);
+}
+
+// One ray that the deferring pass (kTmDefer) left to k_trace_deferred, replayed (kTmReplay) with
+// per-lane loads; its deposit goes to the launch's float64 accumulator and, with REC, its record is
+// written. `all`: the ray is one of every slot of the launch, and only a ray that the deferring pass
+// left open (RayOut.replayed) is output.
+template <int RT, int REC, int HT>
+__device__ __forceinline__ void replay_ray(const TraceArgs &a, int li, long long slot, bool all) {
+    const int h = a.list[li];
+    const DevHelio &H = a.helios[h];
+    const long long r = slot_ray<HT>(H, a.L, slot);
+    if (r < 0) return;
+    const uint64_t *wm = a.wmask + 2 * ((size_t)li * a.wstride + (size_t)(slot >> 6));
+    RayOut o;
+    RayStats st{0u, 0u, 0u};
+    trace_ray<RT, false, kTmReplay, HT, false>(a, H, a.wide + h, wm, r, li, slot, o, st);
+    if (all && !o.replayed) return;
+    if (o.status == QMC_RAY_DEPOSITED) unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
+    if (REC) {
+        qmc_ray_record rec;
+        rec.status = o.status;
+        rec.index = o.index;
+        rec.energy = o.energy;
+        rec.flags = o.flags;
+        a.records[a.rec_base[li] + r] = rec;
+    }
+}
+
+// The rays the map path deferred, traced again; runs after the trace kernel of the same launch. A
+// grid-stride loop over the queue's entries (list entry, slot) — or, when more rays were deferred than
+// the queue holds, over every slot of the launch (n_list entries of wstride * 64 slots), each ray
+// replayed and output only where the deferring pass left it open: slower, and exact the same way.
+template <int RT, int REC, int HT>
+__global__ __launch_bounds__(256) void k_trace_deferred(TraceArgs a) {
+    const unsigned n = *a.defer_n;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    const long long k0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n <= (unsigned)a.defer_cap) {
+        for (long long k = k0; k < (long long)n; k += stride) {
+            const unsigned long long key = a.defer[k];
+            replay_ray<RT, REC, HT>(a, (int)(key >> 32), (long long)(key & 0xffffffffull), false);
+        }
+    } else {
+        const long long per = (long long)a.wstride * 64;
+        const long long total = (long long)a.n_list * per;
+        for (long long k = k0; k < total; k += stride) {
+            const int li = (int)(k / per);
+            replay_ray<RT, REC, HT>(a, li, k - (long long)li * per, true);
+        }
+    }
 }
 
 // Adds a launch's accumulator into the float map (one rounding per pixel) and clears it.
@@ -462,6 +567,8 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     dev_free(c->d_cull_shapes);
     dev_free(c->d_wide);
     dev_free(c->d_wmask);
+    dev_free(c->d_defer);
+    dev_free(c->d_defer_n);
     dev_free(c->d_rec_base);
     dev_free(c->d_band_partials);
     dev_free(c->d_buie_phys);
@@ -680,16 +787,26 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         if (!cells.empty())
             HIP_TRY(hipMemcpyAsync(gb.cells, cells.data(), cells.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    c->stream));
+        const int dims[3] = {hg.nx, hg.ny, hg.nz};
         std::vector<int4> boxes(2 * (size_t)hg.n_sub);
+        std::vector<float4> pbox(2 * (size_t)hg.n_sub);
         for (int q = 0; q < hg.n_sub; ++q) {
             const int *b = &hg.sub_box[6 * (size_t)q];
             boxes[2 * (size_t)q] = make_int4(b[0], b[2], b[4], 0);
             boxes[2 * (size_t)q + 1] = make_int4(b[1], b[3], b[5], 0);
+            const int lo[3] = {b[0], b[2], b[4]}, hi[3] = {b[1], b[3], b[5]};
+            const ProofBox pb = proof_box(hg.d.interval, dims, lo, hi);
+            pbox[2 * (size_t)q] = make_float4(pb.mid[0], pb.mid[1], pb.mid[2], 0.0f);
+            pbox[2 * (size_t)q + 1] = make_float4(pb.lim[0], pb.lim[1], pb.lim[2], 0.0f);
         }
         if ((st = dev_alloc(gb.boxes, boxes.size()))) return st;
-        if (!boxes.empty())
+        if ((st = dev_alloc(gb.pbox, pbox.size()))) return st;
+        if (!boxes.empty()) {
             HIP_TRY(hipMemcpyAsync(gb.boxes, boxes.data(), boxes.size() * sizeof(int4), hipMemcpyHostToDevice,
                                    c->stream));
+            HIP_TRY(hipMemcpyAsync(gb.pbox, pbox.data(), pbox.size() * sizeof(float4), hipMemcpyHostToDevice,
+                                   c->stream));
+        }
         if ((st = dev_alloc(gb.cell_start, hg.cell_start.size()))) return st;
         if ((st = dev_alloc(gb.match, hg.match.size()))) return st;
         if ((st = dev_alloc(gb.quads, quads.size()))) return st;
@@ -710,13 +827,19 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         g.nx = hg.nx;
         g.ny = hg.ny;
         g.nz = hg.nz;
+        const ProofFrame fr = proof_frame(g.pos, g.interval, dims);
+        if ((st = dev_alloc(gb.frame, 1))) return st;
+        HIP_TRY(hipMemcpy(gb.frame, &fr, sizeof(fr), hipMemcpyHostToDevice));
+        g.frame = gb.frame;
         g.cells = gb.cells;
         g.cell_start = gb.cell_start;
         g.match = gb.match;
         g.quads = gb.quads;
         g.boxes = gb.boxes;
+        g.pbox = gb.pbox;
         gb.n_sub = hg.n_sub;
         gb.receiver = hg.d.receiver;
+        gb.htype = hg.d.heliostat_type;
     }
     // heliostats and focus-flat facets
     std::vector<DevFacet> facets;
@@ -1082,15 +1205,78 @@ namespace {
 
 constexpr int kBandBS = 1024;
 
-template <int RT, bool REC>
-void launch_band(dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph, float *parts) {
-    hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC>), g, dim3(kBandBS), lds, st, a, row0, rows, bph, parts);
+// The trace kernels for a receiver type RT, record mode REC, the grid's heliostat type ht and whether
+// the launch defers (a.defer_cap > 0: the map path with culling).
+template <int RT, int REC, bool DEFER>
+void launch_band_d(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph,
+                   float *parts) {
+    if (ht == 0)
+        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 0, DEFER>), g, dim3(kBandBS), lds, st, a, row0, rows, bph,
+                           parts);
+    else
+        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 1, DEFER>), g, dim3(kBandBS), lds, st, a, row0, rows, bph,
+                           parts);
 }
 
-template <bool REC>
-hipError_t band_occupancy(int rt, int &resident, size_t lds) {
-    return rt == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<0, kBandBS, REC>, kBandBS, lds)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<1, kBandBS, REC>, kBandBS, lds);
+template <int RT, int REC>
+void launch_band(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph,
+                 float *parts) {
+    if (REC != 1 && a.defer_cap > 0) launch_band_d<RT, REC, REC != 1>(ht, g, lds, st, a, row0, rows, bph, parts);
+    else launch_band_d<RT, REC, false>(ht, g, lds, st, a, row0, rows, bph, parts);
+}
+
+template <int RT, int REC>
+hipError_t band_occupancy_rt(int ht, bool defer, int &resident, size_t lds) {
+    constexpr bool D = REC != 1;
+    if (defer && D)
+        return ht == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 0, D>,
+                                                                     kBandBS, lds)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 1, D>,
+                                                                     kBandBS, lds);
+    return ht == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 0, false>,
+                                                                 kBandBS, lds)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 1, false>,
+                                                                 kBandBS, lds);
+}
+
+template <int REC>
+hipError_t band_occupancy(int rt, int ht, bool defer, int &resident, size_t lds) {
+    return rt == 0 ? band_occupancy_rt<0, REC>(ht, defer, resident, lds)
+                   : band_occupancy_rt<1, REC>(ht, defer, resident, lds);
+}
+
+template <int RT, int REC, bool STATS>
+void launch_per_ray(int ht, dim3 g, dim3 blk, hipStream_t st, const TraceArgs &a) {
+    constexpr bool D = !STATS && REC != 1;
+    if (D && a.defer_cap > 0) {
+        if (ht == 0) hipLaunchKernelGGL((k_trace<RT, REC, STATS, 0, D>), g, blk, 0, st, a);
+        else hipLaunchKernelGGL((k_trace<RT, REC, STATS, 1, D>), g, blk, 0, st, a);
+    } else {
+        if (ht == 0) hipLaunchKernelGGL((k_trace<RT, REC, STATS, 0, false>), g, blk, 0, st, a);
+        else hipLaunchKernelGGL((k_trace<RT, REC, STATS, 1, false>), g, blk, 0, st, a);
+    }
+}
+
+template <int RT, int REC>
+void launch_deferred_rt(int ht, dim3 g, hipStream_t st, const TraceArgs &a) {
+    if (ht == 0) hipLaunchKernelGGL((k_trace_deferred<RT, REC, 0>), g, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_trace_deferred<RT, REC, 1>), g, dim3(256), 0, st, a);
+}
+
+// The deferred re-trace of a launch whose trace kernel queued rays (a.defer_cap > 0): a grid of two
+// workgroups per CU striding over the queue (empty queues cost one short launch).
+qmc_status launch_deferred(qmc_ctx *c, const TraceArgs &a, int ht, int rec_mode) {
+    if (a.defer_cap <= 0) return QMC_OK;
+    const dim3 g((unsigned)(2 * std::max(c->num_cus, 1)));
+    if (a.recv.type == 0) {
+        if (rec_mode == 2) launch_deferred_rt<0, 2>(ht, g, c->stream, a);
+        else launch_deferred_rt<0, 0>(ht, g, c->stream, a);
+    } else {
+        if (rec_mode == 2) launch_deferred_rt<1, 2>(ht, g, c->stream, a);
+        else launch_deferred_rt<1, 0>(ht, g, c->stream, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return QMC_OK;
 }
 
 // The banded persistent trace when it pays (or when forced, for parity checks of this kernel).
@@ -1100,7 +1286,7 @@ hipError_t band_occupancy(int rt, int &resident, size_t lds) {
 // measured best) — and where every workgroup deposits many times its band (its band store and fold
 // are then a small part of the launch).
 qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, long long max_slots, TraceBufs &b,
-                           bool force, bool &launched) {
+                           bool force, int rec_mode, int ht, bool &launched) {
     launched = false;
     const int per_cu = 1;
     if (qmc_status st = device_limits(c)) return st;
@@ -1115,8 +1301,9 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
     if (!force && 2 * rows < Hh) return QMC_OK;
     const size_t lds = sizeof(float) * (size_t)rows * W;
     int resident = 0;
-    const hipError_t e = rec ? band_occupancy<true>(a.recv.type, resident, lds)
-                             : band_occupancy<false>(a.recv.type, resident, lds);
+    const bool defer = a.defer_cap > 0;
+    const hipError_t e = rec ? band_occupancy<1>(a.recv.type, ht, defer, resident, lds)
+                             : band_occupancy<0>(a.recv.type, ht, defer, resident, lds);
     if (e != hipSuccess || resident < 1) {
         if (force) return fail(QMC_E_UNSUPPORTED, "banded trace: no workgroup of the banded kernel is resident");
         return QMC_OK;
@@ -1144,14 +1331,18 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
     const int row0 = (Hh - rows) / 2;  // flux concentrates around the aim point, the receiver centre
     const bool mark = !rec;
     if (mark && c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
-    if (rec) {
-        if (a.recv.type == 0) launch_band<0, true>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
-        else launch_band<1, true>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+    if (rec && rec_mode == 2) {
+        if (a.recv.type == 0) launch_band<0, 2>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        else launch_band<1, 2>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+    } else if (rec) {
+        if (a.recv.type == 0) launch_band<0, 1>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        else launch_band<1, 1>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
     } else {
-        if (a.recv.type == 0) launch_band<0, false>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
-        else launch_band<1, false>(dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        if (a.recv.type == 0) launch_band<0, 0>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        else launch_band<1, 0>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
     }
     HIP_TRY(hipGetLastError());
+    if (qmc_status st = launch_deferred(c, a, ht, rec_mode)) return st;
     if (mark && c->kev_after >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_after], c->stream));
     hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256), 8), dim3(256), 0, c->stream, rb.acc, b.band,
                        nwg, (int)nband, row0 * W);
@@ -1161,8 +1352,9 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
 }
 
 // One launch over n list entries of one grid (d_list: their heliostat ids): culling kernels, the trace
-// kernel (QMC_KERNEL_*) and the fold of the launch's accumulator into the map. The wave masks of the
-// launch must fit in b (the caller sizes or splits launches).
+// kernel (QMC_KERNEL_*, with QMC_KERNEL_MAP_DECISIONS: records of the map path's decisions) and the
+// fold of the launch's accumulator into the map. The wave masks of the launch must fit in b (the
+// caller sizes or splits launches).
 qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots, qmc_ray_record *rec,
                         const long long *rec_base, StatsAcc *stats, int kernel, TraceBufs &b) {
     const GridBuf &gb = c->grids[grid];
@@ -1195,8 +1387,17 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     a.blocks_per_helio = (int)bx;
     if (bx == 0 || n == 0) return QMC_OK;
     qmc_status st;
+    const int rec_mode = !rec ? 0 : (kernel & QMC_KERNEL_MAP_DECISIONS) ? 2 : 1;
+    kernel &= ~QMC_KERNEL_MAP_DECISIONS;
+    if (!stats && rec_mode != 1 && b.defer_cap > 0 && b.defer && b.defer_n) {
+        a.defer = b.defer;
+        a.defer_n = b.defer_n;
+        a.defer_cap = (int32_t)std::min<size_t>({b.defer_cap, c->defer_limit ? c->defer_limit : b.defer_cap,
+                                                 (size_t)0x7fffffff});
+        c->last_defer_n = b.defer_n;
+    }
     if (!stats) {
-        if ((st = launch_cull_into(c, grid, d_list, n, b.cull, b.wide))) return st;
+        if ((st = launch_cull_into(c, grid, d_list, n, b.cull, b.wide, a.defer_n))) return st;
         a.wide = b.wide;
         if (2 * (size_t)n * (size_t)wave_stride(max_slots) > b.wmask_cap)
             return fail(QMC_E_STATE, "trace: wave masks exceed their buffer");
@@ -1205,7 +1406,8 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     const int npix = rb.w * rb.h;
     if (!stats && kernel != QMC_KERNEL_PER_RAY) {
         bool launched = false;
-        if ((st = try_band_launch(c, rb, a, max_slots, b, kernel == QMC_KERNEL_BANDED, launched))) return st;
+        if ((st = try_band_launch(c, rb, a, max_slots, b, kernel == QMC_KERNEL_BANDED, rec_mode, gb.htype, launched)))
+            return st;
         if (launched) {
             hipLaunchKernelGGL(k_acc_fold, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, rb.flux,
                                rb.acc, npix);
@@ -1217,17 +1419,22 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     const int rt = rb.r.type;
     const bool mark = !rec && !stats;
     if (mark && c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
-    if (rec) {
-        if (rt == 0) hipLaunchKernelGGL((k_trace<0, true, false>), g, blk, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_trace<1, true, false>), g, blk, 0, c->stream, a);
+    const int ht = gb.htype;
+    if (rec_mode == 2) {
+        if (rt == 0) launch_per_ray<0, 2, false>(ht, g, blk, c->stream, a);
+        else launch_per_ray<1, 2, false>(ht, g, blk, c->stream, a);
+    } else if (rec) {
+        if (rt == 0) launch_per_ray<0, 1, false>(ht, g, blk, c->stream, a);
+        else launch_per_ray<1, 1, false>(ht, g, blk, c->stream, a);
     } else if (stats) {
-        if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, true>), g, blk, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_trace<1, false, true>), g, blk, 0, c->stream, a);
+        if (rt == 0) launch_per_ray<0, 0, true>(ht, g, blk, c->stream, a);
+        else launch_per_ray<1, 0, true>(ht, g, blk, c->stream, a);
     } else {
-        if (rt == 0) hipLaunchKernelGGL((k_trace<0, false, false>), g, blk, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_trace<1, false, false>), g, blk, 0, c->stream, a);
+        if (rt == 0) launch_per_ray<0, 0, false>(ht, g, blk, c->stream, a);
+        else launch_per_ray<1, 0, false>(ht, g, blk, c->stream, a);
     }
     HIP_TRY(hipGetLastError());
+    if ((st = launch_deferred(c, a, ht, rec_mode))) return st;
     if (mark && c->kev_after >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_after], c->stream));
     hipLaunchKernelGGL(k_acc_fold, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, rb.flux, rb.acc,
                        npix);
@@ -1260,13 +1467,21 @@ void interleave_by_azimuth(qmc_ctx *c, const DevReceiver &R, int32_t *ids, int n
 }
 
 // Ctx-owned buffers of a trace call of n entries (grown as needed; record bases when rec).
-qmc_status ctx_trace_bufs(qmc_ctx *c, int n, bool rec, size_t wmask_need, TraceBufs &b) {
+qmc_status ctx_trace_bufs(qmc_ctx *c, int n, bool rec, size_t wmask_need, size_t defer_need, TraceBufs &b) {
     qmc_status st;
     const size_t nh = c->h_helios.size();
     // launches queued by earlier calls may still read the buffers that are about to be replaced
     if (n > c->list_cap || c->cull_cap < nh || c->wide_cap < nh || wmask_need > c->wmask_cap ||
-        (rec && (size_t)n > c->rec_base_cap))
+        (rec && (size_t)n > c->rec_base_cap) || defer_need > c->defer_cap)
         HIP_TRY(hipStreamSynchronize(c->stream));
+    if (defer_need > c->defer_cap) {
+        if ((st = dev_alloc(c->d_defer, defer_need))) {
+            c->defer_cap = 0;
+            return st;
+        }
+        c->defer_cap = defer_need;
+    }
+    if (!c->d_defer_n && (st = dev_alloc(c->d_defer_n, 1))) return st;
     if (n > c->list_cap) {
         if ((st = dev_alloc(c->d_list, (size_t)n))) {
             c->list_cap = 0;
@@ -1309,6 +1524,9 @@ qmc_status ctx_trace_bufs(qmc_ctx *c, int n, bool rec, size_t wmask_need, TraceB
     b.wmask_cap = c->wmask_cap;
     b.band = c->d_band_partials;
     b.band_cap = c->band_cap;
+    b.defer = c->d_defer;
+    b.defer_n = c->d_defer_n;
+    b.defer_cap = c->defer_cap;
     b.fixed = false;
     return QMC_OK;
 }
@@ -1329,7 +1547,8 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
     if (!c->pool_ready) return fail(QMC_E_STATE, "trace: set or generate the sample pools first");
     if (!c->offsets_ready) return fail(QMC_E_STATE, "trace: set or generate the micro-facet group offsets first");
     if (n < 0 || (n > 0 && !ids)) return fail(QMC_E_INVALID_ARG, "trace: bad heliostat list");
-    if (kernel != QMC_KERNEL_AUTO && kernel != QMC_KERNEL_PER_RAY && kernel != QMC_KERNEL_BANDED)
+    const int choice = kernel & ~QMC_KERNEL_MAP_DECISIONS;
+    if (choice != QMC_KERNEL_AUTO && choice != QMC_KERNEL_PER_RAY && choice != QMC_KERNEL_BANDED)
         return fail(QMC_E_INVALID_ARG, "trace: unknown kernel choice " + std::to_string(kernel));
     const int nh = (int)c->h_helios.size();
     for (int i = 0; i < n; ++i)
@@ -1363,7 +1582,11 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
         sorted[i] = ids[order[i]];
         if (rec) sbase[i] = rec_base_in[order[i]];
     }
-    size_t wmask_need = 0;
+    size_t wmask_need = 0, defer_need = 0;
+    // the map path defers the rays whose traversals its candidate tests and walk-free proof leave open
+    // (none to a few per 10^4 rays on the fields measured; a launch that overflows its queue replays
+    // all its rays instead)
+    const bool deferring = !stats && (!rec || (kernel & QMC_KERNEL_MAP_DECISIONS));
     for (int i = 0; i < n;) {
         const int p = pass[order[i]], g = c->h_helios[sorted[i]].grid;
         int e = i + 1;
@@ -1375,12 +1598,13 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
         if (max_slots > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
         spans.push_back(TraceSpan{p, g, i, e - i, max_slots});
         wmask_need = std::max(wmask_need, 2 * (size_t)(e - i) * (size_t)wave_stride(max_slots));
+        if (deferring) defer_need = std::max(defer_need, std::max<size_t>(65536, (size_t)(e - i) * (size_t)max_slots / 256));
         i = e;
     }
     TraceBufs b;
     if (fixed) {
         b = *fixed;
-    } else if ((st = ctx_trace_bufs(c, n, rec != nullptr, stats ? 0 : wmask_need, b))) {
+    } else if ((st = ctx_trace_bufs(c, n, rec != nullptr, stats ? 0 : wmask_need, defer_need, b))) {
         return st;
     }
     HIP_TRY(hipMemcpyAsync(b.list, sorted.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, c->stream));
@@ -1444,6 +1668,26 @@ extern "C" qmc_status qmc_offsets_resample(qmc_ctx *c, int32_t enable, uint64_t 
     return QMC_OK;
 }
 
+extern "C" qmc_status qmc_set_defer_capacity(qmc_ctx *c, int64_t rays) {
+    if (!c) return fail(QMC_E_INVALID_ARG, "qmc_set_defer_capacity: null ctx");
+    if (rays < 0) return fail(QMC_E_INVALID_ARG, "qmc_set_defer_capacity: negative capacity");
+    c->defer_limit = (size_t)rays;
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_get_defer_count(qmc_ctx *c, int64_t *rays) {
+    if (!c || !rays) return fail(QMC_E_INVALID_ARG, "qmc_get_defer_count: null argument");
+    *rays = 0;
+    if (!c->last_defer_n) return QMC_OK;
+    qmc_status st = set_device(c);
+    if (st) return st;
+    unsigned n = 0;
+    HIP_TRY(hipMemcpyAsync(&n, c->last_defer_n, sizeof(n), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *rays = (int64_t)n;
+    return QMC_OK;
+}
+
 extern "C" qmc_status qmc_cull_flags(qmc_ctx *c, const int32_t *ids, int32_t n, uint8_t *flags) {
     if (!c || (n > 0 && (!ids || !flags))) return fail(QMC_E_INVALID_ARG, "qmc_cull_flags: null argument");
     if (!c->uploaded) return fail(QMC_E_STATE, "qmc_cull_flags: upload a scene first");
@@ -1465,7 +1709,7 @@ extern "C" qmc_status qmc_cull_flags(qmc_ctx *c, const int32_t *ids, int32_t n, 
             }
         if (list.empty()) continue;
         TraceBufs b;
-        if ((st = ctx_trace_bufs(c, (int)list.size(), false, 0, b))) return st;
+        if ((st = ctx_trace_bufs(c, (int)list.size(), false, 0, 0, b))) return st;
         HIP_TRY(hipMemcpyAsync(b.list, list.data(), sizeof(int32_t) * list.size(), hipMemcpyHostToDevice, c->stream));
         if ((st = launch_cull_into(c, (int)g, b.list, (int)list.size(), b.cull, nullptr))) return st;
         HIP_TRY(hipMemcpyAsync(all.data(), b.cull, sizeof(CullList) * (size_t)nh, hipMemcpyDeviceToHost, c->stream));

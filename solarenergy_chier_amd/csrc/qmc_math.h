@@ -280,6 +280,100 @@ QMC_HD float cal_tmax(float dir, float interval, int idx, float cur) {
 QMC_HD bool dda_equal(float a, float b) { return (a < b + kEps) && (a > b - kEps); }
 QMC_HD bool dda_less(float a, float b) { return a < b - kEps; }
 
+// Proof that the reference's cell walk of the ray o + t*d (rectGridDDA.cu:31-119: start cell
+// (int)((o - pos) / interval), crossing times accumulated in float, steps chosen by the Epsilon rule)
+// visits a cell of the cell box [lo, hi] (inclusive indices) of a grid at pos with cell sizes s and n
+// cells per axis (extent E = n*s), given a point p = o + tp*d, tp > 0, inside the box. Then a quad listed
+// in every cell of that box and hit by the ray at tp is hit by the walk, unless the walk stops at another
+// hit first: either way the traversal reports a hit. False means "not proven" (the caller walks).
+// Argument, per axis a, in position units along a (exact arithmetic): the walk's crossing of a cell face
+// happens at a float time whose position o_a + t*d_a differs from the face by at most
+// err_a = 2 ulp(E) (start cell and first crossing) + per step (distance + cell) * 2^-24 (accumulation
+// and the rounded step) + 2*Epsilon*|d_a|. A step processes the earliest unprocessed crossing together
+// with others at most 2*Epsilon later, and each axis's crossings in their order. Take the first step that
+// processes a crossing later than tp: the cell the walk is in just before it has processed every crossing
+// earlier than tp - 2*Epsilon and none later than tp, so on every axis its index is that of a point
+// within err_a of p_a. The walk gets there: it starts inside the grid (o clear of the grid's faces) and
+// cannot cross a grid face before p, which is inside. So if p is farther than err_a from the box's faces
+// on every axis, the walk visits a cell of the box. An axis with |d_a| <= Epsilon is never stepped
+// (abs_divide): the walk keeps o's cell there, and p_a is taken as o_a. Distances are at most tp
+// (|d| = 1), so the per-step term is bounded once per ray with the largest 1/s and s. The bound is
+// widened four times and by 1 mm. Evaluation is in float: every quantity is at most E in size and takes a
+// few roundings, together below ulp(E) * 8 = E * 2^-20, which `slack` adds to every margin.
+//
+// The proof's per-grid constants (positions relative to the grid's): the largest 1/s and s over the
+// axes, the grid's centre E/2 and its half extent less the origin's margin (1 mm + slack).
+struct ProofFrame {
+    float pos[3], inv_max;
+    float gmid[3], cs_max;
+    float glim[3], pad_;
+};
+
+// A cell box as the proof reads it: its centre relative to the grid's position, and its half size less
+// the face margin before per-ray terms (1 mm + 4 * 2 ulp(E) + slack); negative for an empty box, which
+// the proof then never claims.
+struct ProofBox {
+    float mid[3], lim[3];
+};
+
+// Epsilon as a float: for a float x, x <= kEpsF exactly when x <= Epsilon (a double).
+constexpr float kEpsF = (float)kEps;
+static_assert((double)kEpsF <= kEps && (double)kEpsF + 0x1p-40 > kEps, "kEpsF is the float below Epsilon");
+
+// x rounded to a float not above it.
+inline float round_down(double x) {
+    const float f = (float)x;
+    return (double)f > x ? nextafterf(f, -INFINITY) : f;
+}
+
+// x rounded to a float not below it.
+inline float round_up(double x) {
+    const float f = (float)x;
+    return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+
+inline double proof_slack(double ext) { return ext * 0x1p-20; }
+
+inline ProofFrame proof_frame(const float pos[3], const float s[3], const int n[3]) {
+    ProofFrame f{};
+    for (int a = 0; a < 3; ++a) {
+        const double ext = (double)n[a] * (double)s[a];
+        f.pos[a] = pos[a];
+        f.inv_max = fmaxf(f.inv_max, round_up(1.0 / (double)s[a]));
+        f.cs_max = fmaxf(f.cs_max, s[a]);
+        f.gmid[a] = (float)(0.5 * ext);
+        f.glim[a] = round_down(0.5 * ext - 1e-3 - proof_slack(ext));
+    }
+    return f;
+}
+
+inline ProofBox proof_box(const float s[3], const int n[3], const int lo[3], const int hi[3]) {
+    ProofBox b{};
+    const bool empty = lo[0] > hi[0] || lo[1] > hi[1] || lo[2] > hi[2];
+    for (int a = 0; a < 3; ++a) {
+        const double ext = (double)n[a] * (double)s[a];
+        const double margin = 1e-3 + 4.0 * ext * 0x1p-22 + proof_slack(ext);
+        b.mid[a] = (float)(0.5 * (double)(lo[a] + hi[a] + 1) * (double)s[a]);
+        b.lim[a] = empty ? -1.0f : round_down(0.5 * (double)(hi[a] - lo[a] + 1) * (double)s[a] - margin);
+    }
+    return b;
+}
+
+QMC_HD bool dda_visit_proof(V3 o, V3 d, float tp, const ProofFrame &f, const float mid[3], const float lim[3]) {
+    // straight-line: every condition is evaluated and combined, no early exit
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+    // four times the per-step term at distance tp plus 2*Epsilon: the per-ray part of every axis's margin
+    const float S = 4.0f * fmaf(fmaf(tp, f.inv_max, 2.0f) * (tp + f.cs_max), 0x1p-24f, 2.0e-5f);
+    bool ok = tp > 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        const float ro = oo[a] - f.pos[a];  // the origin relative to the grid
+        const bool still = fabsf(dd[a]) <= kEpsF;
+        const float rp = still ? ro : fmaf(tp, dd[a], ro);
+        ok = ok & (fabsf(rp - mid[a]) < lim[a] - S) & (fabsf(ro - f.gmid[a]) < f.glim[a]);
+    }
+    return ok;
+}
+
 // Counter-based hashing (splitmix64 finaliser) for seeded offsets and Gaussian slope errors.
 QMC_HD uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;

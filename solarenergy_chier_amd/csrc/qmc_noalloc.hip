@@ -22,6 +22,7 @@ using namespace qmc;
 namespace {
 constexpr size_t kScratchBytes = 128ull << 20;  // 128 MiB per device
 constexpr int kMaxList = 65536;                 // list entries per call
+constexpr size_t kDeferEntries = 1u << 20;      // deferred rays queued per launch (8 MiB)
 constexpr int kMaxDevices = 64;
 }  // namespace
 
@@ -64,7 +65,8 @@ extern "C" qmc_status qmc_trace_noalloc(qmc_ctx *c, const int32_t *ids, int32_t 
     void *base_addr = nullptr;
     if ((st = scratch_address(c->device, &base_addr))) return st;
     if ((st = device_limits(c))) return st;
-    // layout: list | candidate lists (CullList, WideList per heliostat) | one band per CU | wave masks
+    // layout: list | candidate lists (CullList, WideList per heliostat) | one band per CU | deferred-ray
+    // queue (a full queue sends the rest through the full traversal in place) | wave masks
     char *base = static_cast<char *>(base_addr);
     size_t off = 0;
     auto take = [&](size_t bytes) {
@@ -79,6 +81,9 @@ extern "C" qmc_status qmc_trace_noalloc(qmc_ctx *c, const int32_t *ids, int32_t 
     b.wide = reinterpret_cast<WideList *>(take(sizeof(WideList) * nh));
     b.band_cap = (size_t)c->num_cus * (c->max_lds / sizeof(float));
     b.band = reinterpret_cast<float *>(take(sizeof(float) * b.band_cap));
+    b.defer_n = reinterpret_cast<unsigned *>(take(sizeof(unsigned)));
+    b.defer_cap = kDeferEntries;
+    b.defer = reinterpret_cast<unsigned long long *>(take(sizeof(unsigned long long) * kDeferEntries));
     if (off + 256 > kScratchBytes)
         return fail(QMC_E_RANGE, "qmc_trace_noalloc: the candidate lists of " + std::to_string(nh) +
                                      " heliostats and the workgroup bands exceed the " +

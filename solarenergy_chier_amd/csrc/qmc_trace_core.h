@@ -180,9 +180,11 @@ __device__ __forceinline__ FastDiv tile_divisor(const DevHelio &H, const FacetTi
 }
 
 // Ray slot -> ray id of heliostat H: flat heliostats take their rays in facet tiles (a wave's origins
-// form a compact patch); -1 for slots past the heliostat's rays.
+// form a compact patch); -1 for slots past the heliostat's rays. HT: the heliostat type when the
+// caller knows it at compile time (every heliostat of a grid has the grid's type), -1 to read it.
+template <int HT = -1>
 __device__ __forceinline__ long long slot_ray(const DevHelio &H, int L, long long slot) {
-    if (H.type == 0) {
+    if ((HT < 0 ? H.type : HT) == 0) {
         const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
         const FacetTiling tl = facet_tiling(R, C, L);
         return slot < tl.slots ? facet_tiled_ray(tl, tile_divisor(H, tl), R, C, (int)slot) : -1;
@@ -229,6 +231,13 @@ __device__ __forceinline__ T ld_uniform(const T *p) {
     return out;
 }
 
+// ld_uniform when U (the address is wave-uniform), else an ordinary load.
+template <bool U, class T>
+__device__ __forceinline__ T ld_maybe(const T *p) {
+    if constexpr (U) return ld_uniform(p);
+    else return *p;
+}
+
 // Whether a traversal has to run for this ray, given its heliostat's candidate list for that
 // traversal: not when no quad can be met (count 0) or when the ray misses every listed candidate —
 // the traversal tests only quads, the others provably miss, and a listed one gives the same
@@ -250,11 +259,13 @@ __device__ __forceinline__ bool traversal_needed(const CullList *cl, int kind, V
     return false;
 }
 
-// A ray's tests of its wave's masked candidates (traversal_plan): the (one or two) quads it hits and
-// the union of the cell ranges whose lists hold them (x0..x1, y0..y1, z0..z1).
+// A ray's tests of its wave's masked candidates (traversal_plan): how many it hits, the first and (with
+// two) the second in mask order, with the ray's parameter at each, and — with at most two hits — the
+// union of the cell ranges whose lists hold them (x0..x1, y0..y1, z0..z1).
 struct RayHits {
     int n;
     int id0, id1;
+    float t0, t1;  // the ray's parameter at each hit
     int x0, y0, z0, x1, y1, z1;
 };
 
@@ -262,38 +273,52 @@ struct RayHits {
 // mask of its wave (wm: the wave's two masks, from k_cull_waves): every quad outside the mask is out
 // of reach of every ray of the wave, so ray_quad misses it. Returns
 //   0 — no traversal: no candidate, or the ray misses every masked one, so no quad can be hit;
-//   1 — the ray hits one or two masked candidates (rh): the traversal's cell walk runs (dda_known),
-//       and since every quad's ray_quad result is known (a hit for those, a miss for all others), it
-//       only visits the cells listing them; its first hit is the reference traversal's;
-//   2 — the traversal runs with its own tests (too many candidates to list or mask, or more than two
-//       hit).
-// Records are read with scalar loads.
+//   1 — the ray hits rh.n >= 1 masked candidates (rh): the traversal reports a hit. With at most two,
+//       since every quad's ray_quad result is known (a hit for those, a miss for all others), the
+//       traversal's cell walk only needs the cells listing them (dda_known), and its first hit is the
+//       reference traversal's; with more, the first hit needs the traversal's own tests;
+//   2 — the traversal runs with its own tests (too many candidates to list, or, unless ALL, more than
+//       kRayTests in the wave's mask).
+// ALL: every masked candidate is tested, however many. Records are read with scalar loads when U (the
+// wave's lanes share the heliostat and the wave mask), with vector loads otherwise.
+template <bool U = true, bool ALL = false>
 __device__ __forceinline__ int traversal_plan(const WideList *wl, const uint64_t *wm, int kind, V3 o, V3 d,
                                               const DevGrid &g, RayHits &rh) {
-    const int n = ld_uniform(&wl->n[kind]);
+    const int n = ld_maybe<U>(&wl->n[kind]);
     if (n == 0) return 0;
     if (n > kCullWide) return 2;
-    uint64_t m = ld_uniform(wm + kind);
-    if (__builtin_popcountll(m) > kRayTests) return 2;
+    uint64_t m = ld_maybe<U>(wm + kind);
+    if (!ALL && __builtin_popcountll(m) > kRayTests) return 2;
     rh.n = 0;
     rh.id0 = rh.id1 = -1;
-#pragma unroll
-    for (int j = 0; j < kRayTests; ++j) {
-        if (!m) break;
-        const int i = __builtin_ctzll(m);
-        m &= m - 1;
-        const int id = ld_uniform(&wl->ids[kind][i]);
-        const float4 qa = ld_uniform(g.quads + 3 * id), qb = ld_uniform(g.quads + 3 * id + 1),
-                     qc = ld_uniform(g.quads + 3 * id + 2);
+    auto test = [&](int i) {
+        const int id = ld_maybe<U>(&wl->ids[kind][i]);
+        const float4 qa = ld_maybe<U>(g.quads + 3 * id), qb = ld_maybe<U>(g.quads + 3 * id + 1),
+                     qc = ld_maybe<U>(g.quads + 3 * id + 2);
         float t, u, v;
         if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t, u, v)) {
-            if (rh.n == 0) rh.id0 = id;
-            else rh.id1 = id;
+            // id1 ends as the last hit, the second one when there are two (with more it is not read)
+            if (rh.n == 0) rh.id0 = id, rh.t0 = t;
+            else rh.id1 = id, rh.t1 = t;
             ++rh.n;
+        }
+    };
+    if (ALL) {
+        while (m) {
+            const int i = __builtin_ctzll(m);
+            m &= m - 1;
+            test(i);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kRayTests; ++j) {
+            if (!m) break;
+            const int i = __builtin_ctzll(m);
+            m &= m - 1;
+            test(i);
         }
     }
     if (rh.n == 0) return 0;
-    if (rh.n > 2) return 2;
     // the cells that list the hit quads (grid_match's ranges; the union when there are two)
     const int4 lo0 = g.boxes[2 * rh.id0], hi0 = g.boxes[2 * rh.id0 + 1];
     rh.x0 = lo0.x, rh.y0 = lo0.y, rh.z0 = lo0.z, rh.x1 = hi0.x, rh.y1 = hi0.y, rh.z1 = hi0.z;
@@ -305,11 +330,11 @@ __device__ __forceinline__ int traversal_plan(const WideList *wl, const uint64_t
     return 1;
 }
 
-// The traversal of a ray with plan 1 (traversal_plan): the reference's cell walk in its order, where
-// only cells inside the hit quads' range can list a quad the ray hits. There the cell's candidates are
-// checked in list order against the hit ids; elsewhere the cell is stepped over without reading it;
-// once the walk has passed the range on any axis (indices only grow along the ray) no later cell is in
-// it and the walk ends without a hit. Its first hit is the reference traversal's.
+// The traversal of a ray with plan 1 and at most two hits (traversal_plan): the reference's cell walk
+// in its order, where only cells inside the hit quads' range can list a quad the ray hits. There the
+// cell's candidates are checked in list order against the hit ids; elsewhere the cell is stepped over
+// without reading it; once the walk has passed the range on any axis (indices only grow along the ray)
+// no later cell is in it and the walk ends without a hit. Its first hit is the reference traversal's.
 template <bool STATS>
 __device__ __forceinline__ bool dda_known(const RayHits &rh, V3 o, V3 d, const DevGrid &g, int own_lo, int own_hi,
                                           int &hit_id, RayStats &st) {
@@ -327,6 +352,24 @@ __device__ __forceinline__ bool dda_known(const RayHits &rh, V3 o, V3 d, const D
                     });
 }
 
+// The traversal's outcome for a ray with plan 1 without its walk, where that can be proven: the walk
+// would visit a cell of the cell box of the first quad the ray's tests hit (dda_visit_proof at that
+// hit; every cell of the box lists the quad), so the traversal reports a hit — that quad's or an
+// earlier one. True with hit_id = that quad (not necessarily the traversal's first hit: for callers
+// that need only whether a hit exists); false when the proof does not hold (the caller walks).
+// Straight-line code: the frame is read with vector loads (its address moved to vector registers
+// behind an empty barrier), so that a persistent caller keeps none of it in scalar registers across
+// its loop.
+__device__ __forceinline__ bool known_hit_proven(const RayHits &rh, V3 o, V3 d, const DevGrid &g, int &hit_id) {
+    const ProofFrame *pf = g.frame;
+    asm volatile("" : "+v"(pf));
+    const ProofFrame f = *pf;
+    const float4 m = g.pbox[2 * rh.id0], l = g.pbox[2 * rh.id0 + 1];
+    const float mid[3] = {m.x, m.y, m.z}, lim[3] = {l.x, l.y, l.z};
+    hit_id = rh.id0;
+    return dda_visit_proof(o, d, rh.t0, f, mid, lim);
+}
+
 // Half-angles (radians) at or above which a cone is not culled: the bound needs theta < 90 deg.
 constexpr double kCullMaxAngle = 80.0 * 3.14159265358979323846 / 180.0;
 
@@ -337,10 +380,12 @@ constexpr double kCullMargin = 0.01;
 // Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
 // onto the facet normal (returned by value: H may be a copy in registers).
 // Facet indices use the heliostat's multiply-high divisors (exact for m < 2^31).
+// HT as for slot_ray.
+template <int HT = -1>
 __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
                                                DevFrame &pert_frame, V3 &normal) {
     const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
-    if (H.type == 0) {
+    if ((HT < 0 ? H.type : HT) == 0) {
         origin = flat_facet_origin(H, H.fr, m);
         pert_frame = H.fr;
         normal = V3{H.fr.nx, H.fr.ny, H.fr.nz};

@@ -369,6 +369,17 @@ class Device:
         a = np.ascontiguousarray(list(ids), np.int32)
         N.call("qmc_trace", self.handle, N.iptr(a), len(a))
 
+    def set_defer_capacity(self, rays: int) -> None:
+        """Cap the queue of rays a map-path launch defers to its exact re-trace (0: sized from the
+        launch); a launch that defers more re-traces all its rays instead (same results, slower)."""
+        N.call("qmc_set_defer_capacity", self.handle, C.c_int64(rays))
+
+    def defer_count(self) -> int:
+        """Rays the last map-path launch deferred (waits for the context's stream)."""
+        n = C.c_int64(0)
+        N.call("qmc_get_defer_count", self.handle, C.byref(n))
+        return int(n.value)
+
     def probe_atomic_rates(self) -> dict:
         """Measured rates (G ops/s) of scattered float / 64-bit global and float LDS atomic adds."""
         r = N.AtomicRates()
@@ -389,8 +400,8 @@ class Device:
 
     def trace_records_many(self, ids: Iterable[int], kernel: int = N.KERNEL_AUTO) -> np.ndarray:
         """One trace of distinct heliostats with the chosen kernel (KERNEL_AUTO: the one trace() would
-        run); returns every ray's record, heliostat after heliostat in list order, each F*L records
-        indexed by ray id."""
+        run; or KERNEL_MAP_DECISIONS into it for records of the map path's own decisions); returns every
+        ray's record, heliostat after heliostat in list order, each F*L records indexed by ray id."""
         a = np.ascontiguousarray(list(ids), np.int32)
         L = self.scene.config.num_sunshape_lights_per_group
         n = sum(self.scene.heliostat_info(int(h)).n_micro for h in a) * L

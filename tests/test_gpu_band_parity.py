@@ -4,7 +4,9 @@
   its own slot, wave-mask and band indexing; every record is compared bit for bit with the oracle's
   (map_raytracing, RectangleReceiverRectangleGridRayTracing.cu:6-41) on the PS10-like field, under the
   bench's 50-degree sun and under low suns whose shadow traversals have candidates, and on the config-3
-  cylinder field.
+  cylinder field. The same holds for the records of the map path's own decisions (the path qmc_trace
+  runs, which decides traversals from the ray's candidate tests and the walk-free proof and re-traces
+  the rays these leave open), up to which hit quad a shadowed or blocked record names.
 * The full 483M-ray PS10-like map and the full config-3 map are compared with the oracle's
   double-precision maps (relative L1 <= 1e-5: float atomics reorder sums).
 * A heliostat listed twice is traced twice (one reference rayTracing call per occurrence), and with
@@ -33,8 +35,13 @@ def rel_l1(a, b):
     return float(np.abs(a - b).sum() / den) if den > 0 else float(np.abs(a).sum())
 
 
-def mismatches(gpu, cpu):
-    same = ((gpu["status"] == cpu["status"]) & (gpu["index"] == cpu["index"]) &
+def mismatches(gpu, cpu, map_decisions=False):
+    """Rays whose records differ. With the map path's decisions a shadowed or blocked record may name
+    another quad than the traversal's first hit (the ray hits both), so its index is not compared."""
+    index = gpu["index"] == cpu["index"]
+    if map_decisions:
+        index |= (cpu["status"] == N.RAY_SHADOWED) | (cpu["status"] == N.RAY_BLOCKED)
+    same = ((gpu["status"] == cpu["status"]) & index &
             (gpu["energy"].view(np.uint32) == cpu["energy"].view(np.uint32)) & (gpu["flags"] == cpu["flags"]))
     return int((~same).sum())
 
@@ -62,7 +69,7 @@ def records_vs_oracle(device, scene, orc, dirs, perts, L, ids, kernel):
         F = scene.heliostat_info(h).n_micro
         offs = device.offsets_get(h, F)
         _, crec, _ = orc.trace_heliostat(h, dirs, perts, L, offs, flux=cflux)
-        bad += mismatches(rec[at:at + F * L], crec)
+        bad += mismatches(rec[at:at + F * L], crec, bool(kernel & N.KERNEL_MAP_DECISIONS))
         at += F * L
     assert at == len(rec)
     dep = int((rec["status"] == N.RAY_DEPOSITED).sum())
@@ -112,6 +119,33 @@ def test_banded_kernel_records_cylinder_field(device):
     scene, orc, dirs, perts, L = upload(device, workloads.cylinder_field())
     bad, _, dep, _ = records_vs_oracle(device, scene, orc, dirs, perts, L, list(range(0, 2650, 16)),
                                        N.KERNEL_BANDED)
+    assert bad == 0 and dep > 0
+
+
+@pytest.mark.parametrize("kernel", [N.KERNEL_BANDED, N.KERNEL_PER_RAY])
+def test_map_path_decisions_ps10(device, kernel):
+    """The decisions of the path qmc_trace runs, where a traversal the walk-free proof (dda_visit_proof)
+    shows to reach a hit quad's cells runs no walk: every ray's status, pixel, energy and flags equal the
+    oracle's, on the sampled heliostats, every sixteenth heliostat and under a low sun with shadow
+    candidates."""
+    for sun, ids in (((180.0, 50.0), [0, 17, 300, 623]), ((180.0, 50.0), list(range(4, 624, 16))),
+                     ((135.0, 15.0), [0, 17, 100, 300, 450, 623])):
+        scene, orc, dirs, perts, L = upload(device, workloads.ps10_field(sun_deg=sun))
+        bad, _, dep, _ = records_vs_oracle(device, scene, orc, dirs, perts, L, ids,
+                                           kernel | N.KERNEL_MAP_DECISIONS)
+        assert bad == 0 and dep > 0
+
+
+def test_map_path_decisions_cylinder_and_small(device):
+    """The same on the config-3 field (focus-flat heliostats, cylinder receiver) and the small field."""
+    scene, orc, dirs, perts, L = upload(device, workloads.cylinder_field())
+    for kernel in (N.KERNEL_AUTO, N.KERNEL_BANDED):
+        bad, _, dep, _ = records_vs_oracle(device, scene, orc, dirs, perts, L, list(range(3, 2650, 16)),
+                                           kernel | N.KERNEL_MAP_DECISIONS)
+        assert bad == 0 and dep > 0
+    scene, orc, dirs, perts, L = upload(device, workloads.small_field())
+    bad, _, dep, _ = records_vs_oracle(device, scene, orc, dirs, perts, L, list(range(25)),
+                                       N.KERNEL_AUTO | N.KERNEL_MAP_DECISIONS)
     assert bad == 0 and dep > 0
 
 
