@@ -78,40 +78,27 @@ __device__ __forceinline__ int traversal_outcome(const TraceArgs &a, int plan, c
 }
 
 // One ray of map_raytracing (RectangleReceiverRectangleGridRayTracing.cu:6-41,
-// CylinderReceiverRectangleGridRayTracing.cu:6-40) up to, not including, its deposit. The ray is ray id
-// r of list entry li, held in slot `slot`. TM: how traversals are decided (kTmExact, kTmDefer,
-// kTmReplay); a deferred ray gets status kRayDeferred (no deposit, no record). HT: the heliostats'
-// type (that of the grid). U: the wave's lanes share the heliostat and the wave's candidate masks
-// (scalar loads); off in k_trace_deferred.
-template <int RT, bool STATS, int TM, int HT, bool U = true>
-__device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H, const WideList *wl,
-                                          const uint64_t *wm, long long r, int li, long long slot, RayOut &o,
-                                          RayStats &st) {
-    const int L = a.L;
-    const int m = (int)fast_div((uint32_t)r, a.fdL);
-    const int j = (int)r - m * L;
-    const int32_t *grp = a.offsets + H.offsets_base;
-    const int P = a.P;
+// CylinderReceiverRectangleGridRayTracing.cu:6-40) from its traversals on, up to, not including, its
+// deposit, given what its micro-facet and pool samples fix: the incident direction `dir` (the sun-shape
+// sample in world space), the facet's origin, slope-error frame pf and normal, and the slope-error sample
+// q. The ray is held in slot `slot` of list entry li; first: the ray is its facet's first. TM: how
+// traversals are decided (kTmExact, kTmDefer, kTmReplay); a deferred ray gets status kRayDeferred (no
+// deposit, no record). U: the wave's lanes share the heliostat and the wave's candidate masks (scalar
+// loads); off in k_trace_deferred.
+template <int RT, bool STATS, int TM, bool U = true>
+__device__ __forceinline__ void trace_from(const TraceArgs &a, const DevHelio &H, const WideList *wl,
+                                           const uint64_t *wm, int li, long long slot, bool first, V3 dir,
+                                           V3 origin, const DevFrame &pf, float4 q, RayOut &o, RayStats &st) {
     const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
     o.index = -1;
     o.energy = 0.0f;
     o.flags = 0;
     o.row = o.col = -1;
     o.unshadowed = o.unblocked = false;
-    o.first_of_facet = j == 0;
+    o.first_of_facet = first;
     o.replayed = false;
 
-    // Step 1: incident direction and shadowing (RectangleReceiverRectangleGridRayTracing.cu:14-22)
-    // both pool indices and samples are read before any test, so the four loads form two
-    // dependent pairs instead of a chain of four
-    const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
-    const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
-    const float4 s = a.pool[a1];
-    const float4 q = a.pert[a2];
-    V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
-    V3 origin, fnormal;
-    DevFrame pf;
-    facet_geometry<HT>(H, a.facets, m, origin, pf, fnormal);
+    // Step 1: shadowing (RectangleReceiverRectangleGridRayTracing.cu:14-22)
     int hit = -1;
     // traversals proved hit-free for this ray are skipped, and the others look up the hits that the
     // ray's candidate tests found (never while counting work)
@@ -159,6 +146,32 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     o.col = col;
     o.index = row * a.recv.res_x + col;
     o.status = QMC_RAY_DEPOSITED;
+}
+
+// One ray of map_raytracing: ray id r of list entry li, held in slot `slot`. Its micro-facet m = r / L
+// and sample j = r mod L give the pool indices (RectangleReceiverRectangleGridRayTracing.cu:14-16,
+// 24-26: group offsets of facets m and m + 1, plus j, modulo the pool size) and the facet's origin,
+// then trace_from traces it. HT: the heliostats' type (that of the grid). TM, U as for trace_from.
+template <int RT, bool STATS, int TM, int HT, bool U = true>
+__device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H, const WideList *wl,
+                                          const uint64_t *wm, long long r, int li, long long slot, RayOut &o,
+                                          RayStats &st) {
+    const int L = a.L;
+    const int m = (int)fast_div((uint32_t)r, a.fdL);
+    const int j = (int)r - m * L;
+    const int32_t *grp = a.offsets + H.offsets_base;
+    const int P = a.P;
+    // both pool indices and samples are read before any test, so the four loads form two
+    // dependent pairs instead of a chain of four
+    const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
+    const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
+    const float4 s = a.pool[a1];
+    const float4 q = a.pert[a2];
+    const V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
+    V3 origin, fnormal;
+    DevFrame pf;
+    facet_geometry<HT>(H, a.facets, m, origin, pf, fnormal);
+    trace_from<RT, STATS, TM, U>(a, H, wl, wm, li, slot, j == 0, dir, origin, pf, q, o, st);
 }
 
 // One lane per ray slot; grid.y = heliostat of the list, grid.x = blocks of 256 slots. Deposits
@@ -224,61 +237,142 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     }
 }
 
+// A band deposit: inside the band rows [row0, row0 + rows) an LDS atomic, else a device-scope atomic on
+// the launch's double accumulator; with REC the ray's record too.
+template <int REC>
+__device__ __forceinline__ void band_deposit(const TraceArgs &a, float *band, int row0, int rows, int W, int li,
+                                             long long r, const RayOut &o) {
+    if (o.status == QMC_RAY_DEPOSITED) {
+        const int br = o.row - row0;
+        if ((unsigned)br < (unsigned)rows)
+            atomicAdd(band + br * W + o.col, o.energy);
+        else
+            unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
+    }
+    if (REC && o.status != kRayDeferred) {
+        qmc_ray_record rec;
+        rec.status = o.status;
+        rec.index = o.index;
+        rec.energy = o.energy;
+        rec.flags = o.flags;
+        a.records[ld_uniform(a.rec_base + __builtin_amdgcn_readfirstlane(li)) + r] = rec;
+    }
+}
+
+// The heliostat-type parameter of k_trace_band for flat heliostats in the facet-shared order.
+constexpr int kHtFlatShared = 2;
+
+// Lane src's value of x, for every lane (ds_bpermute; every lane of the wave must be active).
+__device__ __forceinline__ int lane_read(int x, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, x); }
+__device__ __forceinline__ float lane_read(float x, int src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+}
+
 // Persistent form of k_trace for large launches. Global float atomics run at a fixed rate (about
 // 21 G/s for scattered addresses on MI355X, against 180 G/s in LDS), and a large field deposits
 // hundreds of millions of rays into one map, so each workgroup keeps a band of map rows
 // [row0, row0 + rows) in LDS for the whole launch: deposits inside it are LDS atomics, the others
-// device-scope atomics on the launch's double accumulator. Workgroups take BS-slot blocks
-// (heliostat, block) round robin; at the end each stores its band, with plain stores, to its own
-// slice of `partials`, which k_band_fold adds into the accumulator. With REC every ray also writes its
-// record at records[rec_base[li] + ray id] (parity checks of this kernel's own slot, wave-mask and
-// band indexing). DEFER as for k_trace.
+// device-scope atomics on the launch's double accumulator. At the end each workgroup stores its band,
+// with plain stores, to its own slice of `partials`, which k_band_fold adds into the accumulator. With
+// REC every ray also writes its record at records[rec_base[li] + ray id] (parity checks of this kernel's
+// own slot, wave-mask and band indexing). DEFER as for k_trace.
+//
+// Two orders of work over the same slots:
+//  * HT = 0 or 1 (the heliostats' type): workgroups take BS-slot blocks (list entry, block) round robin,
+//    one 64-slot wave each;
+//  * HT = kHtFlatShared (flat heliostats with L = 2^lLs rays per facet, 16 <= L <= 64, whose slots are
+//    their ray ids): each wave takes blocks of 64 micro-facets (64 L slots) round robin, lane k computes
+//    facet k's origin and group offsets once, and the wave then runs the block's L steps of 64 slots,
+//    each lane reading its facet's values from the lane that computed them. The L rays of a facet share
+//    all of it, so this is done once per facet instead of once per ray. blocks_per_helio counts such
+//    blocks. (A kernel of its own: with both loops in one kernel the scalar registers of one constrain
+//    the other.)
 template <int RT, int BS, int REC, int HT, bool DEFER>
 __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int rows, int blocks_per_helio,
-                                                    float *partials) {
+                                                    float *partials, int lLs) {
     static_assert(!DEFER || REC != 1, "deferral is for the map path");
     extern __shared__ float band[];
     const int W = a.recv.res_x, nband = rows * W;
     for (int p = threadIdx.x; p < nband; p += BS) band[p] = 0.0f;
     __syncthreads();
     RayStats st{0u, 0u, 0u};
-    // block (li, bx) of the launch, advanced by gridDim.x blocks per step; the heliostat id is made
-    // explicitly wave-uniform so that its record is read with scalar loads, as in k_trace
-    int li = (int)(blockIdx.x / (unsigned)blocks_per_helio);
-    int bx = (int)blockIdx.x - li * blocks_per_helio;
+    constexpr int TM = DEFER ? kTmDefer : kTmExact;
+    if constexpr (HT == kHtFlatShared) {
+        const int L = 1 << lLs, fpi = 64 >> lLs;  // rays per facet, facets per 64-slot step
+        const int lane = (int)(threadIdx.x & 63u);
+        const int lfac = lane >> lLs, lray = lane & (L - 1);
+        const int P = a.P;
+        const int nw = (int)gridDim.x * (BS / 64);
+        const int gw = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BS / 64) + (threadIdx.x >> 6)));
+        int li = gw / blocks_per_helio;
+        int bx = gw - li * blocks_per_helio;
 #pragma nounroll
-    while (li < a.n_list) {
-        // the launch writes none of these records, so they are read with scalar loads
-        const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
-        const DevHelio H = ld_uniform(a.helios + h);
-        const long long slot = (long long)__builtin_amdgcn_readfirstlane(bx) * BS + threadIdx.x;
-        const long long r = slot_ray<HT>(H, a.L, slot);
-        if (r >= 0) {
-            RayOut o;
-            const int wave = __builtin_amdgcn_readfirstlane(bx * (BS / 64) + (int)(threadIdx.x >> 6));
-            const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
-            trace_ray<RT, false, DEFER ? kTmDefer : kTmExact, HT>(a, H, a.wide ? a.wide + h : nullptr, wm, r, li,
-                                                                  slot, o, st);
-            if (o.status == QMC_RAY_DEPOSITED) {
-                const int br = o.row - row0;
-                if ((unsigned)br < (unsigned)rows)
-                    atomicAdd(band + br * W + o.col, o.energy);
-                else
-                    unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
+        while (li < a.n_list) {
+            const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
+            const DevHelio H = ld_uniform(a.helios + h);
+            const int f0 = __builtin_amdgcn_readfirstlane(bx) * 64;
+            // this lane's facet of the block: origin (RectangleHelio.cu:64-79) and group offsets
+            V3 fo{0.0f, 0.0f, 0.0f};
+            int g0 = 0, g1 = 0;
+            if (f0 + lane < H.F) {
+                const int f = f0 + lane;
+                fo = flat_facet_origin(H, H.fr, f);
+                const int32_t *grp = a.offsets + H.offsets_base;
+                g0 = grp[f];
+                g1 = grp[fast_mod((uint32_t)(f + 1), P, a.fdP)];
             }
-            if (REC && o.status != kRayDeferred) {
-                qmc_ray_record rec;
-                rec.status = o.status;
-                rec.index = o.index;
-                rec.energy = o.energy;
-                rec.flags = o.flags;
-                a.records[ld_uniform(a.rec_base + __builtin_amdgcn_readfirstlane(li)) + r] = rec;
+            const int nf = min(64, H.F - f0);  // facets of this block
+#pragma nounroll
+            for (int i = 0; i * fpi < nf; ++i) {
+                const int src = i * fpi + lfac;  // this step's facet for this lane, within the block
+                const V3 origin{lane_read(fo.x, src), lane_read(fo.y, src), lane_read(fo.z, src)};
+                const int o1 = lane_read(g0, src), o2 = lane_read(g1, src);
+                if (src < nf) {
+                    const long long slot = (long long)(f0 + src) * L + lray;  // = the ray id
+                    // (g + j) mod P for g in [0, P] and j < L <= P: one conditional subtraction
+                    const uint32_t u1 = (uint32_t)(o1 + lray), u2 = (uint32_t)(o2 + lray);
+                    const int a1 = (int)min(u1, u1 - (uint32_t)P), a2 = (int)min(u2, u2 - (uint32_t)P);
+                    const float4 s = a.pool[a1];
+                    const float4 q = a.pert[a2];
+                    const V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
+                    RayOut o;
+                    const int wave = __builtin_amdgcn_readfirstlane((f0 >> 6) * L + i);  // slot >> 6
+                    const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
+                    trace_from<RT, false, TM>(a, H, a.wide ? a.wide + h : nullptr, wm, li, slot, lray == 0, dir,
+                                              origin, H.fr, q, o, st);
+                    band_deposit<REC>(a, band, row0, rows, W, li, slot, o);
+                }
+            }
+            bx += nw;
+            while (bx >= blocks_per_helio) {
+                bx -= blocks_per_helio;
+                ++li;
             }
         }
-        bx += (int)gridDim.x;
-        while (bx >= blocks_per_helio) {
-            bx -= blocks_per_helio;
-            ++li;
+    } else {
+        // block (li, bx) of the launch, advanced by gridDim.x blocks per step; the heliostat id is made
+        // explicitly wave-uniform so that its record is read with scalar loads, as in k_trace
+        int li = (int)(blockIdx.x / (unsigned)blocks_per_helio);
+        int bx = (int)blockIdx.x - li * blocks_per_helio;
+#pragma nounroll
+        while (li < a.n_list) {
+            // the launch writes none of these records, so they are read with scalar loads
+            const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
+            const DevHelio H = ld_uniform(a.helios + h);
+            const long long slot = (long long)__builtin_amdgcn_readfirstlane(bx) * BS + threadIdx.x;
+            const long long r = slot_ray<HT>(H, a.L, slot);
+            if (r >= 0) {
+                RayOut o;
+                const int wave = __builtin_amdgcn_readfirstlane(bx * (BS / 64) + (int)(threadIdx.x >> 6));
+                const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
+                trace_ray<RT, false, TM, HT>(a, H, a.wide ? a.wide + h : nullptr, wm, r, li, slot, o, st);
+                band_deposit<REC>(a, band, row0, rows, W, li, r, o);
+            }
+            bx += (int)gridDim.x;
+            while (bx >= blocks_per_helio) {
+                bx -= blocks_per_helio;
+                ++li;
+            }
         }
     }
     __syncthreads();
@@ -1209,40 +1303,47 @@ constexpr int kBandBS = 1024;
 // the launch defers (a.defer_cap > 0: the map path with culling).
 template <int RT, int REC, bool DEFER>
 void launch_band_d(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph,
-                   float *parts) {
-    if (ht == 0)
+                   float *parts, int lLs) {
+    if (ht == 0 && lLs)
+        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, kHtFlatShared, DEFER>), g, dim3(kBandBS), lds, st, a, row0,
+                           rows, bph, parts, lLs);
+    else if (ht == 0)
         hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 0, DEFER>), g, dim3(kBandBS), lds, st, a, row0, rows, bph,
-                           parts);
+                           parts, 0);
     else
         hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 1, DEFER>), g, dim3(kBandBS), lds, st, a, row0, rows, bph,
-                           parts);
+                           parts, 0);
 }
 
 template <int RT, int REC>
 void launch_band(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph,
-                 float *parts) {
-    if (REC != 1 && a.defer_cap > 0) launch_band_d<RT, REC, REC != 1>(ht, g, lds, st, a, row0, rows, bph, parts);
-    else launch_band_d<RT, REC, false>(ht, g, lds, st, a, row0, rows, bph, parts);
+                 float *parts, int lLs) {
+    if (REC != 1 && a.defer_cap > 0) launch_band_d<RT, REC, REC != 1>(ht, g, lds, st, a, row0, rows, bph, parts, lLs);
+    else launch_band_d<RT, REC, false>(ht, g, lds, st, a, row0, rows, bph, parts, lLs);
 }
 
-template <int RT, int REC>
-hipError_t band_occupancy_rt(int ht, bool defer, int &resident, size_t lds) {
-    constexpr bool D = REC != 1;
-    if (defer && D)
-        return ht == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 0, D>,
-                                                                     kBandBS, lds)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 1, D>,
-                                                                     kBandBS, lds);
-    return ht == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 0, false>,
+template <int RT, int REC, bool D>
+hipError_t band_occupancy_d(int ht, bool shared, int &resident, size_t lds) {
+    if (ht == 0 && shared)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, kHtFlatShared, D>,
+                                                           kBandBS, lds);
+    return ht == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 0, D>,
                                                                  kBandBS, lds)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 1, false>,
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 1, D>,
                                                                  kBandBS, lds);
 }
 
+template <int RT, int REC>
+hipError_t band_occupancy_rt(int ht, bool defer, bool shared, int &resident, size_t lds) {
+    constexpr bool D = REC != 1;
+    if (defer && D) return band_occupancy_d<RT, REC, D>(ht, shared, resident, lds);
+    return band_occupancy_d<RT, REC, false>(ht, shared, resident, lds);
+}
+
 template <int REC>
-hipError_t band_occupancy(int rt, int ht, bool defer, int &resident, size_t lds) {
-    return rt == 0 ? band_occupancy_rt<0, REC>(ht, defer, resident, lds)
-                   : band_occupancy_rt<1, REC>(ht, defer, resident, lds);
+hipError_t band_occupancy(int rt, int ht, bool defer, bool shared, int &resident, size_t lds) {
+    return rt == 0 ? band_occupancy_rt<0, REC>(ht, defer, shared, resident, lds)
+                   : band_occupancy_rt<1, REC>(ht, defer, shared, resident, lds);
 }
 
 template <int RT, int REC, bool STATS>
@@ -1300,17 +1401,20 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
     }
     if (!force && 2 * rows < Hh) return QMC_OK;
     const size_t lds = sizeof(float) * (size_t)rows * W;
+    // flat heliostats with 16, 32 or 64 rays per micro-facet take the facet-shared order (k_trace_band):
+    // blocks of 64 facets per wave; their slots are their ray ids (slot_ray)
+    const int lLs = ht != 0 || a.P < a.L ? 0 : a.L == 16 ? 4 : a.L == 32 ? 5 : a.L == 64 ? 6 : 0;
     int resident = 0;
     const bool defer = a.defer_cap > 0;
-    const hipError_t e = rec ? band_occupancy<1>(a.recv.type, ht, defer, resident, lds)
-                             : band_occupancy<0>(a.recv.type, ht, defer, resident, lds);
+    const hipError_t e = rec ? band_occupancy<1>(a.recv.type, ht, defer, lLs > 0, resident, lds)
+                             : band_occupancy<0>(a.recv.type, ht, defer, lLs > 0, resident, lds);
     if (e != hipSuccess || resident < 1) {
         if (force) return fail(QMC_E_UNSUPPORTED, "banded trace: no workgroup of the banded kernel is resident");
         return QMC_OK;
     }
     const int nwg = c->num_cus * std::min(per_cu, resident);
-    const long long bph = (max_slots + kBandBS - 1) / kBandBS;
-    const long long total = bph * kBandBS * a.n_list;
+    const long long bph = lLs ? (max_slots / a.L + 63) / 64 : (max_slots + kBandBS - 1) / kBandBS;
+    const long long total = lLs ? bph * 64 * a.L * a.n_list : bph * kBandBS * a.n_list;
     const long long nband = (long long)rows * W;
     if (!force && total < 8 * (long long)nwg * nband) return QMC_OK;
     if ((size_t)nwg * nband > b.band_cap) {
@@ -1332,14 +1436,14 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
     const bool mark = !rec;
     if (mark && c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
     if (rec && rec_mode == 2) {
-        if (a.recv.type == 0) launch_band<0, 2>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
-        else launch_band<1, 2>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        if (a.recv.type == 0) launch_band<0, 2>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
+        else launch_band<1, 2>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
     } else if (rec) {
-        if (a.recv.type == 0) launch_band<0, 1>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
-        else launch_band<1, 1>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        if (a.recv.type == 0) launch_band<0, 1>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
+        else launch_band<1, 1>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
     } else {
-        if (a.recv.type == 0) launch_band<0, 0>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
-        else launch_band<1, 0>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band);
+        if (a.recv.type == 0) launch_band<0, 0>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
+        else launch_band<1, 0>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
     }
     HIP_TRY(hipGetLastError());
     if (qmc_status st = launch_deferred(c, a, ht, rec_mode)) return st;

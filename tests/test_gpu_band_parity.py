@@ -122,6 +122,29 @@ def test_banded_kernel_records_cylinder_field(device):
     assert bad == 0 and dep > 0
 
 
+@pytest.mark.parametrize("L", [16, 32, 64])
+def test_banded_facet_shared_order_ragged_blocks(device, L):
+    """Flat heliostats with 16, 32 or 64 rays per micro-facet take the banded kernel's facet-shared
+    order (each wave computes 64 facets' origins and group offsets once, then traces their rays in L
+    steps). Heliostats 12.79 m wide have 255 x 189 facets, not a multiple of 64, so every heliostat ends
+    in a partial block; records of both record modes equal the oracle's, under the bench's sun and a low
+    sun with shadow candidates."""
+    import dataclasses
+
+    for sun in ((180.0, 50.0), (135.0, 15.0)):
+        wl = workloads.ps10_field(L=L, groups=1024, n=80, sun_deg=sun)
+        wl.heliostats = [dataclasses.replace(h, size=(12.79, 0.1, 9.45)) for h in wl.heliostats]
+        scene, orc, dirs, perts, L_ = upload(device, wl)
+        assert L_ == L
+        F = scene.heliostat_info(0).n_micro
+        assert F % 64 != 0
+        ids = [0, 9, 40, 79]
+        for kernel in (N.KERNEL_BANDED, N.KERNEL_BANDED | N.KERNEL_MAP_DECISIONS):
+            bad, n, dep, _ = records_vs_oracle(device, scene, orc, dirs, perts, L, ids, kernel)
+            assert n == len(ids) * F * L
+            assert bad == 0 and dep > 0
+
+
 @pytest.mark.parametrize("kernel", [N.KERNEL_BANDED, N.KERNEL_PER_RAY])
 def test_map_path_decisions_ps10(device, kernel):
     """The decisions of the path qmc_trace runs, where a traversal the walk-free proof (dda_visit_proof)
