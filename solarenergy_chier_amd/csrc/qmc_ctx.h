@@ -61,6 +61,10 @@ struct qmc_ctx {
     int64_t offsets_total = 0;
     float4 *d_pool = nullptr;
     float4 *d_pert = nullptr;
+    // the sun-shape pool rotated by the sun frame (what every traced ray starts from), recomputed by
+    // the next plain trace after the pool or the sun changes
+    float4 *d_pool_world = nullptr;
+    bool pool_world_ready = false;
     int P = 0, L = 0;
     bool pool_ready = false, offsets_ready = false;
     int32_t *d_list = nullptr;

@@ -168,10 +168,9 @@ struct WaveArgs {
 struct TraceArgs {
     DevGrid grid;
     DevReceiver recv;
-    DevFrame sun;              // local2world frame of -sun_dir
     const DevHelio *helios;
     const DevFacet *facets;
-    const float4 *pool;
+    const float4 *pool;        // sun-shape samples in world space (rotated by the sun frame)
     const float4 *pert;
     const int32_t *offsets;
     const int32_t *list;       // heliostat ids of this launch

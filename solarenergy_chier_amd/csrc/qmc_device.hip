@@ -165,9 +165,9 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, const DevHelio &H,
     // dependent pairs instead of a chain of four
     const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
     const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
-    const float4 s = a.pool[a1];
+    const float4 s = a.pool[a1];  // the sample in world space (k_pool_world)
     const float4 q = a.pert[a2];
-    const V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
+    const V3 dir{s.x, s.y, s.z};
     V3 origin, fnormal;
     DevFrame pf;
     facet_geometry<HT>(H, a.facets, m, origin, pf, fnormal);
@@ -332,9 +332,9 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                     // (g + j) mod P for g in [0, P] and j < L <= P: one conditional subtraction
                     const uint32_t u1 = (uint32_t)(o1 + lray), u2 = (uint32_t)(o2 + lray);
                     const int a1 = (int)min(u1, u1 - (uint32_t)P), a2 = (int)min(u2, u2 - (uint32_t)P);
-                    const float4 s = a.pool[a1];
+                    const float4 s = a.pool[a1];  // world space (k_pool_world)
                     const float4 q = a.pert[a2];
-                    const V3 dir = fr_apply(a.sun, V3{s.x, s.y, s.z});
+                    const V3 dir{s.x, s.y, s.z};
                     RayOut o;
                     const int wave = __builtin_amdgcn_readfirstlane((f0 >> 6) * L + i);  // slot >> 6
                     const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
@@ -583,6 +583,17 @@ __global__ __launch_bounds__(256) void k_offsets_list(int32_t *off, const DevHel
     off[H.offsets_base + m] = group_offset(seed, h, m, P);
 }
 
+// The sun-shape pool in world space: local2world_rotate of each sample by the sun frame, the rotation
+// every ray of map_raytracing applies to its sample (RectangleReceiverRectangleGridRayTracing.cu:14-16),
+// done once per pool entry with the same float operations.
+__global__ __launch_bounds__(256) void k_pool_world(float4 *world, const float4 *pool, int P, DevFrame sun) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const float4 s = pool[i];
+    const V3 d = fr_apply(sun, V3{s.x, s.y, s.z});
+    world[i] = make_float4(d.x, d.y, d.z, 0.0f);
+}
+
 __global__ __launch_bounds__(256) void k_facets(const DevHelio *helios, const DevFacet *facets, int h, float *origins,
                                                 float *normals) {
     const DevHelio &H = helios[h];
@@ -648,6 +659,7 @@ extern "C" qmc_status qmc_destroy(qmc_ctx *c) {
     free_scene(c);
     dev_free(c->d_pool);
     dev_free(c->d_pert);
+    dev_free(c->d_pool_world);
     dev_free(c->d_list);
     dev_free(c->d_stats);
     dev_free(c->d_rec);
@@ -828,6 +840,7 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
     // sun frame: local2world_rotate(sample, -sun_dir)
     V3 sun{s->cfg.sun_dir[0], s->cfg.sun_dir[1], s->cfg.sun_dir[2]};
     c->sun = to_dev(make_frame(-sun, false));
+    c->pool_world_ready = false;
     // receivers
     c->recvs.resize(s->receivers.size());
     for (size_t i = 0; i < s->receivers.size(); ++i) {
@@ -1026,6 +1039,7 @@ extern "C" qmc_status qmc_update(qmc_ctx *c, const qmc_scene *s) {
     c->cfg = n;
     V3 sun{n.sun_dir[0], n.sun_dir[1], n.sun_dir[2]};
     c->sun = to_dev(make_frame(-sun, false));
+    c->pool_world_ready = false;
     c->h_helios = helios;
     aim_points_f64(*s, c->h_aim64);
     HIP_TRY(hipMemcpyAsync(c->d_helios, helios.data(), helios.size() * sizeof(DevHelio), hipMemcpyHostToDevice,
@@ -1060,7 +1074,9 @@ static qmc_status ensure_pool(qmc_ctx *c, int P, int L) {
         qmc_status st;
         if ((st = dev_alloc(c->d_pool, (size_t)P))) return st;
         if ((st = dev_alloc(c->d_pert, (size_t)P))) return st;
+        if ((st = dev_alloc(c->d_pool_world, (size_t)P))) return st;
     }
+    c->pool_world_ready = false;  // the caller writes a new pool
     c->P = P;
     c->L = L;
     return QMC_OK;
@@ -1467,10 +1483,15 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     std::memset(&a, 0, sizeof(a));
     a.grid = gb.g;
     a.recv = rb.r;
-    a.sun = c->sun;
     a.helios = c->d_helios;
     a.facets = c->d_facets;
-    a.pool = c->d_pool;
+    if (!c->pool_world_ready) {
+        hipLaunchKernelGGL(k_pool_world, dim3((unsigned)((c->P + 255) / 256)), dim3(256), 0, c->stream,
+                           c->d_pool_world, c->d_pool, c->P, c->sun);
+        HIP_TRY(hipGetLastError());
+        c->pool_world_ready = true;
+    }
+    a.pool = c->d_pool_world;
     a.pert = c->d_pert;
     a.offsets = c->d_offsets;
     a.list = d_list;
