@@ -357,6 +357,13 @@ typedef struct qmc_atomic_rates {
     double global_f32, global_f64, lds_f32;
 } qmc_atomic_rates;
 qmc_status qmc_probe_atomic_rates(qmc_ctx *ctx, qmc_atomic_rates *rates);
+/* Check of the device's short square-root and reciprocal-square-root sequences (used by the tracer's
+ * normalisations for arguments in [2^-8, 2^8]) against the general correctly rounded sequences, for
+ * every float in that range: `checked` receives the number of arguments, `mismatches[0]` those whose
+ * short sqrtf differs from sqrtf, `mismatches[1]` those whose short 1.0f / sqrtf differs, and
+ * `mismatches[2]` those where sqrtf or 1.0f / sqrtf differs from the double-precision computation
+ * rounded to float. No counterpart in the reference. Synchronous. */
+qmc_status qmc_check_fast_sqrt(qmc_ctx *ctx, uint64_t *checked, uint64_t mismatches[3]);
 
 /* Traversal-culling flags the trace uses for the listed heliostats under the current pools and
  * scene: bit 0 set = the shadow traversal can meet another quad, bit 1 set = the blocking traversal

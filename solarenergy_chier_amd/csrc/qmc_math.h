@@ -48,9 +48,52 @@ QMC_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 QMC_HD V3 cross(V3 a, V3 b) {
     return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-QMC_HD float length(V3 v) { return sqrtf(dot(v, v)); }
+// Correctly rounded sqrtf(x) and 1.0f / sqrtf(x) (each operation rounded once, as IEEE arithmetic and the
+// host compute them). On the device, x in [2^-8, 2^8] takes a short sequence: the hardware square root
+// (within 1 ulp), replaced by a neighbour when that neighbour's residual shows it is nearer, and the
+// reciprocal of that root by a Newton-Raphson step on the hardware reciprocal followed by two residual
+// corrections, the last one Markstein's (P. Markstein, "Computation of elementary functions on the IBM
+// RISC System/6000 processor", IBM J. Res. Develop. 34(1), 1990). In that range no operand, quotient or
+// residual is denormal, infinite or near overflow, so the general sequences' exponent scaling and
+// special-value steps change nothing and are left out. Both results are compared with the general
+// sequences for every float of the range on the GPU (qmc_check_fast_sqrt). Other x take the general
+// sequences.
+constexpr uint32_t kFastSqrtLo = 0x3B800000u;  // 2^-8
+constexpr uint32_t kFastSqrtHi = 0x43800000u;  // 2^8
+#if defined(__HIPCC__)
+__device__ __forceinline__ bool fast_sqrt_range(float x) {
+    return __float_as_uint(x) - kFastSqrtLo <= kFastSqrtHi - kFastSqrtLo;
+}
+__device__ __forceinline__ float sqrt_in_range(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u), sp = __uint_as_float(__float_as_uint(s) + 1u);
+    float r = __builtin_fmaf(-sm, s, x) <= 0.0f ? sm : s;
+    r = __builtin_fmaf(-sp, s, x) > 0.0f ? sp : r;
+    return r;
+}
+__device__ __forceinline__ float recip_in_range(float d) {
+    const float y0 = __builtin_amdgcn_rcpf(d);
+    const float y = __builtin_fmaf(__builtin_fmaf(-d, y0, 1.0f), y0, y0);
+    const float q = __builtin_fmaf(__builtin_fmaf(-d, y, 1.0f), y, y);
+    return __builtin_fmaf(__builtin_fmaf(-d, q, 1.0f), y, q);
+}
+#endif
+QMC_HD float sqrt_cr(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (fast_sqrt_range(x)) return sqrt_in_range(x);
+#endif
+    return sqrtf(x);
+}
+QMC_HD float inv_sqrt_cr(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (fast_sqrt_range(x)) return recip_in_range(sqrt_in_range(x));
+#endif
+    return 1.0f / sqrtf(x);
+}
+
+QMC_HD float length(V3 v) { return sqrt_cr(dot(v, v)); }
 QMC_HD V3 normalize(V3 v) {
-    float inv = 1.0f / sqrtf(dot(v, v));
+    float inv = inv_sqrt_cr(dot(v, v));
     return v * inv;
 }
 QMC_HD V3 reflect(V3 i, V3 n) { return i - (2.0f * n) * dot(n, i); }
@@ -61,10 +104,10 @@ struct V2 {
 };
 QMC_HD float dot2(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
 QMC_HD V2 normalize2(V2 v) {
-    float inv = 1.0f / sqrtf(dot2(v, v));
+    float inv = inv_sqrt_cr(dot2(v, v));
     return V2{v.x * inv, v.y * inv};
 }
-QMC_HD float length2(V2 v) { return sqrtf(dot2(v, v)); }
+QMC_HD float length2(V2 v) { return sqrt_cr(dot2(v, v)); }
 
 // Local frame of an "aligned normal": u = normalize((0,1,0) x n), v = normalize(u x n); a local
 // vector d maps to d.x*u + d.y*n + d.z*v. When |n.x| and |n.z| are below Epsilon the map is the

@@ -1,7 +1,9 @@
-// qmc_probe.hip — device capability probe: the rates of the atomic operations the tracer deposits
-// with (no counterpart in the reference). The trace's achieved deposit rates are reported against
-// these ceilings: float and 64-bit device-scope atomic adds scattered over a map the size of the
-// PS10-like field's (256 x 256), and float LDS atomic adds scattered over a 64 KB tile.
+// qmc_probe.hip — device capability probes (no counterpart in the reference):
+//  * the rates of the atomic operations the tracer deposits with. The trace's achieved deposit rates
+//    are reported against these ceilings: float and 64-bit device-scope atomic adds scattered over a
+//    map the size of the PS10-like field's (256 x 256), and float LDS atomic adds scattered over a
+//    64 KB tile;
+//  * the exhaustive check of the short square-root sequences of qmc_math.h against the general ones.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -43,7 +45,48 @@ __global__ __launch_bounds__(256) void k_probe_lds(float *sink) {
     if (threadIdx.x == 0) sink[blockIdx.x] = tile[0];
 }
 
+// Every float x in [2^-8, 2^8] (bit patterns kFastSqrtLo + i): the short sequences against sqrtf and
+// 1.0f / sqrtf, and those against the double computation rounded to float (double rounding of a float
+// square root or quotient through double is exact: 53 >= 2 * 24 + 2).
+__global__ __launch_bounds__(256) void k_check_fast_sqrt(unsigned long long *bad, uint32_t n) {
+    unsigned long long b0 = 0, b1 = 0, b2 = 0;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const float x = __uint_as_float(kFastSqrtLo + i);
+        const float s = sqrtf(x), r = 1.0f / s;
+        const float fs = sqrt_in_range(x), fr = recip_in_range(fs);
+        const float ds = (float)sqrt((double)x), dr = (float)(1.0 / (double)ds);
+        b0 += __float_as_uint(fs) != __float_as_uint(s);
+        b1 += __float_as_uint(fr) != __float_as_uint(r);
+        b2 += (__float_as_uint(ds) != __float_as_uint(s)) | (__float_as_uint(dr) != __float_as_uint(r));
+    }
+    if (b0) atomicAdd(bad, b0);
+    if (b1) atomicAdd(bad + 1, b1);
+    if (b2) atomicAdd(bad + 2, b2);
+}
+
 }  // namespace
+
+extern "C" qmc_status qmc_check_fast_sqrt(qmc_ctx *c, uint64_t *checked, uint64_t mismatches[3]) {
+    if (!c || !checked || !mismatches) return fail(QMC_E_INVALID_ARG, "qmc_check_fast_sqrt: null argument");
+    qmc_status st = set_device(c);
+    if (st) return st;
+    unsigned long long *bad = nullptr;
+    if ((st = dev_alloc(bad, 3))) return st;
+    const uint32_t n = kFastSqrtHi - kFastSqrtLo + 1u;
+    unsigned long long h[3] = {0, 0, 0};
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(h), c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_check_fast_sqrt, dim3(4096), dim3(256), 0, c->stream, bad, n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h, bad, sizeof(h), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dev_free(bad);
+    if (e != hipSuccess) return fail(QMC_E_HIP, std::string("qmc_check_fast_sqrt: ") + hipGetErrorString(e));
+    *checked = n;
+    for (int k = 0; k < 3; ++k) mismatches[k] = h[k];
+    return QMC_OK;
+}
 
 extern "C" qmc_status qmc_probe_atomic_rates(qmc_ctx *c, qmc_atomic_rates *out) {
     if (!c || !out) return fail(QMC_E_INVALID_ARG, "qmc_probe_atomic_rates: null argument");

@@ -386,6 +386,14 @@ class Device:
         N.call("qmc_probe_atomic_rates", self.handle, C.byref(r))
         return {"global_f32": r.global_f32, "global_f64": r.global_f64, "lds_f32": r.lds_f32}
 
+    def check_fast_sqrt(self) -> tuple[int, list[int]]:
+        """Exhaustive check of the short square-root sequences over [2^-8, 2^8]: (arguments checked,
+        [short sqrt mismatches, short 1/sqrt mismatches, general-vs-double mismatches])."""
+        n = C.c_uint64(0)
+        bad = (C.c_uint64 * 3)()
+        N.call("qmc_check_fast_sqrt", self.handle, C.byref(n), bad)
+        return int(n.value), [int(b) for b in bad]
+
     def cull_flags(self, ids: Iterable[int]) -> np.ndarray:
         """Per heliostat: bit 0 = shadow traversal needed, bit 1 = blocking traversal needed."""
         a = np.ascontiguousarray(list(ids), np.int32)

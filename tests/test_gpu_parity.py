@@ -443,6 +443,18 @@ def test_atomic_rate_probe(device):
     assert r["lds_f32"] > r["global_f32"]
 
 
+def test_fast_sqrt_sequences_exhaustive(device):
+    """The tracer's normalisations take a short square-root and reciprocal sequence for arguments in
+    [2^-8, 2^8] (no exponent scaling or special-value steps). For every one of the 134,217,729 floats
+    in that range both results equal the general correctly rounded sqrtf and 1.0f / sqrtf bit for bit,
+    and those equal the double-precision computation rounded to float (an independent reference: a
+    float square root or quotient rounded through double is exact)."""
+    n, bad = device.check_fast_sqrt()
+    print(f"fast sqrt check: {n} arguments, mismatches {bad}", flush=True)
+    assert n == (0x43800000 - 0x3B800000) + 1
+    assert bad == [0, 0, 0]
+
+
 def test_kernel_events_bracket_the_trace_kernel(device):
     """qmc_kernel_events: the library's events around the trace kernel of a plain trace measure a
     positive time within the whole call's (which adds the culling kernels and folds); bad slots are
