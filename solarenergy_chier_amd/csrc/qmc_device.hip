@@ -84,11 +84,12 @@ __device__ __forceinline__ int traversal_outcome(const TraceArgs &a, int plan, c
 // q. The ray is held in slot `slot` of list entry li; first: the ray is its facet's first. TM: how
 // traversals are decided (kTmExact, kTmDefer, kTmReplay); a deferred ray gets status kRayDeferred (no
 // deposit, no record). U: the wave's lanes share the heliostat and the wave's candidate masks (scalar
-// loads); off in k_trace_deferred.
+// loads); off in k_trace_deferred. ot (nullable): the origin's terms of the rectangle receiver's test.
 template <int RT, bool STATS, int TM, bool U = true>
 __device__ __forceinline__ void trace_from(const TraceArgs &a, const DevHelio &H, const WideList *wl,
                                            const uint64_t *wm, int li, long long slot, bool first, V3 dir,
-                                           V3 origin, const DevFrame &pf, float4 q, RayOut &o, RayStats &st) {
+                                           V3 origin, const DevFrame &pf, float4 q, RayOut &o, RayStats &st,
+                                           const QuadOriginTerms *ot = nullptr) {
     const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
     o.index = -1;
     o.energy = 0.0f;
@@ -133,7 +134,7 @@ __device__ __forceinline__ void trace_from(const TraceArgs &a, const DevHelio &H
     o.unblocked = true;
     // Step 3: receiver hit (Receiver/*ReceiverIntersection.cu)
     float t, u, v;
-    if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v)) {
+    if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v, ot)) {
         o.status = QMC_RAY_MISSED;
         return;
     }
@@ -311,15 +312,21 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
             const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
             const DevHelio H = ld_uniform(a.helios + h);
             const int f0 = __builtin_amdgcn_readfirstlane(bx) * 64;
-            // this lane's facet of the block: origin (RectangleHelio.cu:64-79) and group offsets
+            // this lane's facet of the block: origin (RectangleHelio.cu:64-79), group offsets and, for a
+            // rectangle receiver, the origin's terms of the receiver test
             V3 fo{0.0f, 0.0f, 0.0f};
             int g0 = 0, g1 = 0;
+            QuadOriginTerms fot{};
             if (f0 + lane < H.F) {
                 const int f = f0 + lane;
                 fo = flat_facet_origin(H, H.fr, f);
                 const int32_t *grp = a.offsets + H.offsets_base;
                 g0 = grp[f];
                 g1 = grp[fast_mod((uint32_t)(f + 1), P, a.fdP)];
+                if (RT == 0)
+                    fot = quad_origin_terms(fo, V3{a.recv.A[0], a.recv.A[1], a.recv.A[2]},
+                                            V3{a.recv.E1[0], a.recv.E1[1], a.recv.E1[2]},
+                                            V3{a.recv.E2[0], a.recv.E2[1], a.recv.E2[2]});
             }
             const int nf = min(64, H.F - f0);  // facets of this block
 #pragma nounroll
@@ -327,6 +334,12 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                 const int src = i * fpi + lfac;  // this step's facet for this lane, within the block
                 const V3 origin{lane_read(fo.x, src), lane_read(fo.y, src), lane_read(fo.z, src)};
                 const int o1 = lane_read(g0, src), o2 = lane_read(g1, src);
+                QuadOriginTerms ot;
+                if (RT == 0) {
+                    ot.T = V3{lane_read(fot.T.x, src), lane_read(fot.T.y, src), lane_read(fot.T.z, src)};
+                    ot.q = V3{lane_read(fot.q.x, src), lane_read(fot.q.y, src), lane_read(fot.q.z, src)};
+                    ot.tq = lane_read(fot.tq, src);
+                }
                 if (src < nf) {
                     const long long slot = (long long)(f0 + src) * L + lray;  // = the ray id
                     // (g + j) mod P for g in [0, P] and j < L <= P: one conditional subtraction
@@ -339,7 +352,7 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                     const int wave = __builtin_amdgcn_readfirstlane((f0 >> 6) * L + i);  // slot >> 6
                     const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
                     trace_from<RT, false, TM>(a, H, a.wide ? a.wide + h : nullptr, wm, li, slot, lray == 0, dir,
-                                              origin, H.fr, q, o, st);
+                                              origin, H.fr, q, o, st, RT == 0 ? &ot : nullptr);
                     band_deposit<REC>(a, band, row0, rows, W, li, slot, o);
                 }
             }

@@ -197,6 +197,33 @@ QMC_HD bool ray_quad(V3 o, V3 d, V3 A, V3 E1, V3 E2, float &t, float &u, float &
     return true;
 }
 
+// The terms of ray_quad that depend on the ray's origin only: T = o - A, q = T x E1 and E2 . q. Rays
+// from one micro-facet share them; ray_quad_pre takes them precomputed and performs the same operations
+// in the same order as ray_quad.
+struct QuadOriginTerms {
+    V3 T, q;
+    float tq;
+};
+QMC_HD QuadOriginTerms quad_origin_terms(V3 o, V3 A, V3 E1, V3 E2) {
+    QuadOriginTerms r;
+    r.T = o - A;
+    r.q = cross(r.T, E1);
+    r.tq = dot(E2, r.q);
+    return r;
+}
+QMC_HD bool ray_quad_pre(V3 d, V3 E1, V3 E2, const QuadOriginTerms &ot, float &t, float &u, float &v) {
+    V3 p = cross(d, E2);
+    float det = dot(E1, p);
+    if (fabsf(det) < kEps) return false;
+    u = dot(ot.T, p) / det;
+    if (u < 0 || u > 1) return false;
+    v = dot(d, ot.q) / det;
+    if (v < 0 || v > 1) return false;
+    t = ot.tq / det;
+    if (t < kEps) return false;
+    return true;
+}
+
 // DDA helpers (QuasiMonteCarloRayTracing/Grid/rectGridDDA.cuh:15-32).
 // Exact quotient of 0 <= n < 2^31 by a divisor d >= 1 fixed ahead of the kernel, as a multiply-high:
 // l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, q = (mulhi(n, m) + n) >> l (the round-up

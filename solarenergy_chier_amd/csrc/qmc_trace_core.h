@@ -407,11 +407,15 @@ __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet
     }
 }
 
-// Receiver hit: true with (t, u, v) when the ray reaches the receiver surface.
+// Receiver hit: true with (t, u, v) when the ray reaches the receiver surface. For a rectangle
+// receiver, ot (nullable) holds the ray origin's terms of the test (quad_origin_terms with the
+// receiver's A, E1, E2), computed once for the rays of a micro-facet.
 template <int RT>
-__device__ __forceinline__ bool receiver_hit(const DevReceiver &R, V3 o, V3 d, float &t, float &u, float &v) {
+__device__ __forceinline__ bool receiver_hit(const DevReceiver &R, V3 o, V3 d, float &t, float &u, float &v,
+                                             const QuadOriginTerms *ot = nullptr) {
     if (RT == 0) {
         // RectangleReceiver::GIntersect (RectangleReceiver.cuh:32-34)
+        if (ot) return ray_quad_pre(d, V3{R.E1[0], R.E1[1], R.E1[2]}, V3{R.E2[0], R.E2[1], R.E2[2]}, *ot, t, u, v);
         return ray_quad(o, d, V3{R.A[0], R.A[1], R.A[2]}, V3{R.E1[0], R.E1[1], R.E1[2]},
                         V3{R.E2[0], R.E2[1], R.E2[2]}, t, u, v);
     } else {
