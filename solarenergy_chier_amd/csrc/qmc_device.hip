@@ -68,7 +68,15 @@ __device__ __forceinline__ int traversal_outcome(const TraceArgs &a, int plan, c
     if (plan == 0) return 0;
     if (TM != kTmExact && plan == 1 && known_hit_proven(rh, o, d, a.grid, hit)) return 1;
     if (TM == kTmDefer) {
-        const unsigned k = atomicAdd(a.defer_n, 1u);
+        // ballot compaction: the wave's lanes that defer here take consecutive queue entries, reserved
+        // by one atomic of the lowest such lane (its result read back by every lane), so the deferred
+        // rays end up dense for the re-trace
+        const uint64_t act = __ballot(1);
+        const unsigned rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+        unsigned base = 0;
+        if (rank == 0) base = atomicAdd(a.defer_n, (unsigned)__builtin_popcountll(act));
+        const unsigned k = (unsigned)__builtin_amdgcn_readlane((int)base, __builtin_ctzll(act)) + rank;
         if (k < (unsigned)a.defer_cap) a.defer[k] = ((unsigned long long)(unsigned)li << 32) | (unsigned long long)slot;
         return 2;
     }
