@@ -533,12 +533,18 @@ def bench_adjoint(dev, steps: int = 64) -> dict:
             "optimisation_s": opt_s}
 
 
-def crn_finite_differences(dev, scene, offs, buf, npix, T, PK, LAM, n_check: int = 8) -> dict:
+def crn_finite_differences(dev, scene, offs, buf, npix, T, PK, LAM, n_check: int = 8, nodes: int = 128) -> dict:
     """Central differences with common random numbers (same pools, offsets, frozen visibility; the
     masked forward accumulates in double) of the loss w.r.t. the n_check largest aim-offset gradient
-    components, against the adjoint, at the configuration's 1e-3 m step and at 1e-6 m. The bilinear
-    splat is piecewise linear in each hit position, kinked where a hit crosses a pixel centre, so a
-    step that is not small against the pixel averages the derivative over those kinks."""
+    components, against the adjoint, at the configuration's 1e-3 m step and at 1e-4 and 1e-6 m.
+
+    The bilinear splat is piecewise linear in each hit position, kinked where a hit crosses a pixel
+    centre, so the loss is continuous and piecewise smooth along a coordinate and its central
+    difference over [x - h, x + h] equals the mean of the derivative over that interval (fundamental
+    theorem of calculus), not the derivative at x. Beside the pointwise comparison, the difference
+    quotient is therefore also compared with the adjoint averaged over the interval: the adjoint of
+    the frozen-visibility model at `nodes` midpoint nodes (dL/dF from the double masked map at each
+    node, the visibility of the base forward)."""
     import numpy as np
 
     scene.set_aim_offsets(offs)
@@ -558,21 +564,43 @@ def crn_finite_differences(dev, scene, offs, buf, npix, T, PK, LAM, n_check: int
         dev.update(scene)
         return dev.adjoint_forward_masked(0)
 
+    def adjoint_at(o):
+        f = flux_at(o)
+        dev.flux_write(0, (2.0 * (f - T) + 2.0 * LAM * np.maximum(f - PK, 0.0)).astype(np.float32))
+        g, _ = dev.adjoint_backward(scene, 0, d_grad=buf)
+        return g
+
     order = np.argsort(-np.abs(ga).ravel())[:n_check]
-    out = {"components": int(n_check), "bar": 1e-3}
+    out = {"components": int(n_check), "bar": 1e-3, "interval_mean_nodes": int(nodes)}
+    t0 = time.perf_counter()
     for h in (1e-3, 1e-4, 1e-6):
-        fd_v, ad_v = [], []
+        fd_v, ad_v, im_v = [], [], []
         for flat in order:
             hh, k = divmod(int(flat), 2)
             op, om = offs.copy(), offs.copy()
             op[hh, k] += h
             om[hh, k] -= h
-            step = float(op[hh, k]) - float(om[hh, k])
-            fd_v.append(dloss(flux_at(op), flux_at(om)) / step)
+            lo, hi = float(om[hh, k]), float(op[hh, k])  # the float32 offsets actually used
+            fd_v.append(dloss(flux_at(op), flux_at(om)) / (hi - lo))
             ad_v.append(ga[hh, k])
-        err = float(np.linalg.norm(np.array(fd_v) - np.array(ad_v)) / np.linalg.norm(ad_v))
+            if h >= 1e-4:
+                acc = 0.0
+                for m in range(nodes):
+                    o = offs.copy()
+                    o[hh, k] = np.float32(lo + (m + 0.5) * (hi - lo) / nodes)
+                    acc += float(adjoint_at(o)[hh, k])
+                im_v.append(acc / nodes)
+        fd_v, ad_v = np.array(fd_v), np.array(ad_v)
+        err = float(np.linalg.norm(fd_v - ad_v) / np.linalg.norm(ad_v))
         out[f"rel_err_step_{h:g}m"] = err
-        log(f"adjoint: CRN central differences, step {h:g} m: relative error {err:.2e} over {n_check} components")
+        msg = f"adjoint: CRN central differences, step {h:g} m: relative error {err:.2e} over {n_check} components"
+        if im_v:
+            im_v = np.array(im_v)
+            e_im = float(np.linalg.norm(fd_v - im_v) / np.linalg.norm(im_v))
+            out[f"rel_err_step_{h:g}m_interval_mean_adjoint"] = e_im
+            msg += f"; against the adjoint averaged over the step interval {e_im:.2e}"
+        log(msg)
+    out["seconds"] = time.perf_counter() - t0
     flux_at(offs)
     return out
 
