@@ -268,8 +268,9 @@ __device__ __forceinline__ void band_deposit(const TraceArgs &a, float *band, in
     }
 }
 
-// The heliostat-type parameter of k_trace_band for flat heliostats in the facet-shared order.
-constexpr int kHtFlatShared = 2;
+// The heliostat-type parameters of k_trace_band for flat and focus-flat heliostats in the facet-shared
+// order.
+constexpr int kHtFlatShared = 2, kHtFocusShared = 3;
 
 // Lane src's value of x, for every lane (ds_bpermute; every lane of the wave must be active).
 __device__ __forceinline__ int lane_read(int x, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, x); }
@@ -289,13 +290,14 @@ __device__ __forceinline__ float lane_read(float x, int src) {
 // Two orders of work over the same slots:
 //  * HT = 0 or 1 (the heliostats' type): workgroups take BS-slot blocks (list entry, block) round robin,
 //    one 64-slot wave each;
-//  * HT = kHtFlatShared (flat heliostats with L = 2^lLs rays per facet, 16 <= L <= 64, whose slots are
-//    their ray ids): each wave takes blocks of 64 micro-facets (64 L slots) round robin, lane k computes
-//    facet k's origin and group offsets once, and the wave then runs the block's L steps of 64 slots,
-//    each lane reading its facet's values from the lane that computed them. The L rays of a facet share
-//    all of it, so this is done once per facet instead of once per ray. blocks_per_helio counts such
-//    blocks. (A kernel of its own: with both loops in one kernel the scalar registers of one constrain
-//    the other.)
+//  * HT = kHtFlatShared or kHtFocusShared (flat heliostats with L = 2^lLs rays per facet, 16 <= L <= 64,
+//    or focus-flat ones with 2 <= L <= 64: slots that are their ray ids): each wave takes blocks of 64
+//    micro-facets (64 L slots) round robin, lane k computes facet k's origin, group offsets and (focus-
+//    flat) canted-facet index once, and the wave then runs the block's L steps of 64 slots, each lane
+//    reading its facet's values from the lane that computed them. The L rays of a facet share all of
+//    it, so this is done once per facet instead of once per ray. blocks_per_helio counts such blocks.
+//    (Kernels of their own: with both loops in one kernel the scalar registers of one constrain the
+//    other.)
 template <int RT, int BS, int REC, int HT, bool DEFER>
 __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int rows, int blocks_per_helio,
                                                     float *partials, int lLs) {
@@ -306,7 +308,8 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
     __syncthreads();
     RayStats st{0u, 0u, 0u};
     constexpr int TM = DEFER ? kTmDefer : kTmExact;
-    if constexpr (HT == kHtFlatShared) {
+    if constexpr (HT == kHtFlatShared || HT == kHtFocusShared) {
+        constexpr bool focus = HT == kHtFocusShared;
         const int L = 1 << lLs, fpi = 64 >> lLs;  // rays per facet, facets per 64-slot step
         const int lane = (int)(threadIdx.x & 63u);
         const int lfac = lane >> lLs, lray = lane & (L - 1);
@@ -320,14 +323,15 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
             const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
             const DevHelio H = ld_uniform(a.helios + h);
             const int f0 = __builtin_amdgcn_readfirstlane(bx) * 64;
-            // this lane's facet of the block: origin (RectangleHelio.cu:64-79), group offsets and, for a
-            // rectangle receiver, the origin's terms of the receiver test
+            // this lane's facet of the block: origin (RectangleHelio.cu:64-79, FocusFlatRectangleHelio.cu:
+            // 21-39,53-64), canted facet (focus-flat), group offsets and, for a rectangle receiver, the
+            // origin's terms of the receiver test
             V3 fo{0.0f, 0.0f, 0.0f};
-            int g0 = 0, g1 = 0;
+            int g0 = 0, g1 = 0, fx = 0;
             QuadOriginTerms fot{};
             if (f0 + lane < H.F) {
                 const int f = f0 + lane;
-                fo = flat_facet_origin(H, H.fr, f);
+                fo = focus ? focus_facet_origin(H, a.facets, f, fx) : flat_facet_origin(H, H.fr, f);
                 const int32_t *grp = a.offsets + H.offsets_base;
                 g0 = grp[f];
                 g1 = grp[fast_mod((uint32_t)(f + 1), P, a.fdP)];
@@ -342,6 +346,7 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                 const int src = i * fpi + lfac;  // this step's facet for this lane, within the block
                 const V3 origin{lane_read(fo.x, src), lane_read(fo.y, src), lane_read(fo.z, src)};
                 const int o1 = lane_read(g0, src), o2 = lane_read(g1, src);
+                const int fidx = focus ? lane_read(fx, src) : 0;
                 QuadOriginTerms ot;
                 if (RT == 0) {
                     ot.T = V3{lane_read(fot.T.x, src), lane_read(fot.T.y, src), lane_read(fot.T.z, src)};
@@ -359,8 +364,10 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                     RayOut o;
                     const int wave = __builtin_amdgcn_readfirstlane((f0 >> 6) * L + i);  // slot >> 6
                     const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
+                    // the slope-error frame: the heliostat's, or the canted facet's (a small table)
+                    const DevFrame pf = focus ? a.facets[fidx].fr_pert : H.fr;
                     trace_from<RT, false, TM>(a, H, a.wide ? a.wide + h : nullptr, wm, li, slot, lray == 0, dir,
-                                              origin, H.fr, q, o, st, RT == 0 ? &ot : nullptr);
+                                              origin, pf, q, o, st, RT == 0 ? &ot : nullptr);
                     band_deposit<REC>(a, band, row0, rows, W, li, slot, o);
                 }
             }
@@ -1344,6 +1351,9 @@ void launch_band_d(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &
     if (ht == 0 && lLs)
         hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, kHtFlatShared, DEFER>), g, dim3(kBandBS), lds, st, a, row0,
                            rows, bph, parts, lLs);
+    else if (lLs)
+        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, kHtFocusShared, DEFER>), g, dim3(kBandBS), lds, st, a, row0,
+                           rows, bph, parts, lLs);
     else if (ht == 0)
         hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 0, DEFER>), g, dim3(kBandBS), lds, st, a, row0, rows, bph,
                            parts, 0);
@@ -1363,6 +1373,9 @@ template <int RT, int REC, bool D>
 hipError_t band_occupancy_d(int ht, bool shared, int &resident, size_t lds) {
     if (ht == 0 && shared)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, kHtFlatShared, D>,
+                                                           kBandBS, lds);
+    if (shared)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, kHtFocusShared, D>,
                                                            kBandBS, lds);
     return ht == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_trace_band<RT, kBandBS, REC, 0, D>,
                                                                  kBandBS, lds)
@@ -1417,12 +1430,12 @@ qmc_status launch_deferred(qmc_ctx *c, const TraceArgs &a, int ht, int rec_mode)
     return QMC_OK;
 }
 
-// The banded persistent trace when it pays (or when forced, for parity checks of this kernel).
-// Measured on MI355X: its persistent loop runs about 15% slower than one lane per ray (traversal-heavy
-// fields lose more), so it is used only where the band takes most deposits off the global atomics — a
-// band of at least half the map's rows, one workgroup per CU with the whole LDS (4 waves per SIMD
-// measured best) — and where every workgroup deposits many times its band (its band store and fold
-// are then a small part of the launch).
+// The banded persistent trace when it pays (or when forced, for parity checks of this kernel): one
+// workgroup per CU with the whole LDS (4 waves per SIMD). Device-scope atomics bound the one-lane-per-ray
+// kernel on large fields (about 24 G/s on MI355X), so the band is used where it holds at least a quarter
+// of the map's rows (measured: the config-3 cylinder field, 80 of 256 rows taking 48.5% of its deposits,
+// 6.3 -> 4.7 ms per call) and where every workgroup deposits many times its band (its band store and
+// fold are then a small part of the launch).
 qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, long long max_slots, TraceBufs &b,
                            bool force, int rec_mode, int ht, bool &launched) {
     launched = false;
@@ -1436,11 +1449,13 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
         if (force) return fail(QMC_E_UNSUPPORTED, "banded trace: one map row does not fit in LDS");
         return QMC_OK;
     }
-    if (!force && 2 * rows < Hh) return QMC_OK;
+    if (!force && 4 * rows < Hh) return QMC_OK;
     const size_t lds = sizeof(float) * (size_t)rows * W;
-    // flat heliostats with 16, 32 or 64 rays per micro-facet take the facet-shared order (k_trace_band):
-    // blocks of 64 facets per wave; their slots are their ray ids (slot_ray)
-    const int lLs = ht != 0 || a.P < a.L ? 0 : a.L == 16 ? 4 : a.L == 32 ? 5 : a.L == 64 ? 6 : 0;
+    // flat heliostats with 16, 32 or 64 rays per micro-facet and focus-flat ones with 2 to 64 (a power of
+    // two) take the facet-shared order (k_trace_band): blocks of 64 facets per wave; their slots are their
+    // ray ids (slot_ray)
+    const int lL = a.L == 2 ? 1 : a.L == 4 ? 2 : a.L == 8 ? 3 : a.L == 16 ? 4 : a.L == 32 ? 5 : a.L == 64 ? 6 : 0;
+    const int lLs = a.P < a.L ? 0 : ht == 0 ? (lL >= 4 ? lL : 0) : lL;
     int resident = 0;
     const bool defer = a.defer_cap > 0;
     const hipError_t e = rec ? band_occupancy<1>(a.recv.type, ht, defer, lLs > 0, resident, lds)

@@ -377,6 +377,24 @@ constexpr double kCullMaxAngle = 80.0 * 3.14159265358979323846 / 180.0;
 // and the intersection test.
 constexpr double kCullMargin = 0.01;
 
+// Origin of micro-facet m of a focus-flat heliostat (FocusFlatRectangleHelio.cu:21-39,53-64): its
+// centre in its canted facet's local frame, placed by the facet's frame and centre, then by the
+// heliostat's focus-variant frame and position. fidx receives the canted facet's DevFacet index.
+__device__ __forceinline__ V3 focus_facet_origin(const DevHelio &H, const DevFacet *facets, int m, int &fidx) {
+    const int cpr = H.cols * H.sub_cols;
+    int row = (int)fast_div((uint32_t)m, H.fd_cpr);
+    int col = m - row * cpr;
+    const int brow = (int)fast_div((uint32_t)row, H.fd_sub_rows), bcol = (int)fast_div((uint32_t)col, H.fd_sub_cols);
+    fidx = H.facet_base + brow * H.cols + bcol;
+    const DevFacet &Fc = facets[fidx];
+    row -= brow * H.sub_rows;
+    col -= bcol * H.sub_cols;
+    const float pl = H.pl;
+    V3 local{(col + 0.5f) * pl - H.len_col / 2, 0.0f, (row + 0.5f) * pl - H.len_row / 2};
+    local = fr_apply(Fc.fr_local, local) + V3{Fc.center[0], Fc.center[1], Fc.center[2]};
+    return fr_apply(H.fr_focus, local) + V3{H.pos[0], H.pos[1], H.pos[2]};
+}
+
 // Micro-facet m of heliostat H: world origin and the frame that rotates slope-error samples
 // onto the facet normal (returned by value: H may be a copy in registers).
 // Facet indices use the heliostat's multiply-high divisors (exact for m < 2^31).
@@ -384,24 +402,14 @@ constexpr double kCullMargin = 0.01;
 template <int HT = -1>
 __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet *facets, int m, V3 &origin,
                                                DevFrame &pert_frame, V3 &normal) {
-    const V3 pos{H.pos[0], H.pos[1], H.pos[2]};
     if ((HT < 0 ? H.type : HT) == 0) {
         origin = flat_facet_origin(H, H.fr, m);
         pert_frame = H.fr;
         normal = V3{H.fr.nx, H.fr.ny, H.fr.nz};
     } else {
-        // FocusFlatRectangleHelio.cu:21-39,53-64
-        const int cpr = H.cols * H.sub_cols;
-        int row = (int)fast_div((uint32_t)m, H.fd_cpr);
-        int col = m - row * cpr;
-        const int brow = (int)fast_div((uint32_t)row, H.fd_sub_rows), bcol = (int)fast_div((uint32_t)col, H.fd_sub_cols);
-        const DevFacet &Fc = facets[H.facet_base + brow * H.cols + bcol];
-        row -= brow * H.sub_rows;
-        col -= bcol * H.sub_cols;
-        const float pl = H.pl;
-        V3 local{(col + 0.5f) * pl - H.len_col / 2, 0.0f, (row + 0.5f) * pl - H.len_row / 2};
-        local = fr_apply(Fc.fr_local, local) + V3{Fc.center[0], Fc.center[1], Fc.center[2]};
-        origin = fr_apply(H.fr_focus, local) + pos;
+        int fidx;
+        origin = focus_facet_origin(H, facets, m, fidx);
+        const DevFacet &Fc = facets[fidx];
         pert_frame = Fc.fr_pert;
         normal = V3{Fc.normal[0], Fc.normal[1], Fc.normal[2]};
     }

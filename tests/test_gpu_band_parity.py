@@ -145,6 +145,24 @@ def test_banded_facet_shared_order_ragged_blocks(device, L):
             assert bad == 0 and dep > 0
 
 
+@pytest.mark.parametrize("L", [2, 32])
+def test_banded_facet_shared_order_focus_flat(device, L):
+    """Focus-flat heliostats take the facet-shared order for any power-of-two L up to 64 (their slots
+    are their ray ids): each lane computes its micro-facet's origin and canted-facet index once and the
+    rays read them from it. The config-3 heliostats have 9016 micro-facets, not a multiple of 64, so
+    every heliostat ends in a partial block; records of both record modes equal the oracle's."""
+    wl = workloads.cylinder_field(L=L, groups=max(64, 8192 * 8 // (L * 8)), n=120)
+    scene, orc, dirs, perts, L_ = upload(device, wl)
+    assert L_ == L
+    F = scene.heliostat_info(0).n_micro
+    assert F % 64 != 0
+    ids = [0, 7, 60, 119]
+    for kernel in (N.KERNEL_BANDED, N.KERNEL_BANDED | N.KERNEL_MAP_DECISIONS):
+        bad, n, dep, _ = records_vs_oracle(device, scene, orc, dirs, perts, L, ids, kernel)
+        assert n == len(ids) * F * L
+        assert bad == 0 and dep > 0
+
+
 @pytest.mark.parametrize("kernel", [N.KERNEL_BANDED, N.KERNEL_PER_RAY])
 def test_map_path_decisions_ps10(device, kernel):
     """The decisions of the path qmc_trace runs, where a traversal the walk-free proof (dda_visit_proof)
