@@ -425,7 +425,10 @@ typedef struct qmc_schedule {
     int32_t block_threads;      /* 256, 512 or 1024 */
     int32_t workgroups_per_cu;  /* persistent workgroups per compute unit (capped at what is resident) */
     int32_t lds_flux_tile;      /* 1: accumulate each instance's map in LDS; 0: atomics into its HBM map */
-    int32_t facet_tiles;        /* 1: a wave takes a 2-D tile of micro-facets; 0: a row (facet order) */
+    int32_t facet_tiles;        /* 1: a wave takes a 2-D tile of micro-facets; 0: a row (facet order);
+                                 * 2: facet-shared, each wave computes 64 micro-facets' origins and group
+                                 * offsets once and traces their rays in steps (2 to 64 rays per facet, a
+                                 * power of two) */
 } qmc_schedule;
 
 /* Aim-point sets for batched traces: offsets holds n_sets * heliostats * 2 floats (per set, per

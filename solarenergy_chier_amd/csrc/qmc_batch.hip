@@ -52,6 +52,7 @@ struct BatchArgs {
     const int32_t *__restrict__ offsets;
     int32_t P, L;
     FastDiv fdP, fdL;
+    int32_t lLs;  // log2 L for the facet-shared order (0: it does not apply)
     double tan_sun, tan_block;  // traversal-culling cone tangents (negative: no culling)
     float rho;
     float *maps;
@@ -182,7 +183,11 @@ __global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
 
 // 1024-thread workgroups are built for two per CU (8 waves per SIMD, at most 64 VGPRs): with a 64 KB LDS
 // map each, that is the only way an LDS-accumulated instance runs at full occupancy.
-template <int RT, bool LDSMAP, bool TILES, int BS>
+// ORDER: how a heliostat's rays are dealt to lanes. 0: ray order (slot = ray id, the workgroup strides
+// over them); 1: facet tiles (slot_ray's tiled order); 2: facet-shared (L = 2^a.lLs rays per facet,
+// 2 <= L <= 64): each wave takes blocks of 64 micro-facets, lane k computes facet k's origin and group
+// offsets once, and the block's L steps of 64 rays read them from the computing lanes (ds_bpermute).
+template <int RT, bool LDSMAP, int ORDER, int BS>
 __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchArgs a) {
     extern __shared__ float map_lds[];
     __shared__ int s_next;
@@ -214,32 +219,21 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
             const int32_t *grp = a.offsets + H.offsets_base;
             const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
             const CullList *cl = a.cull ? a.cull + (size_t)ii * a.n_helio + k : nullptr;
-            const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
-            const FacetTiling tl = facet_tiling(R, C, L);
-            const int nslots = TILES ? tl.slots : H.F * L;
-#pragma nounroll
-            for (int slot = tid; slot < nslots; slot += bs) {
-                const int r = TILES ? facet_tiled_ray(tl, tile_divisor(H, tl), R, C, slot) : slot;
-                if (r < 0) continue;
-                const int m = (int)fast_div((uint32_t)r, a.fdL), j = r - m * L;
-                // both samples are read before the tests: two dependent load pairs, not a chain of four
-                const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
-                const int a2 = (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
-                const float4 s = a.pool[a1];
-                const float4 q = a.pert[a2];
+            // one ray from its facet's origin and its two pool samples (map_raytracing,
+            // RectangleReceiverRectangleGridRayTracing.cu:14-41)
+            auto trace_one = [&](V3 origin, float4 s, float4 q) {
                 V3 dir = fr_apply(sunf, V3{s.x, s.y, s.z});
-                const V3 origin = flat_facet_origin(H, fr, m);
                 int hit;
                 if ((!cl || traversal_needed(cl, 0, origin, dir, quads)) &&
                     dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
-                    continue;
+                    return;
                 const V3 normal = normalize(fr_apply(fr, V3{q.x, q.y, q.z}));
                 dir = normalize(reflect(-dir, normal));
                 if ((!cl || traversal_needed(cl, 1, origin, dir, quads)) &&
                     dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
-                    continue;
+                    return;
                 float t, u, v;
-                if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v)) continue;
+                if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v)) return;
                 const float energy = ray_energy(t, dir, normal, factor);
                 int row = (int)(u * (float)a.recv.res_y);
                 int col = (int)(v * (float)a.recv.res_x);
@@ -250,6 +244,52 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
                     atomicAdd(map_lds + addr, energy);
                 else
                     __hip_atomic_fetch_add(gmap + addr, energy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            };
+            if constexpr (ORDER == 2) {
+                const int lLs = a.lLs, fpi = 64 >> lLs;  // log2 of rays per facet, facets per step
+                const int lane = tid & 63, lfac = lane >> lLs, lray = lane & (L - 1);
+                const int nblk = (H.F + 63) >> 6;
+#pragma nounroll
+                for (int blk = __builtin_amdgcn_readfirstlane(tid >> 6); blk < nblk; blk += bs / 64) {
+                    const int f0 = blk * 64, nf = min(64, H.F - f0);
+                    V3 fo{0.0f, 0.0f, 0.0f};
+                    int g0 = 0, g1 = 0;  // offsets in [0, P]; 0 past the last facet
+                    if (lane < nf) {
+                        fo = flat_facet_origin(H, fr, f0 + lane);
+                        g0 = grp[f0 + lane];
+                        g1 = grp[fast_mod((uint32_t)(f0 + lane + 1), P, a.fdP)];
+                    }
+#pragma nounroll
+                    for (int i = 0; i * fpi < nf; ++i) {
+                        const int src = i * fpi + lfac;
+                        const V3 origin{lane_read(fo.x, src), lane_read(fo.y, src), lane_read(fo.z, src)};
+                        const uint32_t u1 = (uint32_t)(lane_read(g0, src) + lray),
+                                       u2 = (uint32_t)(lane_read(g1, src) + lray);
+                        if (src < nf) {
+                            // (g + j) mod P for g in [0, P] and j < L <= P: one conditional subtraction
+                            const float4 s = a.pool[min(u1, u1 - (uint32_t)P)];
+                            const float4 q = a.pert[min(u2, u2 - (uint32_t)P)];
+                            trace_one(origin, s, q);
+                        }
+                    }
+                }
+            } else {
+                const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
+                const FacetTiling tl = facet_tiling(R, C, L);
+                const int nslots = ORDER == 1 ? tl.slots : H.F * L;
+#pragma nounroll
+                for (int slot = tid; slot < nslots; slot += bs) {
+                    const int r = ORDER == 1 ? facet_tiled_ray(tl, tile_divisor(H, tl), R, C, slot) : slot;
+                    if (r < 0) continue;
+                    const int m = (int)fast_div((uint32_t)r, a.fdL), j = r - m * L;
+                    // both samples are read before the tests: two dependent load pairs, not a chain of four
+                    const int a1 = (int)fast_mod((uint32_t)(grp[m] + j), P, a.fdP);
+                    const int a2 =
+                        (int)fast_mod((uint32_t)(grp[fast_mod((uint32_t)(m + 1), P, a.fdP)] + j), P, a.fdP);
+                    const float4 s = a.pool[a1];
+                    const float4 q = a.pert[a2];
+                    trace_one(flat_facet_origin(H, fr, m), s, q);
+                }
             }
         }
         if (LDSMAP) {
@@ -344,49 +384,61 @@ qmc_status ensure_work(qmc_ctx *c, BatchArgs &a, int chunk) {
     return QMC_OK;
 }
 
-template <int RT, bool LDSMAP, bool TILES>
+template <int RT, bool LDSMAP, int ORDER>
 void launch_trace_bs(int bs, dim3 g, size_t lds, hipStream_t st, const BatchArgs &b) {
-    if (bs == 256) hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, TILES, 256>), g, dim3(256), lds, st, b);
-    else if (bs == 512) hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, TILES, 512>), g, dim3(512), lds, st, b);
-    else hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, TILES, 1024>), g, dim3(1024), lds, st, b);
+    if (bs == 256) hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, ORDER, 256>), g, dim3(256), lds, st, b);
+    else if (bs == 512) hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, ORDER, 512>), g, dim3(512), lds, st, b);
+    else hipLaunchKernelGGL((k_trace_batched<RT, LDSMAP, ORDER, 1024>), g, dim3(1024), lds, st, b);
 }
 
-template <int RT>
-void launch_trace_rt(bool lds_map, bool tiles, int bs, dim3 g, size_t lds, hipStream_t st, const BatchArgs &b) {
-    if (lds_map) {
-        if (tiles) launch_trace_bs<RT, true, true>(bs, g, lds, st, b);
-        else launch_trace_bs<RT, true, false>(bs, g, lds, st, b);
+template <int RT, bool LDSMAP>
+void launch_trace_order(int order, int bs, dim3 g, size_t lds, hipStream_t st, const BatchArgs &b) {
+    if (order == 2) launch_trace_bs<RT, LDSMAP, 2>(bs, g, lds, st, b);
+    else if (order == 1) launch_trace_bs<RT, LDSMAP, 1>(bs, g, lds, st, b);
+    else launch_trace_bs<RT, LDSMAP, 0>(bs, g, lds, st, b);
+}
+
+void launch_trace(int rt, bool lds_map, int order, int bs, dim3 g, size_t lds, hipStream_t st, const BatchArgs &b) {
+    if (rt == 0) {
+        if (lds_map) launch_trace_order<0, true>(order, bs, g, lds, st, b);
+        else launch_trace_order<0, false>(order, bs, g, lds, st, b);
     } else {
-        if (tiles) launch_trace_bs<RT, false, true>(bs, g, lds, st, b);
-        else launch_trace_bs<RT, false, false>(bs, g, lds, st, b);
+        if (lds_map) launch_trace_order<1, true>(order, bs, g, lds, st, b);
+        else launch_trace_order<1, false>(order, bs, g, lds, st, b);
     }
 }
 
-void launch_trace(int rt, bool lds_map, bool tiles, int bs, dim3 g, size_t lds, hipStream_t st, const BatchArgs &b) {
-    if (rt == 0) launch_trace_rt<0>(lds_map, tiles, bs, g, lds, st, b);
-    else launch_trace_rt<1>(lds_map, tiles, bs, g, lds, st, b);
-}
-
-template <int RT, bool LDSMAP, bool TILES>
+template <int RT, bool LDSMAP, int ORDER>
 hipError_t occupancy_bs(int bs, size_t lds, int &n) {
     if (bs == 256)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, TILES, 256>, 256, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, ORDER, 256>, 256, lds);
     if (bs == 512)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, TILES, 512>, 512, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, TILES, 1024>, 1024, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, ORDER, 512>, 512, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_trace_batched<RT, LDSMAP, ORDER, 1024>, 1024, lds);
 }
 
-template <int RT>
-hipError_t occupancy_rt(bool lds_map, bool tiles, int bs, size_t lds, int &n) {
-    if (lds_map) return tiles ? occupancy_bs<RT, true, true>(bs, lds, n) : occupancy_bs<RT, true, false>(bs, lds, n);
-    return tiles ? occupancy_bs<RT, false, true>(bs, lds, n) : occupancy_bs<RT, false, false>(bs, lds, n);
+template <int RT, bool LDSMAP>
+hipError_t occupancy_order(int order, int bs, size_t lds, int &n) {
+    if (order == 2) return occupancy_bs<RT, LDSMAP, 2>(bs, lds, n);
+    if (order == 1) return occupancy_bs<RT, LDSMAP, 1>(bs, lds, n);
+    return occupancy_bs<RT, LDSMAP, 0>(bs, lds, n);
 }
 
-qmc_status resident_per_cu(int rt, bool lds_map, bool tiles, int bs, size_t lds, int &n) {
-    const hipError_t e = rt == 0 ? occupancy_rt<0>(lds_map, tiles, bs, lds, n) : occupancy_rt<1>(lds_map, tiles, bs, lds, n);
+qmc_status resident_per_cu(int rt, bool lds_map, int order, int bs, size_t lds, int &n) {
+    hipError_t e;
+    if (rt == 0) e = lds_map ? occupancy_order<0, true>(order, bs, lds, n) : occupancy_order<0, false>(order, bs, lds, n);
+    else e = lds_map ? occupancy_order<1, true>(order, bs, lds, n) : occupancy_order<1, false>(order, bs, lds, n);
     if (e != hipSuccess) return fail(QMC_E_HIP, std::string("occupancy query: ") + hipGetErrorString(e));
     if (n < 1) return fail(QMC_E_UNSUPPORTED, "batched trace: the schedule does not fit on a CU");
     return QMC_OK;
+}
+
+// The facet-shared order's log2 of rays per facet, or 0 when it does not apply (L a power of two from 2
+// to 64, at most the pool size).
+int shared_order_lL(const BatchArgs &a) {
+    for (int l = 1; l <= 6; ++l)
+        if (a.L == (1 << l)) return a.P >= a.L ? l : 0;
+    return 0;
 }
 
 // Trace instances [0, n_inst) of the plan into plan.args.maps, chunk by chunk.
@@ -394,6 +446,9 @@ qmc_status launch_batched(qmc_ctx *c, const BatchPlan &plan, const qmc_schedule 
     BatchArgs a = plan.args;
     const bool lds_map = s.lds_flux_tile != 0;
     const size_t lds = lds_bytes(a, lds_map);
+    a.lLs = shared_order_lL(a);
+    if (s.facet_tiles == 2 && !a.lLs)
+        return fail(QMC_E_UNSUPPORTED, "batched trace: the facet-shared order needs 2 to 64 rays per facet (a power of two)");
     if (lds > c->max_lds) return fail(QMC_E_UNSUPPORTED, "batched trace: the flux map does not fit in LDS");
     const size_t budget = c->batch_work_budget ? c->batch_work_budget : kWorkBudget;
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n_inst, budget / work_bytes_per_instance(a)));
@@ -412,11 +467,11 @@ qmc_status launch_batched(qmc_ctx *c, const BatchPlan &plan, const qmc_schedule 
                                c->stream, b);
         // no more workgroups than can be resident: a queue-driven persistent grid
         int resident = 0;
-        const bool tiles = s.facet_tiles != 0;
-        if ((st = resident_per_cu(plan.rt, lds_map, tiles, s.block_threads, lds, resident))) return st;
+        const int order = s.facet_tiles;
+        if ((st = resident_per_cu(plan.rt, lds_map, order, s.block_threads, lds, resident))) return st;
         const int per_cu = std::max(1, std::min(s.workgroups_per_cu, resident));
         const int grid = std::max(1, std::min(b.n_inst, c->num_cus * per_cu));
-        launch_trace(plan.rt, lds_map, tiles, s.block_threads, dim3(grid), lds, c->stream, b);
+        launch_trace(plan.rt, lds_map, order, s.block_threads, dim3(grid), lds, c->stream, b);
         HIP_TRY(hipGetLastError());
     }
     return QMC_OK;
@@ -522,11 +577,12 @@ qmc_status autotune(qmc_ctx *c, const BatchPlan &plan, int n, qmc_schedule &best
     int max_grid = 0;
     for (int bs : {256, 512, 1024})
         for (int lds : {1, 0})
-            for (int tiles : {0, 1}) {
+            for (int tiles : {0, 1, 2}) {
                 const size_t bytes = lds_bytes(plan.args, lds != 0);
                 if (bytes > c->max_lds) continue;
+                if (tiles == 2 && !shared_order_lL(plan.args)) continue;
                 int resident = 0;
-                qmc_status st = resident_per_cu(plan.rt, lds != 0, tiles != 0, bs, bytes, resident);
+                qmc_status st = resident_per_cu(plan.rt, lds != 0, tiles, bs, bytes, resident);
                 if (st) return st;
                 resident = std::min(resident, 2048 / bs);  // 32 waves per CU
                 if (bytes > 0) resident = std::min<int>(resident, (int)(lds_per_cu / bytes));
@@ -620,8 +676,8 @@ extern "C" qmc_status qmc_trace_batched(qmc_ctx *c, int32_t grid, const qmc_inst
             return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: workgroups_per_cu must be 1..16");
         if (s.lds_flux_tile != 0 && s.lds_flux_tile != 1)
             return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: lds_flux_tile must be 0 or 1");
-        if (s.facet_tiles != 0 && s.facet_tiles != 1)
-            return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: facet_tiles must be 0 or 1");
+        if (s.facet_tiles < 0 || s.facet_tiles > 2)
+            return fail(QMC_E_INVALID_ARG, "qmc_trace_batched: facet_tiles must be 0, 1 or 2");
     } else if ((st = autotune(c, plan, n, s))) {
         return st;
     }

@@ -272,12 +272,6 @@ __device__ __forceinline__ void band_deposit(const TraceArgs &a, float *band, in
 // order.
 constexpr int kHtFlatShared = 2, kHtFocusShared = 3;
 
-// Lane src's value of x, for every lane (ds_bpermute; every lane of the wave must be active).
-__device__ __forceinline__ int lane_read(int x, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, x); }
-__device__ __forceinline__ float lane_read(float x, int src) {
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
-}
-
 // Persistent form of k_trace for large launches. Global float atomics run at a fixed rate (about
 // 21 G/s for scattered addresses on MI355X, against 180 G/s in LDS), and a large field deposits
 // hundreds of millions of rays into one map, so each workgroup keeps a band of map rows

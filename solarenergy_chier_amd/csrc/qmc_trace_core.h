@@ -217,6 +217,12 @@ __host__ __device__ inline DevCullShape flat_cull_shape(const DevHelio &H, const
     return s;
 }
 
+// Lane src's value of x, for every lane (ds_bpermute; every lane of the wave must be active).
+__device__ __forceinline__ int lane_read(int x, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, x); }
+__device__ __forceinline__ float lane_read(float x, int src) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+}
+
 // Load of data that no kernel of the launch writes (per-heliostat and per-instance records) through
 // the constant address space: with a wave-uniform address it becomes a scalar load into SGPRs.
 template <class T>

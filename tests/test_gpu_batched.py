@@ -56,7 +56,11 @@ def batched_setup(device):
                                       {"block_threads": 256, "workgroups_per_cu": 8, "lds_flux_tile": 0,
                                        "facet_tiles": 1},
                                       {"block_threads": 1024, "workgroups_per_cu": 1, "lds_flux_tile": 1,
-                                       "facet_tiles": 1}])
+                                       "facet_tiles": 1},
+                                      {"block_threads": 512, "workgroups_per_cu": 2, "lds_flux_tile": 1,
+                                       "facet_tiles": 2},
+                                      {"block_threads": 256, "workgroups_per_cu": 4, "lds_flux_tile": 0,
+                                       "facet_tiles": 2}])
 def test_batched_equals_looped_plain_calls(device, batched_setup, schedule):
     wl, scene, aims, sun, dni, aim = batched_setup
     inst = device.instances(sun, dni, aim)
@@ -134,6 +138,10 @@ def test_batched_errors(device):
     with pytest.raises(N.QmcError) as e:
         device.trace_batched(0, bad)
     assert e.value.status == N.QMC_E_RANGE
+    with pytest.raises(N.QmcError) as e:
+        device.trace_batched(0, inst, {"block_threads": 256, "workgroups_per_cu": 1, "lds_flux_tile": 1,
+                                       "facet_tiles": 3})
+    assert e.value.status == N.QMC_E_INVALID_ARG
     device.trace_batched(0, inst)
     assert device.batch_read(0, 1, device.flux_shape(0)).sum() > 0
 
