@@ -213,6 +213,16 @@ def test_ps10_full_map_vs_oracle(device):
     assert err < FLUX_RTOL_L1
 
 
+def test_ps10_low_sun_full_map_vs_oracle(device):
+    """The bench's field under its 15-degree sun in full (483M rays): 578 heliostats run shadow
+    traversals and a third of the rays are shadowed, through the facet-shared banded kernel's walk-free
+    decisions and deferred re-trace; the map against the oracle's double-precision map."""
+    err, st = full_map_vs_oracle(device, workloads.ps10_field(sun_deg=(180.0, 15.0)))
+    assert st["rays"] == 624 * 48384 * 16
+    assert st["unshadowed"] < 0.7 * st["rays"]
+    assert err < FLUX_RTOL_L1
+
+
 def test_config3_full_map_vs_oracle(device):
     """Config 3 in full (2650 focus-flat heliostats, 191M rays, 512x256 cylinder map) against the
     oracle's double-precision map."""
