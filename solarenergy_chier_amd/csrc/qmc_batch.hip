@@ -219,19 +219,43 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
             const int32_t *grp = a.offsets + H.offsets_base;
             const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;
             const CullList *cl = a.cull ? a.cull + (size_t)ii * a.n_helio + k : nullptr;
+            // whether the traversal of kind `kind` (0 shadow, 1 blocking) hits a quad: none when the ray
+            // misses every listed candidate; a hit when it hits one whose cell box the reference's walk
+            // is proven to reach (known_hit_proven; every cell of a heliostat's bounding-sphere box lists
+            // its quad, whatever the instance's orientation); otherwise the walk decides
+            auto hits_quad = [&](int kind, V3 o, V3 d) -> bool {
+                int hit;
+                if (cl) {
+                    const int n = ld_uniform(&cl->n[kind]);
+                    if (n == 0) return false;
+                    if (n <= kCullList) {
+                        RayHits rh;
+                        rh.n = 0;
+                        for (int i = 0; i < n; ++i) {
+                            const int id = ld_uniform(&cl->ids[kind][i]);
+                            const float4 qa = ld_uniform(quads + 3 * id), qb = ld_uniform(quads + 3 * id + 1),
+                                         qc = ld_uniform(quads + 3 * id + 2);
+                            float t, u, v;
+                            if (ray_quad(o, d, V3{qa.x, qa.y, qa.z}, V3{qb.x, qb.y, qb.z}, V3{qc.x, qc.y, qc.z}, t,
+                                         u, v)) {
+                                rh.n = 1, rh.id0 = id, rh.t0 = t;
+                                break;
+                            }
+                        }
+                        if (rh.n == 0) return false;
+                        if (known_hit_proven(rh, o, d, a.grid, hit)) return true;
+                    }
+                }
+                return dda_collision<false>(o, d, a.grid, quads, own_lo, own_hi, hit, st);
+            };
             // one ray from its facet's origin and its two pool samples (map_raytracing,
             // RectangleReceiverRectangleGridRayTracing.cu:14-41)
             auto trace_one = [&](V3 origin, float4 s, float4 q) {
                 V3 dir = fr_apply(sunf, V3{s.x, s.y, s.z});
-                int hit;
-                if ((!cl || traversal_needed(cl, 0, origin, dir, quads)) &&
-                    dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
-                    return;
+                if (hits_quad(0, origin, dir)) return;
                 const V3 normal = normalize(fr_apply(fr, V3{q.x, q.y, q.z}));
                 dir = normalize(reflect(-dir, normal));
-                if ((!cl || traversal_needed(cl, 1, origin, dir, quads)) &&
-                    dda_collision<false>(origin, dir, a.grid, quads, own_lo, own_hi, hit, st))
-                    return;
+                if (hits_quad(1, origin, dir)) return;
                 float t, u, v;
                 if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v)) return;
                 const float energy = ray_energy(t, dir, normal, factor);
