@@ -335,11 +335,24 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                                             V3{a.recv.E2[0], a.recv.E2[1], a.recv.E2[2]});
             }
             const int nf = min(64, H.F - f0);  // facets of this block
+            // the pool samples of step i's rays; every lane's group offsets are in [0, P] (0 past the last
+            // facet), so its pool indices are in range
+            auto samples = [&](int i, float4 &s, float4 &q) {
+                const int src = i * fpi + lfac;
+                // (g + j) mod P for g in [0, P] and j < L <= P: one conditional subtraction
+                const uint32_t u1 = (uint32_t)(lane_read(g0, src) + lray), u2 = (uint32_t)(lane_read(g1, src) + lray);
+                s = a.pool[min(u1, u1 - (uint32_t)P)];  // world space (k_pool_world)
+                q = a.pert[min(u2, u2 - (uint32_t)P)];
+            };
+            // step i + 1's samples are gathered before step i is traced
+            float4 s_cur, q_cur;
+            samples(0, s_cur, q_cur);
 #pragma nounroll
             for (int i = 0; i * fpi < nf; ++i) {
                 const int src = i * fpi + lfac;  // this step's facet for this lane, within the block
+                float4 s_nxt = s_cur, q_nxt = q_cur;
+                if ((i + 1) * fpi < nf) samples(i + 1, s_nxt, q_nxt);
                 const V3 origin{lane_read(fo.x, src), lane_read(fo.y, src), lane_read(fo.z, src)};
-                const int o1 = lane_read(g0, src), o2 = lane_read(g1, src);
                 const int fidx = focus ? lane_read(fx, src) : 0;
                 QuadOriginTerms ot;
                 if (RT == 0) {
@@ -349,21 +362,18 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                 }
                 if (src < nf) {
                     const long long slot = (long long)(f0 + src) * L + lray;  // = the ray id
-                    // (g + j) mod P for g in [0, P] and j < L <= P: one conditional subtraction
-                    const uint32_t u1 = (uint32_t)(o1 + lray), u2 = (uint32_t)(o2 + lray);
-                    const int a1 = (int)min(u1, u1 - (uint32_t)P), a2 = (int)min(u2, u2 - (uint32_t)P);
-                    const float4 s = a.pool[a1];  // world space (k_pool_world)
-                    const float4 q = a.pert[a2];
-                    const V3 dir{s.x, s.y, s.z};
+                    const V3 dir{s_cur.x, s_cur.y, s_cur.z};
                     RayOut o;
                     const int wave = __builtin_amdgcn_readfirstlane((f0 >> 6) * L + i);  // slot >> 6
                     const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
                     // the slope-error frame: the heliostat's, or the canted facet's (a small table)
                     const DevFrame pf = focus ? a.facets[fidx].fr_pert : H.fr;
                     trace_from<RT, false, TM>(a, H, a.wide ? a.wide + h : nullptr, wm, li, slot, lray == 0, dir,
-                                              origin, pf, q, o, st, RT == 0 ? &ot : nullptr);
+                                              origin, pf, q_cur, o, st, RT == 0 ? &ot : nullptr);
                     band_deposit<REC>(a, band, row0, rows, W, li, slot, o);
                 }
+                s_cur = s_nxt;
+                q_cur = q_nxt;
             }
             bx += nw;
             while (bx >= blocks_per_helio) {
