@@ -33,6 +33,7 @@ struct GridBuf {
 struct TraceSpan {
     int pass, grid, begin, count;
     long long max_slots;
+    bool sectors;  // entries dealt from 64 azimuth sectors
 };
 
 struct RecvBuf {

@@ -246,15 +246,112 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     }
 }
 
-// A band deposit: inside the band rows [row0, row0 + rows) an LDS atomic, else a device-scope atomic on
-// the launch's double accumulator; with REC the ray's record too.
-template <int REC>
-__device__ __forceinline__ void band_deposit(const TraceArgs &a, float *band, int row0, int rows, int W, int li,
-                                             long long r, const RayOut &o) {
+// Where a workgroup's LDS band lies in the map: rows [row0, row0 + rows), and cols columns from the
+// workgroup's own first column c0, cyclically (cols = the map's width: every column, c0 = 0). A band
+// narrower than the map (a column window) is for cylinder receivers, whose columns are azimuth: each
+// workgroup then traces one contiguous run of the launch's blocks in azimuth order and centres its
+// window on its heliostats' azimuth. sectors = 1: the list was dealt from 64 azimuth sectors in turn
+// (n / 64 heliostats each, the last n % 64 in azimuth order), and azimuth order visits it through the
+// inverse of that permutation; 0: the list is visited in its own order.
+struct BandGeom {
+    int32_t row0, rows, cols, sectors;
+};
+
+// List entry of the s-th heliostat in azimuth order, for a list dealt from 64 sectors of q heliostats
+// (entry i = (i mod 64) q + i / 64 in azimuth order for i < 64 q); q = 0: the identity. A bijection of
+// [0, n) for every 64 q <= n.
+__device__ __forceinline__ int azimuth_entry(int s, int q) { return s < 64 * q ? (s % q) * 64 + s / q : s; }
+
+// The column of a cylinder map facing heliostat H: v = azimuth / 2 pi of the heliostat seen from the
+// axis, as the receiver maps a hit point's azimuth to v (CylinderReceiver.cuh:82-86).
+__device__ __forceinline__ int azimuth_column(const DevHelio &H, const DevReceiver &R, int W) {
+    float phi = atan2f(H.pos[2] - R.pos[2], H.pos[0] - R.pos[0]);
+    if (phi < 0.0f) phi += 6.28318531f;
+    return min(W - 1, max(0, (int)(phi * (0.159154943f * (float)W))));
+}
+
+// Work claiming of a column-window launch. Its T units (64-facet blocks, or 64-slot waves) in azimuth
+// order fall into G runs, run k = [T k / G, T (k + 1) / G) for workgroup k, whose window is centred on the
+// run's azimuth; claims[k] counts the units of run k handed out (from 0). A wave claims from its own
+// run, then, once that is exhausted, from the nearest run (cyclically) that still has units, so the
+// workgroups finish together however the units' costs vary over the field. Deposits of claimed units
+// outside the claiming workgroup's window go to the device-scope accumulator as any other.
+constexpr int kMaxWindowRuns = 2048;  // 32 runs per lane of a wave's exclusion mask
+
+__device__ __forceinline__ long long run_begin(long long T, int G, int k) { return T * k / G; }
+
+// The run nearest to run k with units left according to an agent-scope read of the counters, skipping
+// the runs this lane's mask `spent` marks (bit r of lane l: run 64 r + l, found exhausted by a failed
+// claim); -1 if there is none. Wave-uniform.
+__device__ __forceinline__ int nearest_open_run(const unsigned *claims, long long T, int G, int k, unsigned spent) {
+    const int lane = (int)(threadIdx.x & 63u);
+    unsigned best = 0xffffffffu;  // (distance, run) as distance * G + run
+    for (int r = 0; 64 * r < G; ++r) {
+        const int v = 64 * r + lane;
+        if (v < G && !((spent >> r) & 1u)) {
+            const unsigned t = __hip_atomic_load(claims + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((long long)t < run_begin(T, G, v + 1) - run_begin(T, G, v)) {
+                int d = v > k ? v - k : k - v;
+                d = min(d, G - d);
+                best = min(best, (unsigned)d * (unsigned)G + (unsigned)v);
+            }
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) best = min(best, (unsigned)__shfl_xor((int)best, off));
+    best = (unsigned)__builtin_amdgcn_readfirstlane((int)best);
+    return best == 0xffffffffu ? -1 : (int)(best % (unsigned)G);
+}
+
+// Runs the units of a column-window launch for this wave: unit(u) for every unit u it claims, `own`
+// at a time from its workgroup's run and `steal` at a time from others. Every iteration either runs
+// units or marks a run exhausted for this wave, so the loop ends. A claim is a device-scope atomic
+// whose result the wave waits for (microseconds on MI355X): claims of a few thousand rays keep that
+// small against the work they hand out, and the smaller steals keep the workgroups' finishing times
+// close (config-3 field, 8 rays per facet: 8-unit own claims and 4-unit steals, 3.24 ms per call,
+// against 3.57 ms with 1-unit steals and 4.2 ms with no claims and no stealing).
+template <class Unit>
+__device__ __forceinline__ void claim_units(unsigned *claims, long long T, int G, int k, unsigned own, unsigned steal,
+                                            Unit unit) {
+    const int lane = (int)(threadIdx.x & 63u);
+    unsigned spent = 0u;
+    int v = k;
+#pragma nounroll
+    while (v >= 0) {
+        const long long u0 = run_begin(T, G, v), n = run_begin(T, G, v + 1) - u0;
+        const unsigned take = v == k ? own : steal;
+        unsigned t = 0u;
+        if (lane == 0) t = atomicAdd(claims + v, take);
+        t = (unsigned)__builtin_amdgcn_readlane((int)t, 0);
+        if ((long long)t < n) {
+            const long long e = min(n, (long long)t + take);
+#pragma nounroll
+            for (long long u = u0 + t; u < u0 + e; ++u) unit(u);
+        } else {
+            if (lane == (v & 63)) spent |= 1u << (v >> 6);
+            v = nearest_open_run(claims, T, G, k, spent);
+        }
+    }
+}
+
+// Units per claim of a column-window launch: about 4096 rays for a workgroup's own run and 2048 for a
+// steal, given the rays of one unit.
+__device__ __forceinline__ unsigned claim_size(int rays, int unit_rays) { return (unsigned)max(1, rays / unit_rays); }
+
+// A band deposit: inside the band (rows [row0, row0 + rows); with WIN, columns [c0, c0 + cols) modulo the
+// map's width W) an LDS atomic, else a device-scope atomic on the launch's double accumulator; with REC
+// the ray's record too.
+template <int REC, bool WIN>
+__device__ __forceinline__ void band_deposit(const TraceArgs &a, float *band, const BandGeom &bg, int W, int c0,
+                                             int li, long long r, const RayOut &o) {
     if (o.status == QMC_RAY_DEPOSITED) {
-        const int br = o.row - row0;
-        if ((unsigned)br < (unsigned)rows)
-            atomicAdd(band + br * W + o.col, o.energy);
+        const int br = o.row - bg.row0;
+        int bc = o.col;
+        if (WIN) {
+            bc -= c0;
+            bc += bc < 0 ? W : 0;
+        }
+        if ((unsigned)br < (unsigned)bg.rows && (!WIN || (unsigned)bc < (unsigned)bg.cols))
+            atomicAdd(band + br * (WIN ? bg.cols : W) + bc, o.energy);
         else
             unsafeAtomicAdd(a.acc + o.index, (double)o.energy);
     }
@@ -274,31 +371,62 @@ constexpr int kHtFlatShared = 2, kHtFocusShared = 3;
 
 // Persistent form of k_trace for large launches. Global float atomics run at a fixed rate (about
 // 21 G/s for scattered addresses on MI355X, against 180 G/s in LDS), and a large field deposits
-// hundreds of millions of rays into one map, so each workgroup keeps a band of map rows
-// [row0, row0 + rows) in LDS for the whole launch: deposits inside it are LDS atomics, the others
-// device-scope atomics on the launch's double accumulator. At the end each workgroup stores its band,
-// with plain stores, to its own slice of `partials`, which k_band_fold adds into the accumulator. With
-// REC every ray also writes its record at records[rec_base[li] + ray id] (parity checks of this kernel's
-// own slot, wave-mask and band indexing). DEFER as for k_trace.
+// hundreds of millions of rays into one map, so each workgroup keeps a band of the map (BandGeom) in LDS
+// for the whole launch: deposits inside it are LDS atomics, the others device-scope atomics on the
+// launch's double accumulator. At the end each workgroup stores its band, with plain stores, to its own
+// slice of `partials` (and, for a column window, its first column after all the slices, followed by
+// the runs' claim counters, zero at the launch), which
+// k_band_fold or k_window_fold adds into the accumulator. With REC every ray also writes its record at
+// records[rec_base[li] + ray id] (parity checks of this kernel's own slot, wave-mask and band indexing).
+// DEFER as for k_trace.
 //
 // Two orders of work over the same slots:
-//  * HT = 0 or 1 (the heliostats' type): workgroups take BS-slot blocks (list entry, block) round robin,
-//    one 64-slot wave each;
+//  * HT = 0 or 1 (the heliostats' type): workgroups take BS-slot blocks (list entry, block), one 64-slot
+//    wave each;
 //  * HT = kHtFlatShared or kHtFocusShared (flat heliostats with L = 2^lLs rays per facet, 16 <= L <= 64,
 //    or focus-flat ones with 2 <= L <= 64: slots that are their ray ids): each wave takes blocks of 64
-//    micro-facets (64 L slots) round robin, lane k computes facet k's origin, group offsets and (focus-
-//    flat) canted-facet index once, and the wave then runs the block's L steps of 64 slots, each lane
-//    reading its facet's values from the lane that computed them. The L rays of a facet share all of
-//    it, so this is done once per facet instead of once per ray. blocks_per_helio counts such blocks.
+//    micro-facets (64 L slots), lane k computes facet k's origin, group offsets and (focus-flat)
+//    canted-facet index once, and the wave then runs the block's L steps of 64 slots, each lane reading
+//    its facet's values from the lane that computed them. The L rays of a facet share all of it, so
+//    this is done once per facet instead of once per ray. blocks_per_helio counts such blocks.
 //    (Kernels of their own: with both loops in one kernel the scalar registers of one constrain the
 //    other.)
+// Blocks are dealt round robin over the grid's workgroups (waves), or, with a column window, claimed
+// (claim_units) from runs in azimuth order, each workgroup's window centred on its own run.
 template <int RT, int BS, int REC, int HT, bool DEFER>
-__global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int rows, int blocks_per_helio,
-                                                    float *partials, int lLs) {
+__global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, BandGeom bg, int blocks_per_helio, float *partials,
+                                                    int lLs) {
     static_assert(!DEFER || REC != 1, "deferral is for the map path");
     extern __shared__ float band[];
-    const int W = a.recv.res_x, nband = rows * W;
+    // column windows are for cylinder receivers only (RT == 1); for a rectangle receiver cols is W
+    constexpr bool WIN = RT == 1;
+    const int W = a.recv.res_x, nband = bg.rows * (WIN ? bg.cols : W);
+    const bool win = WIN && bg.cols < W;
     for (int p = threadIdx.x; p < nband; p += BS) band[p] = 0.0f;
+    // a column window: the launch's units (64-facet blocks in the facet-shared order, else 64-slot
+    // waves), this workgroup's run of them and its first column (every lane computes the same values)
+    constexpr bool shared_order = HT == kHtFlatShared || HT == kHtFocusShared;
+    const int upw = shared_order ? blocks_per_helio : blocks_per_helio * (BS / 64);  // units per list entry
+    const long long total_units = (long long)a.n_list * upw;
+    int c0 = 0, sq = 0;  // sq: heliostats per azimuth sector of the list (0: the list's own order)
+    unsigned *claims = nullptr;
+    if (win) {
+        claims = reinterpret_cast<unsigned *>(partials + (size_t)gridDim.x * nband) + gridDim.x;
+        const long long run0 = run_begin(total_units, (int)gridDim.x, (int)blockIdx.x);
+        const long long run1 = run_begin(total_units, (int)gridDim.x, (int)blockIdx.x + 1);
+        sq = bg.sectors ? a.n_list / 64 : 0;
+        if (run1 > run0) {
+            const int ha = ld_uniform(a.list + azimuth_entry((int)(run0 / upw), sq));
+            const int hb = ld_uniform(a.list + azimuth_entry((int)((run1 - 1) / upw), sq));
+            const int ca = azimuth_column(ld_uniform(a.helios + ha), a.recv, W);
+            const int cb = azimuth_column(ld_uniform(a.helios + hb), a.recv, W);
+            const int arc = cb - ca + (cb < ca ? W : 0);  // from the run's first azimuth to its last
+            c0 = ca + arc / 2 - bg.cols / 2;
+            c0 += c0 < 0 ? W : 0;
+            c0 -= c0 >= W ? W : 0;
+        }
+        c0 = __builtin_amdgcn_readfirstlane(c0);
+    }
     __syncthreads();
     RayStats st{0u, 0u, 0u};
     constexpr int TM = DEFER ? kTmDefer : kTmExact;
@@ -308,15 +436,10 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
         const int lane = (int)(threadIdx.x & 63u);
         const int lfac = lane >> lLs, lray = lane & (L - 1);
         const int P = a.P;
-        const int nw = (int)gridDim.x * (BS / 64);
-        const int gw = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BS / 64) + (threadIdx.x >> 6)));
-        int li = gw / blocks_per_helio;
-        int bx = gw - li * blocks_per_helio;
-#pragma nounroll
-        while (li < a.n_list) {
+        // one block of 64 facets of list entry li, its first facet f0
+        auto facet_block = [&](int li, int f0) {
             const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
             const DevHelio H = ld_uniform(a.helios + h);
-            const int f0 = __builtin_amdgcn_readfirstlane(bx) * 64;
             // this lane's facet of the block: origin (RectangleHelio.cu:64-79, FocusFlatRectangleHelio.cu:
             // 21-39,53-64), canted facet (focus-flat), group offsets and, for a rectangle receiver, the
             // origin's terms of the receiver test
@@ -370,25 +493,37 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                     const DevFrame pf = focus ? a.facets[fidx].fr_pert : H.fr;
                     trace_from<RT, false, TM>(a, H, a.wide ? a.wide + h : nullptr, wm, li, slot, lray == 0, dir,
                                               origin, pf, q_cur, o, st, RT == 0 ? &ot : nullptr);
-                    band_deposit<REC>(a, band, row0, rows, W, li, slot, o);
+                    band_deposit<REC, WIN>(a, band, bg, W, c0, li, slot, o);
                 }
                 s_cur = s_nxt;
                 q_cur = q_nxt;
             }
-            bx += nw;
-            while (bx >= blocks_per_helio) {
-                bx -= blocks_per_helio;
-                ++li;
+        };
+        if (win) {
+            claim_units(claims, total_units, (int)gridDim.x, (int)blockIdx.x, claim_size(4096, 64 * L),
+                        claim_size(2048, 64 * L), [&](long long u) {
+                            const int s = (int)(u / upw);
+                            facet_block(azimuth_entry(s, sq), (int)(u - (long long)s * upw) * 64);
+                        });
+        } else {
+            const int nw = (int)gridDim.x * (BS / 64);
+            const int gw = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BS / 64) + (threadIdx.x >> 6)));
+            int li = gw / blocks_per_helio;
+            int bx = gw - li * blocks_per_helio;
+#pragma nounroll
+            while (li < a.n_list) {
+                facet_block(li, __builtin_amdgcn_readfirstlane(bx) * 64);
+                bx += nw;
+                while (bx >= blocks_per_helio) {
+                    bx -= blocks_per_helio;
+                    ++li;
+                }
             }
         }
     } else {
-        // block (li, bx) of the launch, advanced by gridDim.x blocks per step; the heliostat id is made
-        // explicitly wave-uniform so that its record is read with scalar loads, as in k_trace
-        int li = (int)(blockIdx.x / (unsigned)blocks_per_helio);
-        int bx = (int)blockIdx.x - li * blocks_per_helio;
-#pragma nounroll
-        while (li < a.n_list) {
-            // the launch writes none of these records, so they are read with scalar loads
+        // block (li, bx) of the launch; the heliostat id is made explicitly wave-uniform so that its record
+        // is read with scalar loads, as in k_trace
+        auto slot_block = [&](int li, int bx) {
             const int h = ld_uniform(a.list + __builtin_amdgcn_readfirstlane(li));
             const DevHelio H = ld_uniform(a.helios + h);
             const long long slot = (long long)__builtin_amdgcn_readfirstlane(bx) * BS + threadIdx.x;
@@ -398,18 +533,48 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, int row0, int ro
                 const int wave = __builtin_amdgcn_readfirstlane(bx * (BS / 64) + (int)(threadIdx.x >> 6));
                 const uint64_t *wm = a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + wave) : nullptr;
                 trace_ray<RT, false, TM, HT>(a, H, a.wide ? a.wide + h : nullptr, wm, r, li, slot, o, st);
-                band_deposit<REC>(a, band, row0, rows, W, li, r, o);
+                band_deposit<REC, WIN>(a, band, bg, W, c0, li, r, o);
             }
-            bx += (int)gridDim.x;
-            while (bx >= blocks_per_helio) {
-                bx -= blocks_per_helio;
-                ++li;
+        };
+        if (win) {
+            // one 64-slot wave of list entry li (wave index j of its slots)
+            claim_units(claims, total_units, (int)gridDim.x, (int)blockIdx.x, claim_size(4096, 64),
+                        claim_size(2048, 64), [&](long long u) {
+                            const int s = __builtin_amdgcn_readfirstlane((int)(u / upw));
+                            const int j = __builtin_amdgcn_readfirstlane((int)(u - (long long)s * upw));
+                            const int li = azimuth_entry(s, sq);
+                            const int h = ld_uniform(a.list + li);
+                            const DevHelio H = ld_uniform(a.helios + h);
+                            const long long slot = (long long)j * 64 + (threadIdx.x & 63u);
+                            const long long r = slot_ray<HT>(H, a.L, slot);
+                            if (r >= 0) {
+                                RayOut o;
+                                const uint64_t *wm =
+                                    a.wide ? a.wmask + 2 * ((size_t)li * a.wstride + j) : nullptr;
+                                trace_ray<RT, false, TM, HT>(a, H, a.wide ? a.wide + h : nullptr, wm, r, li, slot, o,
+                                                             st);
+                                band_deposit<REC, WIN>(a, band, bg, W, c0, li, r, o);
+                            }
+                        });
+        } else {
+            // advanced by gridDim.x blocks per step
+            int li = (int)(blockIdx.x / (unsigned)blocks_per_helio);
+            int bx = (int)blockIdx.x - li * blocks_per_helio;
+#pragma nounroll
+            while (li < a.n_list) {
+                slot_block(li, bx);
+                bx += (int)gridDim.x;
+                while (bx >= blocks_per_helio) {
+                    bx -= blocks_per_helio;
+                    ++li;
+                }
             }
         }
     }
     __syncthreads();
     float *mine = partials + (size_t)blockIdx.x * nband;
     for (int p = threadIdx.x; p < nband; p += BS) mine[p] = band[p];
+    if (win && threadIdx.x == 0) reinterpret_cast<int *>(partials + (size_t)gridDim.x * nband)[blockIdx.x] = c0;
 }
 
 // Adds the workgroups' bands (in double) into rows [row0, row0 + nband / W) of the accumulator;
@@ -423,6 +588,48 @@ __global__ __launch_bounds__(256) void k_band_fold(double *acc, const float *par
     double s = 0.0;
     for (int w = w0; w < w1; ++w) s += (double)partials[(size_t)w * nband + p];
     if (s != 0.0) unsafeAtomicAdd(acc + offset + p, s);
+}
+
+// Sets n counters to zero (a kernel rather than a memset node, for callers that capture the launch).
+__global__ __launch_bounds__(256) void k_zero_counters(unsigned *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Adds the workgroups' column windows (in double) into rows [row0, row0 + rows) of the accumulator.
+// Window w holds rows x cols floats from column c0[w] cyclically; the n_parts (<= kMaxWindowRuns) first
+// columns follow the windows. A workgroup of 256 threads takes 32 pixels of a row span, its 8 groups
+// of 32 threads each summing one slice of the windows in window order (a window that does not cover
+// the column adds nothing), and the slices' sums are added in slice order.
+__global__ __launch_bounds__(256) void k_window_fold(double *acc, const float *partials, int n_parts, int rows,
+                                                     int cols, int W, int row0) {
+    __shared__ int s_c0[kMaxWindowRuns];
+    __shared__ double s_sum[8][32];
+    const size_t nband = (size_t)rows * cols;
+    const int *c0 = reinterpret_cast<const int *>(partials + (size_t)n_parts * nband);
+    for (int w = threadIdx.x; w < n_parts; w += blockDim.x) s_c0[w] = c0[w];
+    __syncthreads();
+    const int lp = (int)(threadIdx.x & 31u), slice = (int)(threadIdx.x >> 5);
+    const int p = blockIdx.x * 32 + lp;
+    const int per = (n_parts + 7) / 8, w0 = slice * per, w1 = min(n_parts, w0 + per);
+    double s = 0.0;
+    if (p < rows * W) {
+        const int r = p / W, c = p - r * W;
+        const float *part = partials + (size_t)r * cols;
+#pragma unroll 8
+        for (int w = w0; w < w1; ++w) {
+            int bc = c - s_c0[w];
+            bc += bc < 0 ? W : 0;
+            if (bc < cols) s += (double)part[(size_t)w * nband + bc];
+        }
+    }
+    s_sum[slice][lp] = s;
+    __syncthreads();
+    if (slice == 0 && p < rows * W) {
+        double t = 0.0;
+        for (int k = 0; k < 8; ++k) t += s_sum[k][lp];
+        if (t != 0.0) acc[(size_t)row0 * W + p] += t;
+    }
 }
 
 // One ray that the deferring pass (kTmDefer) left to k_trace_deferred, replayed (kTmReplay) with
@@ -1350,27 +1557,27 @@ constexpr int kBandBS = 1024;
 // The trace kernels for a receiver type RT, record mode REC, the grid's heliostat type ht and whether
 // the launch defers (a.defer_cap > 0: the map path with culling).
 template <int RT, int REC, bool DEFER>
-void launch_band_d(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph,
+void launch_band_d(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, const BandGeom &bg, int bph,
                    float *parts, int lLs) {
     if (ht == 0 && lLs)
-        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, kHtFlatShared, DEFER>), g, dim3(kBandBS), lds, st, a, row0,
-                           rows, bph, parts, lLs);
+        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, kHtFlatShared, DEFER>), g, dim3(kBandBS), lds, st, a, bg, bph,
+                           parts, lLs);
     else if (lLs)
-        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, kHtFocusShared, DEFER>), g, dim3(kBandBS), lds, st, a, row0,
-                           rows, bph, parts, lLs);
+        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, kHtFocusShared, DEFER>), g, dim3(kBandBS), lds, st, a, bg, bph,
+                           parts, lLs);
     else if (ht == 0)
-        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 0, DEFER>), g, dim3(kBandBS), lds, st, a, row0, rows, bph,
-                           parts, 0);
+        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 0, DEFER>), g, dim3(kBandBS), lds, st, a, bg, bph, parts,
+                           0);
     else
-        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 1, DEFER>), g, dim3(kBandBS), lds, st, a, row0, rows, bph,
-                           parts, 0);
+        hipLaunchKernelGGL((k_trace_band<RT, kBandBS, REC, 1, DEFER>), g, dim3(kBandBS), lds, st, a, bg, bph, parts,
+                           0);
 }
 
 template <int RT, int REC>
-void launch_band(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, int row0, int rows, int bph,
+void launch_band(int ht, dim3 g, size_t lds, hipStream_t st, const TraceArgs &a, const BandGeom &bg, int bph,
                  float *parts, int lLs) {
-    if (REC != 1 && a.defer_cap > 0) launch_band_d<RT, REC, REC != 1>(ht, g, lds, st, a, row0, rows, bph, parts, lLs);
-    else launch_band_d<RT, REC, false>(ht, g, lds, st, a, row0, rows, bph, parts, lLs);
+    if (REC != 1 && a.defer_cap > 0) launch_band_d<RT, REC, REC != 1>(ht, g, lds, st, a, bg, bph, parts, lLs);
+    else launch_band_d<RT, REC, false>(ht, g, lds, st, a, bg, bph, parts, lLs);
 }
 
 template <int RT, int REC, bool D>
@@ -1437,11 +1644,12 @@ qmc_status launch_deferred(qmc_ctx *c, const TraceArgs &a, int ht, int rec_mode)
 // The banded persistent trace when it pays (or when forced, for parity checks of this kernel): one
 // workgroup per CU with the whole LDS (4 waves per SIMD). Device-scope atomics bound the one-lane-per-ray
 // kernel on large fields (about 24 G/s on MI355X), so the band is used where it holds at least a quarter
-// of the map's rows (measured: the config-3 cylinder field, 80 of 256 rows taking 48.5% of its deposits,
-// 6.3 -> 4.7 ms per call) and where every workgroup deposits many times its band (its band store and
-// fold are then a small part of the launch).
+// of the map's rows, or, on a cylinder, a column window at least a quarter of the azimuth (measured on the
+// config-3 cylinder field: 80 of 256 rows taking 48.5% of its deposits, 6.3 -> 4.7 ms per call), and
+// where every workgroup deposits many times its band (its band store and fold are then a small part of
+// the launch). sectors: the launch's list was dealt from 64 azimuth sectors (interleave_by_azimuth).
 qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, long long max_slots, TraceBufs &b,
-                           bool force, int rec_mode, int ht, bool &launched) {
+                           bool force, int rec_mode, int ht, bool sectors, bool &launched) {
     launched = false;
     const int per_cu = 1;
     if (qmc_status st = device_limits(c)) return st;
@@ -1453,8 +1661,19 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
         if (force) return fail(QMC_E_UNSUPPORTED, "banded trace: one map row does not fit in LDS");
         return QMC_OK;
     }
-    if (!force && 4 * rows < Hh) return QMC_OK;
-    const size_t lds = sizeof(float) * (size_t)rows * W;
+    // a cylinder map whose rows do not all fit takes a full-height column window instead where it spans
+    // at least a quarter of the azimuth: the heliostats of a workgroup's run face a few degrees of it, so
+    // the window catches most of their deposits (config-3 field: 160 of 512 columns, 91% of the deposits,
+    // against 48% for 80 whole rows)
+    BandGeom bg{(Hh - rows) / 2, rows, W, 0};  // row band: flux concentrates around the receiver centre
+    if (a.recv.type == 1 && rows < Hh && c->num_cus * per_cu <= kMaxWindowRuns) {
+        const int cols = (int)std::min<size_t>((size_t)W, lds_budget / (sizeof(float) * (size_t)Hh));
+        if (4 * cols >= W) bg = BandGeom{0, Hh, cols, sectors ? 1 : 0};
+    }
+    const bool window = bg.cols < W;
+    if (!force && !window && 4 * rows < Hh) return QMC_OK;
+    const long long nband = (long long)bg.rows * bg.cols;
+    const size_t lds = sizeof(float) * (size_t)nband;
     // flat heliostats with 16, 32 or 64 rays per micro-facet and focus-flat ones with 2 to 64 (a power of
     // two) take the facet-shared order (k_trace_band): blocks of 64 facets per wave; their slots are their
     // ray ids (slot_ray)
@@ -1471,41 +1690,51 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
     const int nwg = c->num_cus * std::min(per_cu, resident);
     const long long bph = lLs ? (max_slots / a.L + 63) / 64 : (max_slots + kBandBS - 1) / kBandBS;
     const long long total = lLs ? bph * 64 * a.L * a.n_list : bph * kBandBS * a.n_list;
-    const long long nband = (long long)rows * W;
     if (!force && total < 8 * (long long)nwg * nband) return QMC_OK;
-    if ((size_t)nwg * nband > b.band_cap) {
+    // the workgroups' bands, then (column windows) their first columns and their runs' claim counters, one
+    // int in a float's slot each
+    const size_t need = (size_t)nwg * nband + (window ? 2 * (size_t)nwg : 0);
+    if (need > b.band_cap) {
         if (b.fixed) {
             if (force) return fail(QMC_E_RANGE, "banded trace: the workgroup bands exceed the scratch");
             return QMC_OK;
         }
         HIP_TRY(hipStreamSynchronize(c->stream));  // earlier launches may still write the old bands
-        qmc_status st = dev_alloc(c->d_band_partials, (size_t)nwg * nband);
+        qmc_status st = dev_alloc(c->d_band_partials, need);
         if (st) {
             c->band_cap = 0;
             return st;
         }
-        c->band_cap = (size_t)nwg * nband;
+        c->band_cap = need;
         b.band = c->d_band_partials;
         b.band_cap = c->band_cap;
     }
-    const int row0 = (Hh - rows) / 2;  // flux concentrates around the aim point, the receiver centre
+    if (window) {
+        hipLaunchKernelGGL(k_zero_counters, dim3((unsigned)((nwg + 255) / 256)), dim3(256), 0, c->stream,
+                           reinterpret_cast<unsigned *>(b.band + (size_t)nwg * nband) + nwg, nwg);
+        HIP_TRY(hipGetLastError());
+    }
     const bool mark = !rec;
     if (mark && c->kev_before >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_before], c->stream));
     if (rec && rec_mode == 2) {
-        if (a.recv.type == 0) launch_band<0, 2>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
-        else launch_band<1, 2>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
+        if (a.recv.type == 0) launch_band<0, 2>(ht, dim3(nwg), lds, c->stream, a, bg, (int)bph, b.band, lLs);
+        else launch_band<1, 2>(ht, dim3(nwg), lds, c->stream, a, bg, (int)bph, b.band, lLs);
     } else if (rec) {
-        if (a.recv.type == 0) launch_band<0, 1>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
-        else launch_band<1, 1>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
+        if (a.recv.type == 0) launch_band<0, 1>(ht, dim3(nwg), lds, c->stream, a, bg, (int)bph, b.band, lLs);
+        else launch_band<1, 1>(ht, dim3(nwg), lds, c->stream, a, bg, (int)bph, b.band, lLs);
     } else {
-        if (a.recv.type == 0) launch_band<0, 0>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
-        else launch_band<1, 0>(ht, dim3(nwg), lds, c->stream, a, row0, rows, (int)bph, b.band, lLs);
+        if (a.recv.type == 0) launch_band<0, 0>(ht, dim3(nwg), lds, c->stream, a, bg, (int)bph, b.band, lLs);
+        else launch_band<1, 0>(ht, dim3(nwg), lds, c->stream, a, bg, (int)bph, b.band, lLs);
     }
     HIP_TRY(hipGetLastError());
     if (qmc_status st = launch_deferred(c, a, ht, rec_mode)) return st;
     if (mark && c->kev_after >= 0) HIP_TRY(hipEventRecord(c->ev[c->kev_after], c->stream));
-    hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256), 8), dim3(256), 0, c->stream, rb.acc, b.band,
-                       nwg, (int)nband, row0 * W);
+    if (window)
+        hipLaunchKernelGGL(k_window_fold, dim3((unsigned)(((size_t)Hh * W + 31) / 32)), dim3(256), 0, c->stream,
+                           rb.acc, b.band, nwg, Hh, bg.cols, W, 0);
+    else
+        hipLaunchKernelGGL(k_band_fold, dim3((unsigned)((nband + 255) / 256), 8), dim3(256), 0, c->stream, rb.acc,
+                           b.band, nwg, (int)nband, bg.row0 * W);
     HIP_TRY(hipGetLastError());
     launched = true;
     return QMC_OK;
@@ -1514,9 +1743,10 @@ qmc_status try_band_launch(qmc_ctx *c, const RecvBuf &rb, const TraceArgs &a, lo
 // One launch over n list entries of one grid (d_list: their heliostat ids): culling kernels, the trace
 // kernel (QMC_KERNEL_*, with QMC_KERNEL_MAP_DECISIONS: records of the map path's decisions) and the
 // fold of the launch's accumulator into the map. The wave masks of the launch must fit in b (the
-// caller sizes or splits launches).
+// caller sizes or splits launches). sectors: the list was dealt from 64 azimuth sectors
+// (interleave_by_azimuth over exactly these n entries).
 qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long long max_slots, qmc_ray_record *rec,
-                        const long long *rec_base, StatsAcc *stats, int kernel, TraceBufs &b) {
+                        const long long *rec_base, StatsAcc *stats, int kernel, TraceBufs &b, bool sectors) {
     const GridBuf &gb = c->grids[grid];
     const RecvBuf &rb = c->recvs[gb.receiver];
     TraceArgs a;
@@ -1571,7 +1801,8 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
     const int npix = rb.w * rb.h;
     if (!stats && kernel != QMC_KERNEL_PER_RAY) {
         bool launched = false;
-        if ((st = try_band_launch(c, rb, a, max_slots, b, kernel == QMC_KERNEL_BANDED, rec_mode, gb.htype, launched)))
+        if ((st = try_band_launch(c, rb, a, max_slots, b, kernel == QMC_KERNEL_BANDED, rec_mode, gb.htype, sectors,
+                                  launched)))
             return st;
         if (launched) {
             hipLaunchKernelGGL(k_acc_fold, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, c->stream, rb.flux,
@@ -1757,11 +1988,12 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
         int e = i + 1;
         while (e < n && pass[order[e]] == p && c->h_helios[sorted[e]].grid == g) ++e;
         const DevReceiver &R = c->recvs[c->grids[g].receiver].r;
-        if (!rec && !stats && R.type == 1) interleave_by_azimuth(c, R, sorted.data() + i, e - i);
+        const bool sectors = !rec && !stats && R.type == 1;
+        if (sectors) interleave_by_azimuth(c, R, sorted.data() + i, e - i);
         long long max_slots = 0;
         for (int k = i; k < e; ++k) max_slots = std::max(max_slots, heliostat_slots(c, c->h_helios[sorted[k]]));
         if (max_slots > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
-        spans.push_back(TraceSpan{p, g, i, e - i, max_slots});
+        spans.push_back(TraceSpan{p, g, i, e - i, max_slots, sectors});
         wmask_need = std::max(wmask_need, 2 * (size_t)(e - i) * (size_t)wave_stride(max_slots));
         if (deferring) defer_need = std::max(defer_need, std::max<size_t>(65536, (size_t)(e - i) * (size_t)max_slots / 256));
         i = e;
@@ -1804,7 +2036,8 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
         for (int k = 0; k < sp.count; k += chunk) {
             const int m = std::min(chunk, sp.count - k);
             if ((st = launch_trace(c, sp.grid, b.list + sp.begin + k, m, sp.max_slots, rec,
-                                   rec ? c->d_rec_base + sp.begin + k : nullptr, stats, kernel, b)))
+                                   rec ? c->d_rec_base + sp.begin + k : nullptr, stats, kernel, b,
+                                   sp.sectors && m == sp.count)))
                 return st;
         }
     }

@@ -79,7 +79,8 @@ extern "C" qmc_status qmc_trace_noalloc(qmc_ctx *c, const int32_t *ids, int32_t 
     b.list = reinterpret_cast<int32_t *>(take(sizeof(int32_t) * (size_t)kMaxList));
     b.cull = reinterpret_cast<CullList *>(take(sizeof(CullList) * nh));
     b.wide = reinterpret_cast<WideList *>(take(sizeof(WideList) * nh));
-    b.band_cap = (size_t)c->num_cus * (c->max_lds / sizeof(float));
+    // bands, and for column windows their first columns and claim counters
+    b.band_cap = (size_t)c->num_cus * (c->max_lds / sizeof(float) + 2);
     b.band = reinterpret_cast<float *>(take(sizeof(float) * b.band_cap));
     b.defer_n = reinterpret_cast<unsigned *>(take(sizeof(unsigned)));
     b.defer_cap = kDeferEntries;
