@@ -123,7 +123,10 @@ struct qmc_ctx {
     // allocates no host memory either)
     std::vector<int32_t> h_pass, h_occ, h_order, h_sorted, h_tmp;
     std::vector<long long> h_sorted_base;
-    std::vector<std::pair<double, int32_t>> h_keys;
+    // each grid's heliostats in azimuth order about its receiver's axis (grid g from h_az_begin[g]),
+    // and a zeroed mark per heliostat (set and cleared within a call)
+    std::vector<int32_t> h_az_order, h_az_begin;
+    std::vector<uint8_t> h_mark;
     std::vector<qmc::TraceSpan> h_spans;
     // adjoint
     void *adj = nullptr;

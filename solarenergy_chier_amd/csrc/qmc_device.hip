@@ -1066,6 +1066,36 @@ std::vector<float4> pack_quads(const HostGrid &hg) {
     return quads;
 }
 
+// Each grid's heliostats in azimuth order about the axis of the grid's receiver (ties by id), grid after
+// grid (h_az_order, grid g from h_az_begin[g]): kept from qmc_upload and qmc_update, so that a trace
+// call only filters it (interleave_by_azimuth). Host memory allocated here, not in trace calls.
+void azimuth_order(qmc_ctx *c) {
+    const int nh = (int)c->h_helios.size(), ng = (int)c->grids.size();
+    struct Key {
+        int grid;
+        double az;
+        int32_t id;
+    };
+    std::vector<Key> key((size_t)nh);
+    for (int h = 0; h < nh; ++h) {
+        const DevHelio &H = c->h_helios[h];
+        const DevReceiver &R = c->recvs[c->grids[H.grid].receiver].r;
+        key[(size_t)h] = {H.grid, std::atan2((double)H.pos[2] - R.pos[2], (double)H.pos[0] - R.pos[0]), h};
+    }
+    std::sort(key.begin(), key.end(), [](const Key &x, const Key &y) {
+        if (x.grid != y.grid) return x.grid < y.grid;
+        return x.az != y.az ? x.az < y.az : x.id < y.id;
+    });
+    c->h_az_order.resize((size_t)nh);
+    c->h_az_begin.assign((size_t)ng + 1, 0);
+    for (int i = 0; i < nh; ++i) {
+        c->h_az_order[(size_t)i] = key[(size_t)i].id;
+        ++c->h_az_begin[(size_t)key[(size_t)i].grid + 1];
+    }
+    for (int g = 0; g < ng; ++g) c->h_az_begin[(size_t)g + 1] += c->h_az_begin[(size_t)g];
+    c->h_mark.assign((size_t)nh, 0);
+}
+
 }  // namespace
 
 extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
@@ -1220,7 +1250,6 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         c->h_order.reserve(nres);
         c->h_sorted.reserve(nres);
         c->h_sorted_base.reserve(nres);
-        c->h_keys.reserve(nres);
         c->h_tmp.reserve(nres);
         c->h_spans.reserve(nres);
     }
@@ -1243,6 +1272,7 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         c->buie.gamma = abcig[4];
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
+    azimuth_order(c);
     c->uploaded = true;
     c->offsets_ready = false;
     if (c->pool_ready && c->P != s->cfg.num_sunshape_groups * s->cfg.num_sunshape_lights_per_group)
@@ -1280,6 +1310,7 @@ extern "C" qmc_status qmc_update(qmc_ctx *c, const qmc_scene *s) {
     c->sun = to_dev(make_frame(-sun, false));
     c->pool_world_ready = false;
     c->h_helios = helios;
+    azimuth_order(c);
     aim_points_f64(*s, c->h_aim64);
     HIP_TRY(hipMemcpyAsync(c->d_helios, helios.data(), helios.size() * sizeof(DevHelio), hipMemcpyHostToDevice,
                            c->stream));
@@ -1839,27 +1870,33 @@ qmc_status launch_trace(qmc_ctx *c, int grid, const int32_t *d_list, int n, long
 }
 
 // Reorders a cylinder grid's heliostats for a plain trace. A cylinder map is lit around the azimuth
-// facing each heliostat, and the trace has only a few heliostats of its list in flight at a time:
-// taken one from each of 64 azimuth sectors in turn, those in flight light different columns, so
-// their atomic adds meet on the same pixels less often (2650-heliostat 512x256 field: 6.92 ms against
-// 7.47 in layout order and 7.76 sorted by azimuth). Only the order of the double-precision
-// accumulation changes.
-void interleave_by_azimuth(qmc_ctx *c, const DevReceiver &R, int32_t *ids, int n) {
+// facing each heliostat, and a trace kernel that deals heliostats round robin has only a few of them in
+// flight at a time: taken one from each of 64 azimuth sectors in turn, those in flight light different
+// columns, so their atomic adds meet on the same pixels less often (2650-heliostat 512x256 field: 6.92
+// ms against 7.47 in layout order and 7.76 sorted by azimuth). The banded trace visits the list in
+// azimuth order through the inverse permutation (azimuth_entry). ids: n distinct heliostats of grid g,
+// put in azimuth order by filtering the grid's (azimuth_order) — ties in id order — then dealt.
+void interleave_by_azimuth(qmc_ctx *c, int g, int32_t *ids, int n) {
     constexpr int kSectors = 64;
-    std::vector<std::pair<double, int32_t>> &key = c->h_keys;
-    std::vector<int32_t> &was = c->h_tmp;
-    key.resize((size_t)n);
-    was.assign(ids, ids + n);
-    for (int i = 0; i < n; ++i) {
-        const DevHelio &H = c->h_helios[ids[i]];
-        key[(size_t)i] = {std::atan2((double)H.pos[2] - R.pos[2], (double)H.pos[0] - R.pos[0]), i};
+    std::vector<uint8_t> &mark = c->h_mark;
+    std::vector<int32_t> &az = c->h_tmp;
+    az.resize((size_t)n);
+    for (int i = 0; i < n; ++i) mark[(size_t)ids[i]] = 1;
+    int k = 0;
+    for (int j = c->h_az_begin[(size_t)g]; j < c->h_az_begin[(size_t)g + 1]; ++j) {
+        const int32_t h = c->h_az_order[(size_t)j];
+        if (mark[(size_t)h]) {
+            mark[(size_t)h] = 0;
+            az[(size_t)k++] = h;
+        }
     }
-    // azimuth order, task order among equal azimuths (std::sort: no temporary buffer)
-    std::sort(key.begin(), key.end());
+    if (k != n) {  // not n distinct heliostats of the grid: keep the list as it is
+        for (int i = 0; i < n; ++i) mark[(size_t)ids[i]] = 0;
+        return;
+    }
     const int q = n / kSectors;  // heliostats per sector; the last n % kSectors follow in azimuth order
     for (int i = 0; i < n; ++i)
-        ids[i] = was[(size_t)(i < kSectors * q ? key[(size_t)((i % kSectors) * q + i / kSectors)].second
-                                               : key[(size_t)i].second)];
+        ids[i] = az[(size_t)(i < kSectors * q ? (i % kSectors) * q + i / kSectors : i)];
 }
 
 // Ctx-owned buffers of a trace call of n entries (grown as needed; record bases when rec).
@@ -1961,16 +1998,20 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
     occ.assign((size_t)nh, 0);
     pass.resize((size_t)n);
     order.resize((size_t)n);
+    bool in_order = true;  // every heliostat once, grids in order: the list is already in that order
     for (int i = 0; i < n; ++i) {
         pass[i] = occ[ids[i]]++;
         order[i] = i;
+        in_order = in_order && pass[i] == 0 &&
+                   (i == 0 || c->h_helios[ids[i]].grid >= c->h_helios[ids[i - 1]].grid);
     }
     // entries ordered by (pass, grid, task order) (std::sort: no temporary buffer)
-    std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-        if (pass[x] != pass[y]) return pass[x] < pass[y];
-        const int gx = c->h_helios[ids[x]].grid, gy = c->h_helios[ids[y]].grid;
-        return gx != gy ? gx < gy : x < y;
-    });
+    if (!in_order)
+        std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            if (pass[x] != pass[y]) return pass[x] < pass[y];
+            const int gx = c->h_helios[ids[x]].grid, gy = c->h_helios[ids[y]].grid;
+            return gx != gy ? gx < gy : x < y;
+        });
     sorted.resize((size_t)n);
     sbase.resize(rec ? (size_t)n : 0);
     spans.clear();
@@ -1989,7 +2030,7 @@ qmc_status trace_common(qmc_ctx *c, const int32_t *ids, int n, qmc_ray_record *r
         while (e < n && pass[order[e]] == p && c->h_helios[sorted[e]].grid == g) ++e;
         const DevReceiver &R = c->recvs[c->grids[g].receiver].r;
         const bool sectors = !rec && !stats && R.type == 1;
-        if (sectors) interleave_by_azimuth(c, R, sorted.data() + i, e - i);
+        if (sectors) interleave_by_azimuth(c, g, sorted.data() + i, e - i);
         long long max_slots = 0;
         for (int k = i; k < e; ++k) max_slots = std::max(max_slots, heliostat_slots(c, c->h_helios[sorted[k]]));
         if (max_slots > 0x7fffffffLL) return fail(QMC_E_UNSUPPORTED, "trace: too many rays per heliostat");
