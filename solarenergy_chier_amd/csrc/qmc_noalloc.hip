@@ -8,8 +8,10 @@
 // grows ctx-owned buffers for them; this entry point carves them from one file-scope array of
 // kScratchBytes instead, of which every device holds its own copy until the process ends. A launch
 // whose wave masks do not fit in what is left of it is split into launches over fewer heliostats.
-// Nothing in the array has to start at zero (each kernel writes what it later reads), so nothing is
-// cleared. Calls that overlap on one device (two contexts on two streams) share the array and race.
+// Nothing in the array has to start at zero: each kernel writes what it later reads, and the counts
+// that must start at zero in every launch (the deferral queue's, set by k_cull, and the column-window
+// trace's claim counters, set by k_zero_counters) are set by kernels of that launch, not by memsets.
+// Calls that overlap on one device (two contexts on two streams) share the array and race.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
