@@ -114,11 +114,21 @@ def test_banded_kernel_records_low_sun(device, sun):
 
 
 def test_banded_kernel_records_cylinder_field(device):
-    """Config-3 field (focus-flat heliostats, cylinder receiver; 80-row band of the 512x256 map) through
-    the banded kernel, every sixteenth heliostat."""
+    """Config-3 field (focus-flat heliostats, cylinder receiver; full-height windows of 160 of the 512x256
+    map's columns, one per workgroup) through the banded kernel, every sixteenth heliostat."""
     scene, orc, dirs, perts, L = upload(device, workloads.cylinder_field())
     bad, _, dep, _ = records_vs_oracle(device, scene, orc, dirs, perts, L, list(range(0, 2650, 16)),
                                        N.KERNEL_BANDED)
+    assert bad == 0 and dep > 0
+
+
+@pytest.mark.parametrize("ids", [[1800], [7, 1320, 2649]])
+def test_banded_column_windows_few_blocks(device, ids):
+    """Column windows with fewer 64-facet blocks than workgroups (141 per config-3 heliostat, 256
+    workgroups): with one heliostat most runs are empty and their waves only steal; with three, runs
+    hold one or two blocks. Every ray's record equals the oracle's."""
+    scene, orc, dirs, perts, L = upload(device, workloads.cylinder_field())
+    bad, _, dep, _ = records_vs_oracle(device, scene, orc, dirs, perts, L, ids, N.KERNEL_BANDED)
     assert bad == 0 and dep > 0
 
 
