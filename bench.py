@@ -14,6 +14,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 from pathlib import Path
@@ -486,8 +487,11 @@ def bench_adjoint(dev, steps: int = 64) -> dict:
     dev.adjoint_forward(0, buf)
     dev.flux_loss(buf, npix, T, PK, LAM)
     dev.adjoint_backward(scene, 0)
-    reps = 5
-    t_plain = t_fwd = t_bwd = 0.0
+    # medians over the repetitions: the backward returns host arrays, so its span includes a stream
+    # synchronisation, which the host's scheduler now and then stretches by milliseconds (one call in
+    # about twelve on the measured boxes; the kernels themselves vary by a few percent)
+    reps = 9
+    tp, tf, tb = [], [], []
     for _ in range(reps):
         dev.sync()
         dev.event_record(6)
@@ -500,10 +504,10 @@ def bench_adjoint(dev, steps: int = 64) -> dict:
         ga, gs = dev.adjoint_backward(scene, 0)
         dev.event_record(10)
         dev.sync()
-        t_plain += dev.event_elapsed_ms(6, 7)
-        t_fwd += dev.event_elapsed_ms(7, 8)
-        t_bwd += dev.event_elapsed_ms(9, 10)
-    t_plain, t_fwd, t_bwd = t_plain / reps, t_fwd / reps, t_bwd / reps
+        tp.append(dev.event_elapsed_ms(6, 7))
+        tf.append(dev.event_elapsed_ms(7, 8))
+        tb.append(dev.event_elapsed_ms(9, 10))
+    t_plain, t_fwd, t_bwd = statistics.median(tp), statistics.median(tf), statistics.median(tb)
     log(f"adjoint: {rays} rays; plain trace {t_plain:.3f} ms, differentiable forward {t_fwd:.3f} ms, "
         f"backward {t_bwd:.3f} ms -> adjoint:forward {t_bwd / t_plain:.2f}, gradient (forward + backward) "
         f"{(t_fwd + t_bwd) / t_plain:.2f}")
@@ -527,7 +531,9 @@ def bench_adjoint(dev, steps: int = 64) -> dict:
             "plain_forward_ms": t_plain, "differentiable_forward_ms": t_fwd, "backward_ms": t_bwd,
             "adjoint_to_forward": t_bwd / t_plain, "adjoint_plus_forward_to_forward": (t_bwd + t_fwd) / t_plain,
             "cost_note": "adjoint_to_forward = backward / plain forward trace of the same rays; one gradient "
-                         "evaluation costs the differentiable forward plus the backward",
+                         "evaluation costs the differentiable forward plus the backward; each time is the "
+                         f"median of {reps} repetitions (HIP events; the backward's includes its host part)",
+            "backward_ms_max": max(tb),
             "crn_finite_differences": fd,
             "optimisation_steps": steps, "loss_first": losses[0], "loss_last": losses[-1],
             "optimisation_s": opt_s}

@@ -187,19 +187,6 @@ __global__ __launch_bounds__(256) void k_batch_cull(BatchArgs a) {
 // over them); 1: facet tiles (slot_ray's tiled order); 2: facet-shared (L = 2^a.lLs rays per facet,
 // 2 <= L <= 64): each wave takes blocks of 64 micro-facets, lane k computes facet k's origin and group
 // offsets once, and the block's L steps of 64 rays read them from the computing lanes (ds_bpermute).
-// A frame's nine components copied into vector registers (an empty asm with a "v" operand).
-__device__ __forceinline__ void vgpr_frame(DevFrame &f) {
-    asm("" : "+v"(f.ux));
-    asm("" : "+v"(f.uy));
-    asm("" : "+v"(f.uz));
-    asm("" : "+v"(f.nx));
-    asm("" : "+v"(f.ny));
-    asm("" : "+v"(f.nz));
-    asm("" : "+v"(f.vx));
-    asm("" : "+v"(f.vy));
-    asm("" : "+v"(f.vz));
-}
-
 template <int RT, bool LDSMAP, int ORDER, int BS>
 __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchArgs a) {
     extern __shared__ float map_lds[];
@@ -209,7 +196,7 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
     RayStats st{0u, 0u, 0u};
     // Constants every ray reads — the receiver's geometry, the instance's sun frame, the heliostat's
     // slope-error frame, energy factor and (per-ray orders) facet geometry — are kept in vector registers
-    // (an empty asm with a "v" operand): in scalar registers they overflow the 106 available, and the
+    // (in_vgpr): in scalar registers they overflow the 106 available, and the
     // compiler spills the rest to lanes of vector registers and reloads kernel arguments inside the ray
     // loop. 1024-lane blocks (at most 64 vector registers) keep them scalar. Batched sweeps: config-1
     // field 64.1k -> 71.9k scenes/s, PS10 sampled 40.9k -> 45.0k.
@@ -217,16 +204,16 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
     if (BS < 1024) {
         for (int k = 0; k < 3; ++k) {
             if (RT == 0) {
-                asm("" : "+v"(recv.A[k]));
-                asm("" : "+v"(recv.E1[k]));
-                asm("" : "+v"(recv.E2[k]));
+                in_vgpr(recv.A[k]);
+                in_vgpr(recv.E1[k]);
+                in_vgpr(recv.E2[k]);
             } else {
-                asm("" : "+v"(recv.pos[k]));
+                in_vgpr(recv.pos[k]);
             }
         }
         if (RT == 1) {
-            asm("" : "+v"(recv.radius));
-            asm("" : "+v"(recv.height));
+            in_vgpr(recv.radius);
+            in_vgpr(recv.height);
         }
     }
 
@@ -252,18 +239,18 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
             DevHelio H = ld_uniform(a.helios + a.helio_begin + k);
             if (BS < 1024 && ORDER != 2) {  // the per-ray facet origin's geometry
                 for (int c = 0; c < 3; ++c) {
-                    asm("" : "+v"(H.pos[c]));
-                    asm("" : "+v"(H.size[c]));
+                    in_vgpr(H.pos[c]);
+                    in_vgpr(H.size[c]);
                 }
-                asm("" : "+v"(H.gap[0]));
-                asm("" : "+v"(H.gap[1]));
-                asm("" : "+v"(H.pl));
+                in_vgpr(H.gap[0]);
+                in_vgpr(H.gap[1]);
+                in_vgpr(H.pl);
             }
             DevFrame fr = ld_uniform(a.fr + (size_t)ii * a.n_helio + k);
             float factor = ld_uniform(a.fac + (size_t)ii * a.n_helio + k);
             if (BS < 1024) {
                 vgpr_frame(fr);
-                asm("" : "+v"(factor));
+                in_vgpr(factor);
             }
             const int32_t *grp = a.offsets + H.offsets_base;
             const int own_lo = H.sub_begin, own_hi = H.sub_begin + H.sub_count;

@@ -434,7 +434,7 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, BandGeom bg, int
         constexpr bool focus = HT == kHtFocusShared;
         // Launch constants that every step reads — the band's geometry, the map width, the accumulator,
         // the receiver's edges or axis — and (below) a flat heliostat's slope-error frame are kept in
-        // vector registers (an empty asm with a "v" operand): the loop's scalar registers run out
+        // vector registers (in_vgpr): the loop's scalar registers run out
         // otherwise, and the compiler then reloads them from the kernel arguments every step, each reload
         // a scalar load the step waits for, and spills others to lanes of vector registers; the vector
         // registers have room (the LDS band holds occupancy at 4 waves per SIMD). PS10-like field:
@@ -443,19 +443,19 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, BandGeom bg, int
         int vW = W;
         double *vacc = a.acc;
         TraceArgs va = a;
-        asm("" : "+v"(vbg.row0));
-        asm("" : "+v"(vbg.rows));
-        asm("" : "+v"(vbg.cols));
-        asm("" : "+v"(vW));
-        asm("" : "+v"(vacc));
+        in_vgpr(vbg.row0);
+        in_vgpr(vbg.rows);
+        in_vgpr(vbg.cols);
+        in_vgpr(vW);
+        in_vgpr(vacc);
         for (int k = 0; k < 3; ++k) {
-            asm("" : "+v"(va.recv.E1[k]));
-            asm("" : "+v"(va.recv.E2[k]));
+            in_vgpr(va.recv.E1[k]);
+            in_vgpr(va.recv.E2[k]);
         }
         if (RT == 1) {  // the cylinder's axis, radius and height (config-3 field: 3.08 -> 3.05 ms per call)
-            for (int k = 0; k < 3; ++k) asm("" : "+v"(va.recv.pos[k]));
-            asm("" : "+v"(va.recv.radius));
-            asm("" : "+v"(va.recv.height));
+            for (int k = 0; k < 3; ++k) in_vgpr(va.recv.pos[k]);
+            in_vgpr(va.recv.radius);
+            in_vgpr(va.recv.height);
         }
         const int L = 1 << lLs, fpi = 64 >> lLs;  // rays per facet, facets per 64-slot step
         const int lane = (int)(threadIdx.x & 63u);
@@ -484,17 +484,7 @@ __global__ __launch_bounds__(BS) void k_trace_band(TraceArgs a, BandGeom bg, int
             }
             const int nf = min(64, H.F - f0);  // facets of this block
             DevFrame hfr = H.fr;
-            if (!focus) {
-                asm("" : "+v"(hfr.ux));
-                asm("" : "+v"(hfr.uy));
-                asm("" : "+v"(hfr.uz));
-                asm("" : "+v"(hfr.nx));
-                asm("" : "+v"(hfr.ny));
-                asm("" : "+v"(hfr.nz));
-                asm("" : "+v"(hfr.vx));
-                asm("" : "+v"(hfr.vy));
-                asm("" : "+v"(hfr.vz));
-            }
+            if (!focus) vgpr_frame(hfr);
             // the pool samples of step i's rays; every lane's group offsets are in [0, P] (0 past the last
             // facet), so its pool indices are in range
             auto samples = [&](int i, float4 &s, float4 &q) {

@@ -25,6 +25,20 @@ struct RayStats {
     unsigned cells, candidates, tests;
 };
 
+// A value kept in a vector register from here on (an empty asm with a "v" operand): for wave-uniform
+// constants that a loop reads every iteration, when the loop's scalar registers would otherwise run
+// out (the compiler then reloads kernel arguments inside the loop and spills to vector-register lanes).
+template <class T>
+__device__ __forceinline__ void in_vgpr(T &x) {
+    asm("" : "+v"(x));
+}
+// A frame's nine components, likewise.
+__device__ __forceinline__ void vgpr_frame(DevFrame &f) {
+    in_vgpr(f.ux), in_vgpr(f.uy), in_vgpr(f.uz);
+    in_vgpr(f.nx), in_vgpr(f.ny), in_vgpr(f.nz);
+    in_vgpr(f.vx), in_vgpr(f.vy), in_vgpr(f.vz);
+}
+
 // Walk of the 3D-DDA over the uniform grid: cells in the reference's order (rectGridDDA.cu:31-119).
 // cell(addr, ix, iy, iz, sx, sy, sz, hit_id, st) is called for every visited cell (address in x-major
 // order, indices, step signs): it returns 1 at a hit (hit_id set), -1 when no later cell can hold one
