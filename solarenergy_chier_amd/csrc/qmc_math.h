@@ -26,6 +26,11 @@
 namespace qmc {
 
 constexpr double kEps = 1e-5;               // Util/global_constant.h:11 (a double)
+// Epsilon as a float: the float just below it. Epsilon is not a float and no float lies in (kEpsF,
+// Epsilon], so for a float x: x < Epsilon and x <= Epsilon exactly when x <= kEpsF, x < -Epsilon exactly
+// when x < -kEpsF — float compares that decide the reference's double ones.
+constexpr float kEpsF = (float)kEps;
+static_assert((double)kEpsF < kEps && (double)kEpsF + 0x1p-40 > kEps, "kEpsF is the float below Epsilon");
 constexpr float kMathPi = 3.141592654f;      // Util/global_constant.h:12
 constexpr float kIntMaxF = 2147483648.0f;    // float(INT_MAX), rectGridDDA.cuh:17-20
 
@@ -185,7 +190,7 @@ QMC_HD float ray_energy(float t, V3 dir, V3 normal, float factor) {
 QMC_HD bool ray_quad(V3 o, V3 d, V3 A, V3 E1, V3 E2, float &t, float &u, float &v) {
     V3 p = cross(d, E2);
     float det = dot(E1, p);
-    if (fabsf(det) < kEps) return false;
+    if (fabsf(det) <= kEpsF) return false;  // |det| < Epsilon
     V3 T = o - A;
     u = dot(T, p) / det;
     if (u < 0 || u > 1) return false;
@@ -193,7 +198,7 @@ QMC_HD bool ray_quad(V3 o, V3 d, V3 A, V3 E1, V3 E2, float &t, float &u, float &
     v = dot(d, q) / det;
     if (v < 0 || v > 1) return false;
     t = dot(E2, q) / det;
-    if (t < kEps) return false;
+    if (t <= kEpsF) return false;  // t < Epsilon
     return true;
 }
 
@@ -214,13 +219,13 @@ QMC_HD QuadOriginTerms quad_origin_terms(V3 o, V3 A, V3 E1, V3 E2) {
 QMC_HD bool ray_quad_pre(V3 d, V3 E1, V3 E2, const QuadOriginTerms &ot, float &t, float &u, float &v) {
     V3 p = cross(d, E2);
     float det = dot(E1, p);
-    if (fabsf(det) < kEps) return false;
+    if (fabsf(det) <= kEpsF) return false;  // |det| < Epsilon
     u = dot(ot.T, p) / det;
     if (u < 0 || u > 1) return false;
     v = dot(d, ot.q) / det;
     if (v < 0 || v > 1) return false;
     t = ot.tq / det;
-    if (t < kEps) return false;
+    if (t <= kEpsF) return false;  // t < Epsilon
     return true;
 }
 
@@ -341,7 +346,7 @@ QMC_HD double table_inverse_cdf(const double *theta, const double *cdf, const do
 }
 
 QMC_HD float abs_divide(float den, float num) {
-    if (num <= kEps && num >= -kEps) return kIntMaxF;
+    if (num <= kEpsF && num >= -kEpsF) return kIntMaxF;  // |num| <= Epsilon
     return fabsf(den / num);
 }
 QMC_HD float cal_tmax(float dir, float interval, int idx, float cur) {
@@ -386,9 +391,6 @@ struct ProofBox {
     float mid[3], lim[3];
 };
 
-// Epsilon as a float: for a float x, x <= kEpsF exactly when x <= Epsilon (a double).
-constexpr float kEpsF = (float)kEps;
-static_assert((double)kEpsF <= kEps && (double)kEpsF + 0x1p-40 > kEps, "kEpsF is the float below Epsilon");
 
 // x rounded to a float not above it.
 inline float round_down(double x) {

@@ -454,7 +454,7 @@ __device__ __forceinline__ bool receiver_hit(const DevReceiver &R, V3 o, V3 d, f
         V2 Ro{R.pos[0] - o.x, R.pos[2] - o.z};
         float tp = dot2(Ro, normalize2(V2{d.x, d.z}));
         float delta = dot2(Ro, Ro) - tp * tp;
-        if (tp < -kEps || delta > R2) return false;
+        if (tp < -kEpsF || delta > R2) return false;  // tp < -Epsilon
         float t_plus = delta <= 0.0f ? R.radius : sqrtf(R2 - delta);
         t = (tp - t_plus) / length2(V2{d.x, d.z});
         V3 ip = t * d + o;
