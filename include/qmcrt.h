@@ -364,6 +364,15 @@ qmc_status qmc_probe_atomic_rates(qmc_ctx *ctx, qmc_atomic_rates *rates);
  * `mismatches[2]` those where sqrtf or 1.0f / sqrtf differs from the double-precision computation
  * rounded to float. No counterpart in the reference. Synchronous. */
 qmc_status qmc_check_fast_sqrt(qmc_ctx *ctx, uint64_t *checked, uint64_t mismatches[3]);
+/* Host check of the cylinder map's column lookup for a map `width` columns wide (1 to 32768): for the
+ * float cosines in [-1, 1] — every one with stride 1; with a larger stride every stride-th one and the
+ * 513 centred on each breakpoint, where the column changes — and both signs of the hit's z offset, the
+ * column from the breakpoint table and the acos estimate (the trace's path) against the column from
+ * the double arc cosine. `checked` receives the number of (cosine, sign) pairs, `mismatches` those
+ * that differ, `max_acos_error` the estimate's largest error in radians. Runs on host threads, needs
+ * no GPU; no counterpart in the reference. */
+qmc_status qmc_check_cylinder_columns(int32_t width, int32_t stride, uint64_t *checked, uint64_t *mismatches,
+                                      double *max_acos_error);
 
 /* Traversal-culling flags the trace uses for the listed heliostats under the current pools and
  * scene: bit 0 set = the shadow traversal can meet another quad, bit 1 set = the blocking traversal

@@ -178,6 +178,7 @@ SIGNATURES = {
     "qmc_cull_flags": [_P, _IP, _I, C.POINTER(C.c_uint8)],
     "qmc_probe_atomic_rates": [_P, C.POINTER(AtomicRates)],
     "qmc_check_fast_sqrt": [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+    "qmc_check_cylinder_columns": [_I, _I, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_double)],
     "qmc_trace_records": [_P, _I, C.POINTER(RayRecord), C.c_int64],
     "qmc_trace_records_many": [_P, _IP, _I, _I, C.POINTER(RayRecord), C.c_int64],
     "qmc_trace_stats_run": [_P, _IP, _I, C.POINTER(TraceStats)],

@@ -293,10 +293,11 @@ __global__ __launch_bounds__(BS, BS == 1024 ? 8 : 1) void k_trace_batched(BatchA
                 dir = normalize(reflect(-dir, normal));
                 if (hits_quad(1, origin, dir)) return;
                 float t, u, v;
-                if (!receiver_hit<RT>(recv, origin, dir, t, u, v)) return;
+                int col = -1;  // set by a cylinder's column lookup
+                if (!receiver_hit<RT>(recv, origin, dir, t, u, v, nullptr, &col)) return;
                 const float energy = ray_energy(t, dir, normal, factor);
                 int row = (int)(u * (float)a.recv.res_y);
-                int col = (int)(v * (float)a.recv.res_x);
+                if (col < 0) col = (int)(v * (float)a.recv.res_x);
                 if (row >= a.recv.res_y) row = a.recv.res_y - 1;
                 if (col >= a.recv.res_x) col = a.recv.res_x - 1;
                 const int addr = row * a.recv.res_x + col;

@@ -39,6 +39,7 @@ struct TraceSpan {
 struct RecvBuf {
     DevReceiver r{};
     float *flux = nullptr;
+    float *colt = nullptr;  // cylinder: the column breakpoints r.colt points to
     double *acc = nullptr;  // a launch's deposits in double, folded into flux after it (zero between launches)
     int w = 0, h = 0;
 };
@@ -225,6 +226,7 @@ inline void free_scene(qmc_ctx *c) {
     for (auto &r : c->recvs) {
         dev_free(r.flux);
         dev_free(r.acc);
+        dev_free(r.colt);
     }
     c->recvs.clear();
     dev_free(c->d_helios);

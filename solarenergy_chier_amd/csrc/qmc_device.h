@@ -63,6 +63,9 @@ struct DevReceiver {
     // cylinder
     float pos[3];
     float radius, height;
+    // cylinder: the map's column breakpoints (cylinder_column_table, 2 res_x + 4 floats; null: the
+    // column from the double arc cosine)
+    const float *colt;
 };
 
 struct DevGrid {

@@ -142,13 +142,14 @@ __device__ __forceinline__ void trace_from(const TraceArgs &a, const DevHelio &H
     o.unblocked = true;
     // Step 3: receiver hit (Receiver/*ReceiverIntersection.cu)
     float t, u, v;
-    if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v, ot)) {
+    int col = -1;  // set by a cylinder's column lookup
+    if (!receiver_hit<RT>(a.recv, origin, dir, t, u, v, ot, &col)) {
         o.status = QMC_RAY_MISSED;
         return;
     }
     o.energy = ray_energy(t, dir, normal, H.factor);
     int row = (int)(u * (float)a.recv.res_y);
-    int col = (int)(v * (float)a.recv.res_x);
+    if (col < 0) col = (int)(v * (float)a.recv.res_x);
     if (row >= a.recv.res_y) { row = a.recv.res_y - 1; o.flags |= QMC_RAYF_EDGE_CLAMPED; }
     if (col >= a.recv.res_x) { col = a.recv.res_x - 1; o.flags |= QMC_RAYF_EDGE_CLAMPED; }
     o.row = row;
@@ -1169,6 +1170,13 @@ extern "C" qmc_status qmc_upload(qmc_ctx *c, const qmc_scene *s) {
         HIP_TRY(hipMemsetAsync(rb.flux, 0, sizeof(float) * (size_t)rb.w * rb.h, c->stream));
         if ((st = dev_alloc(rb.acc, (size_t)rb.w * rb.h))) return st;
         HIP_TRY(hipMemsetAsync(rb.acc, 0, sizeof(double) * (size_t)rb.w * rb.h, c->stream));
+        if (d.type == 1 && d.res_x >= 1 && d.res_x <= kColumnTableMaxW) {
+            std::vector<float> T(2 * (size_t)d.res_x + 4);
+            cylinder_column_table(d.res_x, T.data());
+            if ((st = dev_alloc(rb.colt, T.size()))) return st;
+            HIP_TRY(hipMemcpy(rb.colt, T.data(), sizeof(float) * T.size(), hipMemcpyHostToDevice));
+            d.colt = rb.colt;
+        }
     }
     // grids
     c->grids.resize(s->grids.size());

@@ -7,8 +7,9 @@
 //   divide (the reference's host rsqrtf, Util/vector_arithmetic.cuh:50-53,397-401),
 //   reflect(i,n) = i - (2*n)*dot(n,i) (Util/vector_arithmetic.cuh:410-413),
 //   float->int conversion truncates toward zero (Util/vector_arithmetic.cuh:571-574).
-// Epsilon is the double 1e-5 (Util/global_constant.h:11): every comparison against it is
-// done in double, as C++ promotion does in the reference.
+// Epsilon is the double 1e-5 (Util/global_constant.h:11): every comparison against it decides as
+// in double, as C++ promotion does in the reference (in double, or as the equal float compare
+// against kEpsF).
 //
 // All code including this header must be compiled with -ffp-contract=off and without
 // fast-math, so that no multiply-add is fused and host and device produce the same bits.
@@ -32,6 +33,7 @@ constexpr double kEps = 1e-5;               // Util/global_constant.h:11 (a doub
 constexpr float kEpsF = (float)kEps;
 static_assert((double)kEpsF < kEps && (double)kEpsF + 0x1p-40 > kEps, "kEpsF is the float below Epsilon");
 constexpr float kMathPi = 3.141592654f;      // Util/global_constant.h:12
+constexpr double kInv2Pi = 1.0 / (2 * M_PI);  // the double nearest 1 / (the double nearest 2 pi)
 constexpr float kIntMaxF = 2147483648.0f;    // float(INT_MAX), rectGridDDA.cuh:17-20
 
 struct V3 {
@@ -173,6 +175,77 @@ QMC_HD float atan_r(float x) { return (float)atan((double)x); }
 QMC_HD float pow_r(float x, float y) { return (float)pow((double)x, (double)y); }
 QMC_HD float sin_r(float x) { return (float)sin((double)x); }
 QMC_HD float cos_r(float x) { return (float)cos((double)x); }
+
+// Cylinder map columns. A hit whose direction from the axis has cosine cx (clamped to [-1, 1]) and a
+// negative (neg) or non-negative z component lies in column (int)(v * W) of a W-column map, v =
+// float(double(acos_r(cx)) / 2 pi), or 1 - v when neg (CylinderReceiver.cuh:82-86 and the pixel index).
+QMC_HD int cylinder_column_exact(float cx, bool neg, int W) {
+    float v = (float)((double)acos_r(cx) * kInv2Pi);  // equals the division by 2 pi (exhaustively)
+    if (neg) v = 1 - v;
+    return (int)(v * (float)W);
+}
+
+// For each sign that column is a monotone step function of the float cx (non-increasing without neg,
+// non-decreasing with it), so a receiver keeps its breakpoints, T[0 .. 2W + 3] (cylinder_column_table,
+// built on the host with cylinder_column_exact): T[k] (k <= W + 1) = the smallest float cx in [-1, 1]
+// whose column is below k, T[W + 2 + k] = the smallest whose column is at least k with neg; 2 where
+// there is none. A hit takes an estimate from Abramowitz and Stegun 4.4.45 (acos x = sqrt(1 - x)
+// (a0 + a1 x + a2 x^2 + a3 x^3) on [0, 1], error at most 6.7e-5 rad: under 0.35 of a column for W <=
+// kColumnTableMaxW, so within one of the exact column) and corrects it with two breakpoints. Equal to
+// cylinder_column_exact for every float cx and both signs (checked exhaustively:
+// qmc_check_cylinder_columns).
+constexpr int kColumnTableMaxW = 32768;
+QMC_HD int cylinder_column(const float *T, int W, float cx, bool neg) {
+    const float ax = fabsf(cx);
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = __builtin_amdgcn_sqrtf(1.0f - ax);
+#else
+    const float s = sqrtf(1.0f - ax);
+#endif
+    float ac = s * (1.5707288f + ax * (-0.2121144f + ax * (0.0742610f - 0.0187293f * ax)));
+    if (cx < 0.0f) ac = 3.14159265f - ac;
+    float va = ac * 0.159154943f;
+    if (neg) va = 1.0f - va;
+    int k = (int)(va * (float)W);
+    k = k < 0 ? 0 : k > W ? W : k;
+    const float *B = T + (neg ? W + 2 : 0);
+    const float b0 = B[k], b1 = B[k + 1];
+    return k - 1 + (neg ? (int)(cx >= b0) + (int)(cx >= b1) : (int)(cx < b0) + (int)(cx < b1));
+}
+
+// The breakpoint table of a W-column cylinder map (2 W + 4 floats), by binary search over the floats of
+// [-1, 1] in order (host).
+inline int64_t float_order(float x) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, x);
+    return (b & 0x80000000u) ? -(int64_t)(b & 0x7fffffffu) : (int64_t)b;
+}
+inline float order_float(int64_t k) {
+    return __builtin_bit_cast(float, k >= 0 ? (uint32_t)k : (0x80000000u | (uint32_t)(-k)));
+}
+inline void cylinder_column_table(int W, float *T) {
+    for (int neg = 0; neg < 2; ++neg) {
+        for (int k = 0; k <= W + 1; ++k) {
+            // the first cx at which the predicate holds: column below k (no neg) or at least k (neg)
+            auto holds = [&](float cx) {
+                const int c = cylinder_column_exact(cx, neg != 0, W);
+                return neg ? c >= k : c < k;
+            };
+            float b;
+            if (!holds(1.0f)) b = 2.0f;
+            else if (holds(-1.0f)) b = -1.0f;
+            else {
+                int64_t lo = float_order(-1.0f), hi = float_order(1.0f);  // holds at hi, not at lo
+                while (hi - lo > 1) {
+                    const int64_t mid = lo + (hi - lo) / 2;
+                    if (holds(order_float(mid))) hi = mid;
+                    else lo = mid;
+                }
+                b = order_float(hi);
+            }
+            T[neg * (W + 2) + k] = b;
+        }
+    }
+}
 
 // Atmospheric attenuation and per-ray energy (QuasiMonteCarloRayTracing/Receiver/
 // receiverIntersectionUtil.cuh:17-21,31-34).

@@ -3,10 +3,15 @@
 //    are reported against these ceilings: float and 64-bit device-scope atomic adds scattered over a
 //    map the size of the PS10-like field's (256 x 256), and float LDS atomic adds scattered over a
 //    64 KB tile;
-//  * the exhaustive check of the short square-root sequences of qmc_math.h against the general ones.
+//  * the exhaustive check of the short square-root sequences of qmc_math.h against the general ones;
+//  * the exhaustive host check of the cylinder map's column breakpoints (qmc_math.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "qmc_ctx.h"
 
@@ -133,5 +138,56 @@ extern "C" qmc_status qmc_probe_atomic_rates(qmc_ctx *c, qmc_atomic_rates *out) 
     out->global_f32 = rates[0];
     out->global_f64 = rates[1];
     out->lds_f32 = rates[2];
+    return QMC_OK;
+}
+
+extern "C" qmc_status qmc_check_cylinder_columns(int32_t width, int32_t stride, uint64_t *checked,
+                                                  uint64_t *mismatches, double *max_acos_error) {
+    if (!checked || !mismatches || !max_acos_error)
+        return fail(QMC_E_INVALID_ARG, "qmc_check_cylinder_columns: null argument");
+    if (width < 1 || width > kColumnTableMaxW)
+        return fail(QMC_E_RANGE, "qmc_check_cylinder_columns: width outside [1, " +
+                                     std::to_string(kColumnTableMaxW) + "]");
+    if (stride < 1) return fail(QMC_E_INVALID_ARG, "qmc_check_cylinder_columns: stride below 1");
+    const int W = width;
+    std::vector<float> T(2 * (size_t)W + 4);
+    cylinder_column_table(W, T.data());
+    // the floats of [-1, 1] (one zero) at the stride and, with a stride, the 513 floats centred on each
+    // breakpoint (where the column changes); split over host threads
+    const int64_t k0 = float_order(-1.0f), k1 = float_order(1.0f);
+    std::vector<int64_t> near;
+    if (stride > 1)
+        for (float b : T)
+            if (b >= -1.0f && b <= 1.0f)
+                for (int64_t d = -256; d <= 256; ++d) {
+                    const int64_t k = float_order(b) + d;
+                    if (k >= k0 && k <= k1) near.push_back(k);
+                }
+    const int64_t n_grid = (k1 - k0) / stride + 1, n_all = n_grid + (int64_t)near.size();
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<uint64_t> bad(nt, 0);
+    std::vector<double> err(nt, 0.0);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (int64_t i = n_all * t / nt; i < n_all * (t + 1) / nt; ++i) {
+                const float cx = order_float(i < n_grid ? k0 + i * stride : near[(size_t)(i - n_grid)]);
+                for (int neg = 0; neg < 2; ++neg)
+                    bad[t] += cylinder_column(T.data(), W, cx, neg != 0) != cylinder_column_exact(cx, neg != 0, W);
+                // the estimate's own error (formed as on the device, with a correctly rounded sqrt here)
+                const float ax = std::fabs(cx);
+                float ac = std::sqrt(1.0f - ax) * (1.5707288f + ax * (-0.2121144f + ax * (0.0742610f - 0.0187293f * ax)));
+                if (cx < 0.0f) ac = 3.14159265f - ac;
+                err[t] = std::max(err[t], std::fabs((double)ac - std::acos((double)cx)));
+            }
+        });
+    for (auto &x : th) x.join();
+    *checked = (uint64_t)(2 * n_all);
+    *mismatches = 0;
+    *max_acos_error = 0.0;
+    for (int t = 0; t < nt; ++t) {
+        *mismatches += bad[t];
+        *max_acos_error = std::max(*max_acos_error, err[t]);
+    }
     return QMC_OK;
 }

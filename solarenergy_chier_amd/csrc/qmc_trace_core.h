@@ -437,10 +437,12 @@ __device__ __forceinline__ void facet_geometry(const DevHelio &H, const DevFacet
 
 // Receiver hit: true with (t, u, v) when the ray reaches the receiver surface. For a rectangle
 // receiver, ot (nullable) holds the ray origin's terms of the test (quad_origin_terms with the
-// receiver's A, E1, E2), computed once for the rays of a micro-facet.
+// receiver's A, E1, E2), computed once for the rays of a micro-facet. For a cylinder with column
+// breakpoints and col non-null, *col receives the hit's map column, (int)(v * res_x) without the
+// clamp, in place of v.
 template <int RT>
 __device__ __forceinline__ bool receiver_hit(const DevReceiver &R, V3 o, V3 d, float &t, float &u, float &v,
-                                             const QuadOriginTerms *ot = nullptr) {
+                                             const QuadOriginTerms *ot = nullptr, int *col = nullptr) {
     if (RT == 0) {
         // RectangleReceiver::GIntersect (RectangleReceiver.cuh:32-34)
         if (ot) return ray_quad_pre(d, V3{R.E1[0], R.E1[1], R.E1[2]}, V3{R.E2[0], R.E2[1], R.E2[2]}, *ot, t, u, v);
@@ -462,7 +464,15 @@ __device__ __forceinline__ bool receiver_hit(const DevReceiver &R, V3 o, V3 d, f
         if (u < 0.0f || u > 1.0f) return false;
         V2 c = normalize2(V2{ip.x - R.pos[0], ip.z - R.pos[2]});
         float cx = fminf(fmaxf(c.x, -1.0f), 1.0f);
-        v = (float)((double)acos_r(cx) / (2 * M_PI));
+        if (col && R.colt) {
+            // the map column directly, from the receiver's breakpoints (cylinder_column); v is not read
+            *col = cylinder_column(R.colt, R.res_x, cx, c.y < 0);
+            v = 0.0f;
+            return true;
+        }
+        // float(double(a) / (2 pi)) computed as float(double(a) * (1 / (2 pi))): equal for every float a
+        // in [0, float(pi)], the range of acos_r (checked exhaustively, tests/test_float_compares.py)
+        v = (float)((double)acos_r(cx) * kInv2Pi);
         if (c.y < 0) v = 1 - v;
         return true;
     }

@@ -107,6 +107,16 @@ def _as_c(spec):
     return spec if isinstance(spec, C.Structure) else spec.to_c()
 
 
+def check_cylinder_columns(width: int, stride: int = 1) -> tuple[int, int, float]:
+    """Host check of the trace's cylinder column lookup for a `width`-column map over the float
+    cosines in [-1, 1] (every one with stride 1; else every stride-th one and those around each
+    breakpoint) and both signs: (pairs checked, mismatches against the double arc cosine's column, the
+    acos estimate's largest error in radians). Needs no GPU."""
+    n, bad, err = C.c_uint64(0), C.c_uint64(0), C.c_double(0.0)
+    N.call("qmc_check_cylinder_columns", int(width), int(stride), C.byref(n), C.byref(bad), C.byref(err))
+    return int(n.value), int(bad.value), float(err.value)
+
+
 def default_config() -> N.Config:
     c = N.Config()
     N.call("qmc_config_default", C.byref(c))
