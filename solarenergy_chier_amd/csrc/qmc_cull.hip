@@ -207,16 +207,23 @@ __device__ void wave_shape_bits(const WaveArgs &a, const StagedQuad (*sq)[kCullW
     }
 }
 
-// One lane per wave of slots (grid.y = list entry): the candidate masks of that wave (WaveArgs). The
-// wave's rays start at the centres of a run of micro-facets — a tile of the facet grid (flat
-// heliostats with tiled ray order, as slot_ray) or up to 64 consecutive facets in row-major order
-// (one row, or the end of one row and the start of the next) — and every such centre lies in the
-// parallelogram spanned by the extreme rows and columns of the run: on a flat heliostat the centre
-// grows with column and row over the whole heliostat (affine within a facet, a gap between facets;
-// RectangleHelio.cu:54-80), on a focus-flat one it is affine within each facet
-// (FocusFlatRectangleHelio.cu:21-39), whose own perturbation frame then gives the reflection axis
-// (no canting term). Origins are float-rounded; the 1 cm margin of the bound covers that. A wave whose
-// run needs more than 8 parallelograms keeps every candidate of its heliostat.
+// Waves of slots per lane of k_cull_waves: each lane bounds the rays of two adjacent waves together and
+// gives both the same mask. That halves the lanes (the kernel's cost) for slightly looser masks, which
+// leave the trace kernel's time unchanged on a flat field; with four per lane focus-flat runs cross too
+// many facets, whose waves then keep every candidate.
+constexpr int kWavesPerLane = 2;
+static_assert(kWavesPerLane == 1 || kWavesPerLane == 2 || kWavesPerLane == 4, "one, two or four waves per lane");
+
+// One lane per kWavesPerLane waves of slots (grid.y = list entry): the candidate masks of those waves
+// (WaveArgs), one mask for all of them. Their rays start at the centres of micro-facets: one tile of the
+// facet grid per wave (flat heliostats with tiled ray order, as slot_ray), or the lane's run of
+// consecutive facets in row-major order (one row, or the end of one row and the start of the next).
+// Every centre of a tile or row piece lies in the parallelogram spanned by its extreme rows and
+// columns: on a flat heliostat the centre grows with column and row over the whole heliostat (affine
+// within a facet, a gap between facets; RectangleHelio.cu:54-80), on a focus-flat one it is affine
+// within each facet (FocusFlatRectangleHelio.cu:21-39), whose own perturbation frame then gives the
+// reflection axis (no canting term). Origins are float-rounded; the 1 cm margin of the bound covers
+// that. A lane whose run needs more than 8 parallelograms keeps every candidate of its heliostat.
 __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
     static_assert(kCullWide == 64, "one 64-bit mask per wave and traversal");
     __shared__ StagedQuad sq[2][kCullWide];
@@ -243,7 +250,7 @@ __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
         }
     }
     __syncthreads();
-    if (w >= a.wstride) return;
+    if (w * kWavesPerLane >= a.wstride) return;
     uint64_t m[2] = {0ull, 0ull};
     const bool any = (n[0] > 0 && n[0] <= kCullWide) || (n[1] > 0 && n[1] <= kCullWide);
     const DevHelio &H = a.helios[h];
@@ -251,26 +258,29 @@ __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
         const int R = H.rows * H.sub_rows, C = H.cols * H.sub_cols;
         // slot order as slot_ray: flat heliostats in facet tiles, focus-flat ones in ray order
         const FacetTiling tl = facet_tiling(R, C, a.L);
-        int nr = 0, r0[2], r1[2], c0[2], c1[2];
+        constexpr int kRects = kWavesPerLane > 2 ? kWavesPerLane : 2;
+        int nr = 0, r0[kRects], r1[kRects], c0[kRects], c1[kRects];
         bool full = false;
-        const long long s0 = 64LL * w;
+        const long long s0 = 64LL * kWavesPerLane * w;
         if (H.type == 0 && tl.tiled) {
-            if (s0 < tl.slots) {
-                const int ty = (int)fast_div((uint32_t)w, tile_divisor(H, tl)), tx = w - ty * tl.ntx;
+            for (int j = 0; j < kWavesPerLane; ++j) {
+                const int wj = w * kWavesPerLane + j;
+                if (64LL * wj >= tl.slots) break;
+                const int ty = (int)fast_div((uint32_t)wj, tile_divisor(H, tl)), tx = wj - ty * tl.ntx;
                 const int ra = ty << tl.lth, ca = tx << tl.ltw;
                 if (ra < R && ca < C) {
-                    r0[0] = ra;
-                    r1[0] = min(R, ra + (1 << tl.lth)) - 1;
-                    c0[0] = ca;
-                    c1[0] = min(C, ca + (1 << tl.ltw)) - 1;
-                    nr = 1;
+                    r0[nr] = ra;
+                    r1[nr] = min(R, ra + (1 << tl.lth)) - 1;
+                    c0[nr] = ca;
+                    c1[nr] = min(C, ca + (1 << tl.ltw)) - 1;
+                    ++nr;
                 }
             }
         } else {
             const long long nrays = (long long)H.F * a.L;
             if (s0 < nrays) {
                 // slot counts stay below 2^31 (checked at launch): multiply-high quotients are exact
-                const long long e = min(s0 + 63, nrays - 1);
+                const long long e = min(s0 + 64LL * kWavesPerLane - 1, nrays - 1);
                 const int m0 = (int)fast_div((uint32_t)s0, a.fdL), m1 = (int)fast_div((uint32_t)e, a.fdL);
                 const int ra = (int)fast_div((uint32_t)m0, H.fd_cpr), rb = (int)fast_div((uint32_t)m1, H.fd_cpr);
                 if (ra == rb) {
@@ -352,9 +362,13 @@ __global__ __launch_bounds__(256) void k_cull_waves(WaveArgs a) {
                 m[k] = (n[k] > 0 && n[k] <= kCullWide) ? (n[k] >= 64 ? ~0ull : (1ull << n[k]) - 1ull) : 0ull;
         }
     }
-    uint64_t *out = a.masks + 2 * ((size_t)li * a.wstride + w);
-    out[0] = m[0];
-    out[1] = m[1];
+    for (int j = 0; j < kWavesPerLane; ++j) {
+        const int wj = w * kWavesPerLane + j;
+        if (wj >= a.wstride) break;
+        uint64_t *out = a.masks + 2 * ((size_t)li * a.wstride + wj);
+        out[0] = m[0];
+        out[1] = m[1];
+    }
 }
 
 }  // namespace
@@ -490,7 +504,7 @@ qmc_status qmc::launch_cull_waves_into(qmc_ctx *c, int grid, const int32_t *d_li
     w.fdL = make_fastdiv((uint32_t)c->L);
     w.wstride = (int)wstride;
     w.masks = masks;
-    hipLaunchKernelGGL(k_cull_waves, dim3((unsigned)((wstride + 255) / 256), (unsigned)n), dim3(256), 0, c->stream, w);
+    hipLaunchKernelGGL(k_cull_waves, dim3((unsigned)((wstride + 256LL * kWavesPerLane - 1) / (256LL * kWavesPerLane)), (unsigned)n), dim3(256), 0, c->stream, w);
     HIP_TRY(hipGetLastError());
     a.wmask = masks;
     a.wstride = (int)wstride;

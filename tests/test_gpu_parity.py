@@ -233,10 +233,10 @@ def test_ps10_field_low_sun_shadowing(device, sun, L):
 
 
 def test_wave_masks_spanning_rows_and_facets(device):
-    """Waves whose rays span several micro-facet rows (flat heliostats, 3 rays per facet: no facet tiles,
-    about 21 facets per wave) and several canted facets (focus-flat, 1 ray per facet: 64 facets per
-    wave, up to the parallelogram limit and its keep-every-candidate fallback) still trace every ray
-    as the oracle does."""
+    """Wave masks whose rays span several micro-facet rows (flat heliostats, 3 rays per facet: no facet
+    tiles, about 43 facets per mask lane of two waves) and several canted facets (focus-flat, 1 ray per
+    facet: 128 facets per lane, up to the parallelogram limit and its keep-every-candidate fallback)
+    still trace every ray as the oracle does."""
     wl = workloads.ps10_field(L=3)
     scene = Scene.from_specs(wl.config, wl.receivers, wl.grids, wl.heliostats)
     assert parity_on_scene(device, scene, [0, 300, 623]) == 0
